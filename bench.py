@@ -1,0 +1,119 @@
+"""Headline benchmark: 3D viscoelastic (Zener L=3) FP32 time loop, 512^3 cells.
+
+Workload (BASELINE config "3D viscoelastic Zener FD 512^3"): 512^3 full grid
+incl. 20-cell CPML on every face, 0.1 mm cells, aluminium over steel with
+seeded ellipsoidal density inclusions, L=3 mechanisms fitted to Q=50 over
+2-8 MHz, 5 MHz circular piston on the top face, 32x32 surface receivers,
+dt at CFL 0.5.  A step = stress pass + velocity pass + source/receiver pass
+over the whole grid; inputs are resident in HBM before the timed region.
+
+Prints one JSON line.  value = Gcell-updates/s of the whole job (one GPU).
+roofline: the dominant kernel (stress pass) -- algorithmic bytes per launch /
+its average duration measured here with HIP events on the library's stream.
+cpu_baseline: the FP32 CPU oracle (oracle/, the reference's update order
+restated in C, OpenMP) on a bounded sample of the same grid.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def bytes_per_cell(g, L):
+    """Algorithmic bytes per cell and step: every state field read once and
+    written once, materials read once, psi only inside its axis's CPML band."""
+    n = g.shape
+    band = sum(48.0 * g.band_cells()[a] / n[a] for a in range(3))      # 6 psi x (r+w) x 4 B per axis
+    step = 4.0 * (2 * (3 + 6 + 6 * L) + 3) + band
+    stress = 4.0 * (3 + 6 + 6 * L + 3 + 6 + 6 * L) + band / 2          # v, sigma, r, rho/cp/cs in; sigma, r out
+    velocity = 4.0 * (6 + 3 + 1 + 3) + band / 2                         # sigma, v, rho in; v out
+    return step, stress, velocity
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU-oracle sample (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the shot/slab sharding "
+              "across GPUs is a design, not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from taichi_sim_amd import workloads
+    from taichi_sim_amd.backend import attenuation_from_config
+    from taichi_sim_amd.engine import VSESim
+    from taichi_sim_amd.sim_support.simulator3d import Simulator3D
+
+    total = args.warmup + args.steps
+    cfg = workloads.visco_3d_block(n=args.n, steps=total)
+    sim = Simulator3D(cfg)
+    att = attenuation_from_config(cfg)
+    g = VSESim.from_simulator(sim, variant="wgsl", att=att)
+    ncell = int(np.prod(g.shape))
+    print(f"# setup done: grid {g.shape}, {g.device_bytes() / 2**30:.1f} GiB on {g.device_name()}",
+          file=sys.stderr, flush=True)
+
+    if args.warmup:
+        g.run(args.warmup, check_every=args.warmup)
+    t0 = time.perf_counter()
+    dev_ms = g.run(args.steps, check_every=500, timed=True)      # synchronises before returning
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    gcell = ncell * args.steps / wall / 1e9
+    print(f"# timed {args.steps} steps: wall {wall:.3f} s, device {dev_ms / 1e3:.3f} s",
+          file=sys.stderr, flush=True)
+
+    # per-kernel durations (HIP events around each launch), after the timed loop
+    k_ms = g.profile(args.profile_steps)
+    b_step, b_stress, b_vel = bytes_per_cell(g, g.n_sls)
+    achieved = b_stress * ncell / (k_ms[0] * 1e-3) / 1e9
+    step_frac = gcell * 1e9 * b_step / (HBM_PEAK_GBS * 1e9)
+    g.close()
+
+    cpu = None
+    if args.cpu_steps > 0:
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        o = oracle.Oracle(sim, variant="wgsl", att=att, n_threads=threads)
+        t = time.perf_counter()
+        o.run(args.cpu_steps)
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": round(ncell * args.cpu_steps / cpu_s / 1e9, 6), "unit": "Gcell-upd/s",
+               "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same {args.n}^3 grid, FP32 C oracle "
+                         f"(reference update order, OpenMP), {cpu_s:.1f} s"}
+        del o
+
+    line = {
+        "metric": "Gcell-upd/s (3D viscoelastic Zener L=3 FP32 forward)",
+        "value": round(gcell, 4), "unit": "Gcell-upd/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"3D viscoelastic Zener FD {args.n}^3 (incl. 20-cell CPML), L=3 Q=50, "
+                               f"Al over steel with 24 seeded inclusions, piston source, 32x32 receivers",
+                   "grid": [args.n] * 3, "n_sls": 3, "fd_order": 2, "parallelism": "single GPU"},
+        "hbm_frac_step": round(step_frac, 4),
+        "bytes_per_cell_step": round(b_step, 2),
+        "kernel_ms": {"stress": round(k_ms[0], 4), "velocity": round(k_ms[1], 4), "sparse": round(k_ms[2], 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "stress3d", "bytes_per_cell": round(b_stress, 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

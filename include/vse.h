@@ -1,0 +1,156 @@
+/*
+ * vse.h -- C-ABI of libvse, the MI355X (gfx950) staggered-grid
+ * velocity-stress time loop (elastic and Zener viscoelastic, CPML, 2D/3D).
+ *
+ * The reference has no FFI: its drop-in point is the Python plugin hook
+ * Simulator.implementation() (sim_support/simulator.py:608-611, called at
+ * :337; Simulator3D.py:619-622, called at :356) overridden by each backend
+ * (sim_webgpu_viscoelastic.py:43-555, sim_webgpu_3D_viscoelastic.py:44-713,
+ * sim_taichi_elastic.py:28-351).  These entry points are what such a backend
+ * needs from a native library: each one replaces the backend step named in
+ * its comment.  Python binds them with ctypes (taichi_sim_amd/_lib.py).
+ *
+ * Conventions: plain pointers and sizes; host buffers are caller-owned and
+ * copied; device memory is owned by the handle; every call is synchronous
+ * with respect to host buffers; return 0 on success, a negative VSE_ERR_*
+ * code on failure (vse_last_error() gives the message).  One handle per
+ * thread.  Grids are C order with the last axis fastest (2D (nx, ny) with
+ * y = the ROI's z axis; 3D (nx, ny, nz)), exactly as the reference lays out
+ * its buffers.
+ */
+#ifndef VSE_H
+#define VSE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VSE_ABI_VERSION 1
+
+#define VSE_OK 0
+#define VSE_ERR_ARG (-1)        /* invalid argument / shape */
+#define VSE_ERR_HIP (-2)        /* HIP runtime error */
+#define VSE_ERR_UNSTABLE (-3)   /* |v|^2 reached +inf or NaN (StabilityError) */
+#define VSE_ERR_STATE (-4)      /* call order (e.g. run before set_material) */
+
+#define VSE_VARIANT_WGSL 0      /* shader_webgpu_(3D_)viscoelastic.wgsl semantics */
+#define VSE_VARIANT_TAICHI 1    /* sim_taichi_elastic.py semantics (2D elastic) */
+
+/* field ids for vse_read_field / vse_field_device_ptr */
+#define VSE_F_VX 0
+#define VSE_F_VY 1
+#define VSE_F_VZ 2
+#define VSE_F_SXX 3
+#define VSE_F_SYY 4
+#define VSE_F_SZZ 5
+#define VSE_F_SXY 6
+#define VSE_F_SXZ 7
+#define VSE_F_SYZ 8
+
+typedef struct vse_sim vse_sim;
+
+/* Simulation description.  Replaces the parameter buffers of the WGSL
+ * backends (sim_webgpu_viscoelastic.py:57-71: params_i32/params_f32,
+ * tau_attenuation, alpha_sum; sim_webgpu_3D_viscoelastic.py:62-76). */
+typedef struct vse_desc {
+    int ndim;               /* 2 or 3 */
+    int nx, ny, nz;         /* grid incl. CPML and pad (nz ignored in 2D) */
+    float dx, dy, dz, dt;   /* mm, mm, mm, us */
+    int ord;                /* number of FD coefficients, 2..6 */
+    float coefs[8];         /* staggered coefficients (coefs_Lui row) */
+    int variant;            /* VSE_VARIANT_*; 3D supports WGSL only */
+    int visco;              /* viscoelastic_attn flag */
+    int n_sls;              /* Zener mechanisms L, 1..8 */
+    float tau[32];          /* (4, L) row-major: tau_eps_p, tau_sig_p, tau_eps_s, tau_sig_s */
+    float sum_alpha[2];     /* sum_l tau_eps/tau_sig for P and S */
+    int n_rows;             /* rows of the source and trace buffers (= n_steps) */
+} vse_desc;
+
+int vse_abi_version(void);
+
+/* Allocate device state for one simulation.  Replaces the ~60 buffer
+ * creations per implementation() call (sim_webgpu_viscoelastic.py:83-191). */
+int vse_create(const vse_desc *desc, vse_sim **out);
+
+/* rho, cp, cs on the full grid (nx*ny[*nz] floats each); rho is the kernel
+ * density the host prepared (the reference's aliased, averaged
+ * _rho_grid_vx; simulator.py:88-131).  Replaces b_rho_map/b_cp_map/b_cs_map
+ * (sim_webgpu_viscoelastic.py:131-133). */
+int vse_set_material(vse_sim *s, const float *rho, const float *cp, const float *cs);
+
+/* CPML profiles of one axis (0=x, 1=y, 2=z), each of length
+ * n_axis - 2*(ord-1): full grid a, b, k and half grid a, b, k.  Replaces
+ * b_a_x ... b_k_y_h (sim_webgpu_viscoelastic.py:95-106). */
+int vse_set_cpml(vse_sim *s, int axis, const float *a, const float *b, const float *k,
+                 const float *a_h, const float *b_h, const float *k_h);
+
+/* Source points: n_pts linear cell indices with their source column, and the
+ * (n_rows, n_cols) row-major source matrix (value of column c at step s).
+ * WGSL variants add src[s][c]*dt/rho after the velocity update; the Taichi
+ * variant adds it inside the vy update.  Replaces b_force and b_idx_src
+ * (sim_webgpu_viscoelastic.py:89-92; sim_taichi_elastic.py:120-130). */
+int vse_set_sources(vse_sim *s, int n_pts, const int *cells, const int *cols,
+                    int n_cols, const float *src);
+
+/* Receivers: n_rec receivers; receiver r owns the point cells
+ * cells[beg[r] .. beg[r+1]) (beg has n_rec+1 entries) and records from the
+ * first step s with s + gate >= delay[r] (gate 0: WGSL, 1: Taichi).
+ * Replaces b_delay_rec, b_info_rec_pt, b_offset_sensors and the trace
+ * buffers (sim_webgpu_viscoelastic.py:174-191). */
+int vse_set_receivers(vse_sim *s, int n_rec, const int *beg, const int *cells,
+                      const int *delay, int gate);
+
+/* Zero every state array, the traces and the step counter. */
+int vse_reset(vse_sim *s);
+
+/* Run n_steps time steps (stress, velocity, sources+receivers per step),
+ * captured in hipGraphs of up to check_every steps; after each chunk the
+ * running max |v|^2 is checked and VSE_ERR_UNSTABLE returned if it is +inf
+ * or NaN (StabilityError, sim_webgpu_viscoelastic.py:533-535).  The step
+ * loop of sim_webgpu_viscoelastic.py:470-537 / sim_taichi_elastic.py:312-335.
+ * If elapsed_ms is not NULL it receives the HIP-event time of the loop. */
+int vse_run(vse_sim *s, int n_steps, int check_every, float *elapsed_ms);
+
+/* Run n_steps steps launching each kernel between its own HIP events on the
+ * library's stream; ms[0..2] receive the average duration of the stress,
+ * velocity and sparse (source/receiver/counter) kernels.  For roofline
+ * measurement; no reference counterpart (the reference only times the whole
+ * loop, sim_webgpu_viscoelastic.py:470,537). */
+int vse_profile(vse_sim *s, int n_steps, float *ms);
+
+/* Copy a field (VSE_F_*) to host, n cells in the reference layout.
+ * Replaces the sigma_yy / sigma_zz read-back (sim_webgpu_viscoelastic.py:540-541). */
+int vse_read_field(vse_sim *s, int field, float *out);
+
+/* Copy the (n_rows, n_rec) traces to host (sim_webgpu_viscoelastic.py:542-543). */
+int vse_read_traces(vse_sim *s, float *out);
+
+/* Zener memory variables of component c (0=xx,1=yy,2=zz,3=xy,4=xz,5=yz;
+ * 2D uses 0,1,3) in the reference's (cells, L) interleave. */
+int vse_read_memory(vse_sim *s, int comp, float *out);
+
+/* Running max of |v|^2 since the last reset (v_2 of wgsl2d :374-375). */
+int vse_read_v2max(vse_sim *s, float *out);
+
+/* Steps executed since the last reset. */
+int vse_get_step(vse_sim *s, int *out);
+
+/* Device pointer of a field, for zero-copy use by a tensor library. */
+void *vse_field_device_ptr(vse_sim *s, int field);
+
+/* Bytes of device memory held by the handle. */
+size_t vse_device_bytes(vse_sim *s);
+
+/* Name of the device the handle runs on (the reference's "gpu_str"). */
+const char *vse_device_name(vse_sim *s);
+
+const char *vse_last_error(vse_sim *s);
+
+void vse_destroy(vse_sim *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,86 @@
+"""ctypes binding of libvse (include/vse.h).
+
+The library is built in-tree (``taichi_sim_amd/libvse.so``).  There is no
+fallback: if it is missing or cannot be loaded, every entry point raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvse.so")
+
+VSE_OK = 0
+VSE_ERR_ARG = -1
+VSE_ERR_HIP = -2
+VSE_ERR_UNSTABLE = -3
+VSE_ERR_STATE = -4
+VARIANT_WGSL = 0
+VARIANT_TAICHI = 1
+FIELD = {"vx": 0, "vy": 1, "vz": 2, "sxx": 3, "syy": 4, "szz": 5, "sxy": 6, "sxz": 7, "syz": 8}
+MEMORY = {"xx": 0, "yy": 1, "zz": 2, "xy": 3, "xz": 4, "yz": 5}
+
+# every symbol include/vse.h declares
+EXPORTS = ["vse_abi_version", "vse_create", "vse_set_material", "vse_set_cpml", "vse_set_sources",
+           "vse_set_receivers", "vse_reset", "vse_run", "vse_profile", "vse_read_field", "vse_read_traces",
+           "vse_read_memory", "vse_read_v2max", "vse_get_step", "vse_field_device_ptr",
+           "vse_device_bytes", "vse_device_name", "vse_last_error", "vse_destroy"]
+
+
+class VseDesc(ctypes.Structure):
+    _fields_ = [("ndim", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_float), ("dy", ctypes.c_float), ("dz", ctypes.c_float), ("dt", ctypes.c_float),
+                ("ord", ctypes.c_int), ("coefs", ctypes.c_float * 8), ("variant", ctypes.c_int),
+                ("visco", ctypes.c_int), ("n_sls", ctypes.c_int), ("tau", ctypes.c_float * 32),
+                ("sum_alpha", ctypes.c_float * 2), ("n_rows", ctypes.c_int)]
+
+
+class VSEError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libvse error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libvse.so; raises if it has not been built (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with taichi_sim_amd._build.build() "
+                          "(hipcc --offload-arch=gfx950); the product has no CPU fallback")
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+    fp = ctypes.POINTER(ctypes.c_float)
+    ip = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "vse_abi_version": (I, []),
+        "vse_create": (I, [ctypes.POINTER(VseDesc), ctypes.POINTER(P)]),
+        "vse_set_material": (I, [P, fp, fp, fp]),
+        "vse_set_cpml": (I, [P, I, fp, fp, fp, fp, fp, fp]),
+        "vse_set_sources": (I, [P, I, ip, ip, I, fp]),
+        "vse_set_receivers": (I, [P, I, ip, ip, ip, I]),
+        "vse_reset": (I, [P]),
+        "vse_run": (I, [P, I, I, fp]),
+        "vse_profile": (I, [P, I, fp]),
+        "vse_read_field": (I, [P, I, fp]),
+        "vse_read_traces": (I, [P, fp]),
+        "vse_read_memory": (I, [P, I, fp]),
+        "vse_read_v2max": (I, [P, fp]),
+        "vse_get_step": (I, [P, ip]),
+        "vse_field_device_ptr": (P, [P, I]),
+        "vse_device_bytes": (ctypes.c_size_t, [P]),
+        "vse_device_name": (ctypes.c_char_p, [P]),
+        "vse_last_error": (ctypes.c_char_p, [P]),
+        "vse_destroy": (None, [P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if lib.vse_abi_version() != 1:
+        raise ImportError("libvse ABI version mismatch")
+    _lib = lib
+    return lib

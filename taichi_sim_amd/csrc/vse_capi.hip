@@ -1,0 +1,539 @@
+// vse_capi.hip -- C-ABI of libvse (declared in include/vse.h).
+//
+// Owns the device state of one simulation, turns the reference's host tables
+// into the kernels' compact forms (CPML bands, source list, receiver CSR) and
+// runs the step loop as hipGraphs of up to `check_every` steps on a private
+// stream, checking the running |v|^2 max between graphs.
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "vse_internal.h"
+#include "../../include/vse.h"
+
+namespace {
+
+// axis of the derivative each psi slot belongs to
+const int kSlotAxis[P_COUNT] = {0, 1, 2, 0, 1, 0, 2, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2};
+const bool kSlot2D[P_COUNT] = {true, true, false, true, true, false, false, false, false,
+                               true, false, false, true, true, false, false, false, false};
+
+}  // namespace
+
+struct vse_sim {
+  vse_desc d{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  long ncell = 0;
+  std::string err;
+  std::string devname;
+  size_t bytes = 0;
+  std::vector<void *> allocs;
+
+  float *field[9] = {};
+  float *rho = nullptr, *cp = nullptr, *cs = nullptr;
+  bool have_material = false;
+  float *prof[3][6] = {};
+  AxisProf ax[3]{};
+  bool have_cpml[3] = {false, false, false};
+  float *psi[P_COUNT] = {};
+  bool psi_ready = false;
+  float *r[6] = {};
+  ZenerK zk{};
+  unsigned *ctrl = nullptr;       // [0] = v2max bits, [1] = step counter
+  unsigned *host_ctrl = nullptr;  // pinned mirror
+  // sources
+  int n_src_pts = 0, src_cols = 1;
+  int *src_cell = nullptr, *src_col = nullptr, *src_map = nullptr;
+  float *src = nullptr;
+  // receivers
+  int n_rec = 0, rec_gate = 0;
+  int *rec_beg = nullptr, *rec_cell = nullptr, *rec_delay = nullptr;
+  float *traces = nullptr;
+  // cached graph
+  hipGraphExec_t gexec = nullptr;
+  int gexec_steps = 0;
+
+  int fail(int code, const std::string &m) { err = m; return code; }
+  int hip(hipError_t e, const char *what) {
+    if (e == hipSuccess) return 0;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return VSE_ERR_HIP;
+  }
+  template <typename T>
+  int alloc(T **p, size_t n) {
+    size_t nb = std::max<size_t>(n, 1) * sizeof(T);
+    void *q = nullptr;
+    int rc = hip(hipMalloc(&q, nb), "hipMalloc");
+    if (rc) return rc;
+    allocs.push_back(q);
+    bytes += nb;
+    *p = (T *)q;
+    return hip(hipMemsetAsync(q, 0, nb, stream), "hipMemsetAsync");
+  }
+  void release(void *p) {
+    if (!p) return;
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) allocs.erase(it);
+    (void)hipFree(p);
+  }
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    gexec_steps = 0;
+  }
+  long axis_len(int a) const { return a == 0 ? d.nx : (a == 1 ? d.ny : d.nz); }
+  long psi_elems(int slot) const {
+    const int a = kSlotAxis[slot];
+    if (d.ndim == 2) return a == 0 ? (long)ax[0].nb * d.ny : (long)d.nx * ax[1].nb;
+    if (a == 0) return (long)ax[0].nb * d.ny * d.nz;
+    if (a == 1) return (long)d.nx * ax[1].nb * d.nz;
+    return (long)d.nx * d.ny * ax[2].nb;
+  }
+  bool slot_used(int slot) const { return d.ndim == 3 || kSlot2D[slot]; }
+};
+
+static int check_handle(vse_sim *s) { return s ? 0 : VSE_ERR_ARG; }
+
+extern "C" {
+
+int vse_abi_version(void) { return VSE_ABI_VERSION; }
+
+int vse_create(const vse_desc *desc, vse_sim **out) {
+  if (!desc || !out) return VSE_ERR_ARG;
+  *out = nullptr;
+  const vse_desc &d = *desc;
+  if (d.ndim != 2 && d.ndim != 3) return VSE_ERR_ARG;
+  if (d.ord < 2 || d.ord > VSE_MAX_ORD) return VSE_ERR_ARG;
+  if (d.n_sls < 1 || d.n_sls > VSE_MAX_SLS) return VSE_ERR_ARG;
+  if (d.variant != VSE_VARIANT_WGSL && d.variant != VSE_VARIANT_TAICHI) return VSE_ERR_ARG;
+  if (d.ndim == 3 && d.variant != VSE_VARIANT_WGSL) return VSE_ERR_ARG;
+  const int minn = 2 * d.ord + 1;
+  if (d.nx < minn || d.ny < minn || (d.ndim == 3 && d.nz < minn)) return VSE_ERR_ARG;
+  if (d.n_rows < 1 || !(d.dt > 0.f) || !(d.dx > 0.f) || !(d.dy > 0.f) || (d.ndim == 3 && !(d.dz > 0.f)))
+    return VSE_ERR_ARG;
+
+  vse_sim *s = new vse_sim();
+  s->d = d;
+  if (d.ndim == 2) s->d.nz = 1;
+  s->ncell = (long)s->d.nx * s->d.ny * s->d.nz;
+  int rc = s->hip(hipGetDevice(&s->device), "hipGetDevice");
+  if (!rc) rc = s->hip(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+  if (!rc) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, s->device) == hipSuccess)
+      s->devname = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+  }
+  for (int f = 0; f < 9 && !rc; f++) {
+    const bool used = d.ndim == 3 || f == VSE_F_VX || f == VSE_F_VY || f == VSE_F_SXX || f == VSE_F_SYY ||
+                      f == VSE_F_SXY;
+    if (used) rc = s->alloc(&s->field[f], s->ncell);
+  }
+  if (!rc && d.visco && d.variant == VSE_VARIANT_WGSL) {
+    for (int c = 0; c < 6 && !rc; c++)
+      if (d.ndim == 3 || c == 0 || c == 1 || c == 3) rc = s->alloc(&s->r[c], (size_t)s->ncell * d.n_sls);
+  }
+  if (!rc) rc = s->alloc(&s->ctrl, 2);
+  if (!rc) rc = s->hip(hipHostMalloc((void **)&s->host_ctrl, 2 * sizeof(unsigned), hipHostMallocDefault),
+                       "hipHostMalloc");
+  // Zener constants per mechanism, float32 in the reference's operation order
+  // (wgsl2d :881-894): phi = dt(1-alpha)(1/tau_sig)/sum_alpha, h = 0.5 dt/tau_sig,
+  // m = 1/(1+h); identical to the per-cell values the reference recomputes.
+  if (!rc) {
+    const int L = d.n_sls;
+    for (int l = 0; l < L; l++) {
+      const float t_ep = d.tau[0 * L + l], t_sp = d.tau[1 * L + l];
+      const float t_es = d.tau[2 * L + l], t_ss = d.tau[3 * L + l];
+      const float inv_p = 1.0f / t_sp, inv_s = 1.0f / t_ss;
+      const float al_p = t_ep / t_sp, al_s = t_es / t_ss;
+      s->zk.phi_p[l] = d.dt * (1.0f - al_p) * inv_p / d.sum_alpha[0];
+      s->zk.phi_s[l] = d.dt * (1.0f - al_s) * inv_s / d.sum_alpha[1];
+      s->zk.h_p[l] = 0.5f * d.dt * inv_p;
+      s->zk.h_s[l] = 0.5f * d.dt * inv_s;
+      s->zk.m_p[l] = 1.0f / (1.0f + s->zk.h_p[l]);
+      s->zk.m_s[l] = 1.0f / (1.0f + s->zk.h_s[l]);
+    }
+  }
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "create");
+  if (rc) {
+    vse_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return VSE_OK;
+}
+
+int vse_set_material(vse_sim *s, const float *rho, const float *cp, const float *cs) {
+  if (check_handle(s) || !rho || !cp || !cs) return VSE_ERR_ARG;
+  int rc = 0;
+  if (!s->rho) {
+    rc = s->alloc(&s->rho, s->ncell);
+    if (!rc) rc = s->alloc(&s->cp, s->ncell);
+    if (!rc) rc = s->alloc(&s->cs, s->ncell);
+    if (rc) return rc;
+  }
+  const size_t nb = s->ncell * sizeof(float);
+  rc = s->hip(hipMemcpyAsync(s->rho, rho, nb, hipMemcpyHostToDevice, s->stream), "copy rho");
+  if (!rc) rc = s->hip(hipMemcpyAsync(s->cp, cp, nb, hipMemcpyHostToDevice, s->stream), "copy cp");
+  if (!rc) rc = s->hip(hipMemcpyAsync(s->cs, cs, nb, hipMemcpyHostToDevice, s->stream), "copy cs");
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "set_material");
+  if (!rc) s->have_material = true;
+  return rc;
+}
+
+int vse_set_cpml(vse_sim *s, int axis, const float *a, const float *b, const float *k, const float *a_h,
+                 const float *b_h, const float *k_h) {
+  if (check_handle(s) || axis < 0 || axis >= s->d.ndim) return VSE_ERR_ARG;
+  const float *src[6] = {a, b, k, a_h, b_h, k_h};
+  for (int i = 0; i < 6; i++)
+    if (!src[i]) return VSE_ERR_ARG;
+  const int np = (int)s->axis_len(axis) - 2 * (s->d.ord - 1);
+  // active band: a != 0 or k != 1 on either grid
+  std::vector<char> act(np);
+  for (int j = 0; j < np; j++) act[j] = a[j] != 0.f || k[j] != 1.f || a_h[j] != 0.f || k_h[j] != 1.f;
+  int lo = 0;
+  while (lo < np && act[lo]) lo++;
+  int hi = np;
+  while (hi > lo && act[hi - 1]) hi--;
+  for (int j = lo; j < hi; j++)
+    if (act[j]) { lo = np; hi = np; break; }   // not two end bands: keep the whole axis
+  if (s->psi_ready) {
+    AxisProf &o = s->ax[axis];
+    if (o.lo != lo || o.hi != hi) return s->fail(VSE_ERR_STATE, "CPML band changed after state allocation");
+  }
+  int rc = 0;
+  for (int i = 0; i < 6 && !rc; i++) {
+    if (!s->prof[axis][i]) rc = s->alloc(&s->prof[axis][i], np);
+    if (!rc) rc = s->hip(hipMemcpyAsync(s->prof[axis][i], src[i], np * sizeof(float), hipMemcpyHostToDevice,
+                                        s->stream), "copy cpml");
+  }
+  if (rc) return rc;
+  AxisProf &P = s->ax[axis];
+  P.a = s->prof[axis][0]; P.b = s->prof[axis][1]; P.k = s->prof[axis][2];
+  P.ah = s->prof[axis][3]; P.bh = s->prof[axis][4]; P.kh = s->prof[axis][5];
+  P.lo = lo; P.hi = hi; P.np = np; P.nb = lo + (np - hi);
+  s->have_cpml[axis] = true;
+  s->drop_graph();
+  return s->hip(hipStreamSynchronize(s->stream), "set_cpml");
+}
+
+int vse_set_sources(vse_sim *s, int n_pts, const int *cells, const int *cols, int n_cols, const float *src) {
+  if (check_handle(s) || n_pts < 0 || n_cols < 1 || !src || (n_pts > 0 && (!cells || !cols))) return VSE_ERR_ARG;
+  for (int i = 0; i < n_pts; i++)
+    if (cells[i] < 0 || cells[i] >= s->ncell || cols[i] < 0 || cols[i] >= n_cols)
+      return s->fail(VSE_ERR_ARG, "source point out of range");
+  s->release(s->src_cell); s->release(s->src_col); s->release(s->src); s->release(s->src_map);
+  s->src_cell = s->src_col = s->src_map = nullptr;
+  s->src = nullptr;
+  int rc = s->alloc(&s->src_cell, n_pts);
+  if (!rc) rc = s->alloc(&s->src_col, n_pts);
+  if (!rc) rc = s->alloc(&s->src, (size_t)s->d.n_rows * n_cols);
+  if (!rc && n_pts) rc = s->hip(hipMemcpyAsync(s->src_cell, cells, n_pts * sizeof(int), hipMemcpyHostToDevice, s->stream), "src cells");
+  if (!rc && n_pts) rc = s->hip(hipMemcpyAsync(s->src_col, cols, n_pts * sizeof(int), hipMemcpyHostToDevice, s->stream), "src cols");
+  if (!rc) rc = s->hip(hipMemcpyAsync(s->src, src, (size_t)s->d.n_rows * n_cols * sizeof(float), hipMemcpyHostToDevice, s->stream), "src");
+  std::vector<int> map;
+  if (!rc && s->d.variant == VSE_VARIANT_TAICHI) {
+    map.assign(s->ncell, -1);
+    for (int i = 0; i < n_pts; i++) map[cells[i]] = cols[i];
+    rc = s->alloc(&s->src_map, s->ncell);
+    if (!rc) rc = s->hip(hipMemcpyAsync(s->src_map, map.data(), s->ncell * sizeof(int), hipMemcpyHostToDevice, s->stream), "src map");
+  }
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "set_sources");
+  if (rc) return rc;
+  s->n_src_pts = n_pts;
+  s->src_cols = n_cols;
+  s->drop_graph();
+  return VSE_OK;
+}
+
+int vse_set_receivers(vse_sim *s, int n_rec, const int *beg, const int *cells, const int *delay, int gate) {
+  if (check_handle(s) || n_rec < 0 || (n_rec > 0 && (!beg || !delay))) return VSE_ERR_ARG;
+  const int n_pts = n_rec > 0 ? beg[n_rec] : 0;
+  if (n_rec > 0 && beg[0] != 0) return s->fail(VSE_ERR_ARG, "receiver CSR must start at 0");
+  for (int r = 0; r < n_rec; r++)
+    if (beg[r + 1] < beg[r]) return s->fail(VSE_ERR_ARG, "receiver CSR not monotone");
+  if (n_pts > 0 && !cells) return VSE_ERR_ARG;
+  for (int p = 0; p < n_pts; p++)
+    if (cells[p] < 0 || cells[p] >= s->ncell) return s->fail(VSE_ERR_ARG, "receiver point out of range");
+  s->release(s->rec_beg); s->release(s->rec_cell); s->release(s->rec_delay); s->release(s->traces);
+  s->rec_beg = s->rec_cell = s->rec_delay = nullptr;
+  s->traces = nullptr;
+  int rc = s->alloc(&s->rec_beg, n_rec + 1);
+  if (!rc) rc = s->alloc(&s->rec_cell, n_pts);
+  if (!rc) rc = s->alloc(&s->rec_delay, n_rec);
+  if (!rc) rc = s->alloc(&s->traces, (size_t)s->d.n_rows * std::max(n_rec, 1));
+  if (!rc && n_rec) rc = s->hip(hipMemcpyAsync(s->rec_beg, beg, (n_rec + 1) * sizeof(int), hipMemcpyHostToDevice, s->stream), "rec beg");
+  if (!rc && n_pts) rc = s->hip(hipMemcpyAsync(s->rec_cell, cells, n_pts * sizeof(int), hipMemcpyHostToDevice, s->stream), "rec cells");
+  if (!rc && n_rec) rc = s->hip(hipMemcpyAsync(s->rec_delay, delay, n_rec * sizeof(int), hipMemcpyHostToDevice, s->stream), "rec delay");
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "set_receivers");
+  if (rc) return rc;
+  s->n_rec = n_rec;
+  s->rec_gate = gate;
+  s->drop_graph();
+  return VSE_OK;
+}
+
+static int finalize(vse_sim *s) {
+  if (!s->have_material) return s->fail(VSE_ERR_STATE, "vse_set_material not called");
+  for (int a = 0; a < s->d.ndim; a++)
+    if (!s->have_cpml[a]) return s->fail(VSE_ERR_STATE, "vse_set_cpml not called for every axis");
+  if (!s->src) return s->fail(VSE_ERR_STATE, "vse_set_sources not called");
+  if (!s->rec_beg) return s->fail(VSE_ERR_STATE, "vse_set_receivers not called");
+  if (s->psi_ready) return 0;
+  for (int slot = 0; slot < P_COUNT; slot++) {
+    if (!s->slot_used(slot)) continue;
+    int rc = s->alloc(&s->psi[slot], s->psi_elems(slot));
+    if (rc) return rc;
+  }
+  s->psi_ready = true;
+  return s->hip(hipStreamSynchronize(s->stream), "finalize");
+}
+
+int vse_reset(vse_sim *s) {
+  if (check_handle(s)) return VSE_ERR_ARG;
+  int rc = finalize(s);
+  if (rc) return rc;
+  for (int f = 0; f < 9 && !rc; f++)
+    if (s->field[f]) rc = s->hip(hipMemsetAsync(s->field[f], 0, s->ncell * sizeof(float), s->stream), "reset");
+  for (int c = 0; c < 6 && !rc; c++)
+    if (s->r[c]) rc = s->hip(hipMemsetAsync(s->r[c], 0, (size_t)s->ncell * s->d.n_sls * sizeof(float), s->stream), "reset r");
+  for (int p = 0; p < P_COUNT && !rc; p++)
+    if (s->psi[p]) rc = s->hip(hipMemsetAsync(s->psi[p], 0, std::max<long>(s->psi_elems(p), 1) * sizeof(float), s->stream), "reset psi");
+  if (!rc) rc = s->hip(hipMemsetAsync(s->traces, 0, (size_t)s->d.n_rows * std::max(s->n_rec, 1) * sizeof(float), s->stream), "reset traces");
+  if (!rc) rc = s->hip(hipMemsetAsync(s->ctrl, 0, 2 * sizeof(unsigned), s->stream), "reset ctrl");
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "reset");
+  return rc;
+}
+
+static void make_args(vse_sim *s, StepArgs &A, SparseArgs &S) {
+  const vse_desc &d = s->d;
+  memset(&A, 0, sizeof(A));
+  A.nx = d.nx; A.ny = d.ny; A.nz = d.nz; A.ncell = s->ncell;
+  A.dx = d.dx; A.dy = d.dy; A.dz = d.ndim == 3 ? d.dz : 1.f; A.dt = d.dt;
+  A.one_dx = 1.0f / d.dx; A.one_dy = 1.0f / d.dy; A.one_dz = d.ndim == 3 ? 1.0f / d.dz : 1.f;
+  for (int c = 0; c < d.ord; c++) A.C[c] = d.coefs[c];
+  for (int a = 0; a < 3; a++) A.ax[a] = s->ax[a];
+  A.vx = s->field[VSE_F_VX]; A.vy = s->field[VSE_F_VY]; A.vz = s->field[VSE_F_VZ];
+  A.sxx = s->field[VSE_F_SXX]; A.syy = s->field[VSE_F_SYY]; A.szz = s->field[VSE_F_SZZ];
+  A.sxy = s->field[VSE_F_SXY]; A.sxz = s->field[VSE_F_SXZ]; A.syz = s->field[VSE_F_SYZ];
+  for (int p = 0; p < P_COUNT; p++) A.psi[p] = s->psi[p];
+  for (int c = 0; c < 6; c++) A.r[c] = s->r[c];
+  A.rho = s->rho; A.cp = s->cp; A.cs = s->cs;
+  A.zk = s->zk; A.L = d.n_sls;
+  A.v2max = s->ctrl;
+  A.step = (const int *)(s->ctrl + 1);
+  A.src_map = s->src_map; A.src = s->src; A.src_cols = s->src_cols; A.src_rows = d.n_rows;
+
+  memset(&S, 0, sizeof(S));
+  S.ndim = d.ndim; S.nx = d.nx; S.ny = d.ny; S.nz = d.nz; S.dt = d.dt;
+  S.vx = A.vx; S.vy = A.vy; S.vz = A.vz;
+  S.rec_field = d.ndim == 2 ? A.syy : A.szz;
+  S.rho = s->rho;
+  S.v2max = s->ctrl;
+  S.step = (int *)(s->ctrl + 1);
+  S.n_src_pts = d.variant == VSE_VARIANT_TAICHI ? 0 : s->n_src_pts;
+  S.src_cell = s->src_cell; S.src_col = s->src_col; S.src = s->src; S.src_cols = s->src_cols; S.src_rows = d.n_rows;
+  S.n_rec = s->n_rec; S.rec_beg = s->rec_beg; S.rec_cell = s->rec_cell; S.rec_delay = s->rec_delay;
+  S.rec_gate = s->rec_gate; S.traces = s->traces; S.trace_rows = d.n_rows;
+}
+
+static hipError_t enqueue_step(vse_sim *s, const StepArgs &A, const SparseArgs &S) {
+  const vse_desc &d = s->d;
+  hipError_t e;
+  if (d.ndim == 2) {
+    e = vse_launch_stress2d(d.variant, d.visco, d.ord, A, s->stream);
+    if (e != hipSuccess) return e;
+    e = vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->stream);
+  } else {
+    e = vse_launch_stress3d(d.visco, d.ord, A, s->stream);
+    if (e != hipSuccess) return e;
+    e = vse_launch_velocity3d(d.visco, d.ord, A, s->stream);
+  }
+  if (e != hipSuccess) return e;
+  return vse_launch_sparse(S, s->stream);
+}
+
+static int get_graph(vse_sim *s, int steps, const StepArgs &A, const SparseArgs &S) {
+  if (s->gexec && s->gexec_steps == steps) return 0;
+  s->drop_graph();
+  hipGraph_t g = nullptr;
+  int rc = s->hip(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  if (rc) return rc;
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < steps && e == hipSuccess; i++) e = enqueue_step(s, A, S);
+  hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+  if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return s->hip(e, "capture launch"); }
+  rc = s->hip(e2, "end capture");
+  if (rc) return rc;
+  rc = s->hip(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0), "graph instantiate");
+  (void)hipGraphDestroy(g);
+  if (!rc) s->gexec_steps = steps;
+  return rc;
+}
+
+static int check_stable(vse_sim *s) {
+  int rc = s->hip(hipMemcpyAsync(s->host_ctrl, s->ctrl, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream), "read v2max");
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "sync");
+  if (rc) return rc;
+  if (s->host_ctrl[0] >= 0x7f800000u) {
+    float v;
+    memcpy(&v, &s->host_ctrl[0], 4);
+    char buf[128];
+    snprintf(buf, sizeof buf, "Simulacao tornando-se instavel [|v|^2 = %g at step %u]", v, s->host_ctrl[1]);
+    return s->fail(VSE_ERR_UNSTABLE, buf);
+  }
+  return 0;
+}
+
+int vse_run(vse_sim *s, int n_steps, int check_every, float *elapsed_ms) {
+  if (check_handle(s) || n_steps < 0) return VSE_ERR_ARG;
+  int rc = finalize(s);
+  if (rc) return rc;
+  if (n_steps == 0) { if (elapsed_ms) *elapsed_ms = 0.f; return 0; }
+  StepArgs A;
+  SparseArgs S;
+  make_args(s, A, S);
+  const int chunk = std::max(1, std::min(check_every > 0 ? check_every : 256, std::min(n_steps, 256)));
+  rc = get_graph(s, chunk, A, S);
+  if (rc) return rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (elapsed_ms) {
+    rc = s->hip(hipEventCreate(&e0), "event");
+    if (!rc) rc = s->hip(hipEventCreate(&e1), "event");
+    if (!rc) rc = s->hip(hipEventRecord(e0, s->stream), "event record");
+  }
+  int done = 0, since_check = 0;
+  while (!rc && done < n_steps) {
+    const int left = n_steps - done;
+    if (left >= chunk) {
+      rc = s->hip(hipGraphLaunch(s->gexec, s->stream), "graph launch");
+      done += chunk;
+      since_check += chunk;
+    } else {
+      for (int i = 0; i < left && !rc; i++) rc = s->hip(enqueue_step(s, A, S), "step launch");
+      done += left;
+      since_check += left;
+    }
+    if (!rc && check_every > 0 && since_check >= check_every && done < n_steps) {
+      rc = check_stable(s);
+      since_check = 0;
+    }
+  }
+  if (elapsed_ms && !rc) {
+    rc = s->hip(hipEventRecord(e1, s->stream), "event record");
+    if (!rc) rc = s->hip(hipEventSynchronize(e1), "event sync");
+    if (!rc) rc = s->hip(hipEventElapsedTime(elapsed_ms, e0, e1), "elapsed");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (!rc) rc = check_stable(s);
+  return rc;
+}
+
+int vse_profile(vse_sim *s, int n_steps, float *ms) {
+  if (check_handle(s) || n_steps < 1 || !ms) return VSE_ERR_ARG;
+  int rc = finalize(s);
+  if (rc) return rc;
+  StepArgs A;
+  SparseArgs S;
+  make_args(s, A, S);
+  const vse_desc &d = s->d;
+  hipEvent_t ev[4] = {};
+  for (int i = 0; i < 4 && !rc; i++) rc = s->hip(hipEventCreate(&ev[i]), "event");
+  double acc[3] = {0, 0, 0};
+  for (int it = 0; it < n_steps && !rc; it++) {
+    hipError_t e = hipEventRecord(ev[0], s->stream);
+    if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_stress2d(d.variant, d.visco, d.ord, A, s->stream)
+                                         : vse_launch_stress3d(d.visco, d.ord, A, s->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], s->stream);
+    if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->stream)
+                                         : vse_launch_velocity3d(d.visco, d.ord, A, s->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
+    if (e == hipSuccess) e = vse_launch_sparse(S, s->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[3], s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
+    rc = s->hip(e, "profile step");
+    for (int k = 0; k < 3 && !rc; k++) {
+      float t = 0.f;
+      rc = s->hip(hipEventElapsedTime(&t, ev[k], ev[k + 1]), "elapsed");
+      acc[k] += t;
+    }
+  }
+  for (int i = 0; i < 4; i++)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
+  if (!rc)
+    for (int k = 0; k < 3; k++) ms[k] = (float)(acc[k] / n_steps);
+  if (!rc) rc = check_stable(s);
+  return rc;
+}
+
+int vse_read_field(vse_sim *s, int field, float *out) {
+  if (check_handle(s) || field < 0 || field > 8 || !out) return VSE_ERR_ARG;
+  if (!s->field[field]) return s->fail(VSE_ERR_ARG, "field not present in this simulation");
+  int rc = s->hip(hipMemcpyAsync(out, s->field[field], s->ncell * sizeof(float), hipMemcpyDeviceToHost, s->stream), "read field");
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "read field sync");
+  return rc;
+}
+
+int vse_read_traces(vse_sim *s, float *out) {
+  if (check_handle(s) || !out) return VSE_ERR_ARG;
+  if (!s->traces) return s->fail(VSE_ERR_STATE, "no receivers");
+  int rc = s->hip(hipMemcpyAsync(out, s->traces, (size_t)s->d.n_rows * s->n_rec * sizeof(float), hipMemcpyDeviceToHost, s->stream), "read traces");
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "read traces sync");
+  return rc;
+}
+
+int vse_read_memory(vse_sim *s, int comp, float *out) {
+  if (check_handle(s) || comp < 0 || comp > 5 || !out) return VSE_ERR_ARG;
+  if (!s->r[comp]) return s->fail(VSE_ERR_ARG, "memory variable not present");
+  const int L = s->d.n_sls;
+  std::vector<float> planes((size_t)s->ncell * L);
+  int rc = s->hip(hipMemcpyAsync(planes.data(), s->r[comp], planes.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream), "read r");
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "read r sync");
+  if (rc) return rc;
+  for (long c = 0; c < s->ncell; c++)
+    for (int l = 0; l < L; l++) out[c * L + l] = planes[(size_t)l * s->ncell + c];
+  return 0;
+}
+
+int vse_read_v2max(vse_sim *s, float *out) {
+  if (check_handle(s) || !out) return VSE_ERR_ARG;
+  int rc = s->hip(hipMemcpyAsync(s->host_ctrl, s->ctrl, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream), "read ctrl");
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "sync");
+  if (!rc) memcpy(out, &s->host_ctrl[0], 4);
+  return rc;
+}
+
+int vse_get_step(vse_sim *s, int *out) {
+  if (check_handle(s) || !out) return VSE_ERR_ARG;
+  int rc = s->hip(hipMemcpyAsync(s->host_ctrl, s->ctrl, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream), "read ctrl");
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "sync");
+  if (!rc) *out = (int)s->host_ctrl[1];
+  return rc;
+}
+
+void *vse_field_device_ptr(vse_sim *s, int field) {
+  if (!s || field < 0 || field > 8) return nullptr;
+  return s->field[field];
+}
+
+size_t vse_device_bytes(vse_sim *s) { return s ? s->bytes : 0; }
+
+const char *vse_device_name(vse_sim *s) { return s ? s->devname.c_str() : ""; }
+
+const char *vse_last_error(vse_sim *s) { return s ? s->err.c_str() : "null handle"; }
+
+void vse_destroy(vse_sim *s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  s->drop_graph();
+  for (void *p : s->allocs) (void)hipFree(p);
+  s->allocs.clear();
+  if (s->host_ctrl) (void)hipHostFree(s->host_ctrl);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// vse_device.h -- device helpers shared by the 2D and 3D kernels.
+#pragma once
+#include "vse_internal.h"
+
+// One staggered-difference term.  WGSL divides each term by the spacing
+// (wgsl2d :850: coef*(a-b)/dx); Taichi multiplies by f32(1/dx)
+// (sim_taichi_elastic.py:154).  Keeping the reference's form keeps FP32
+// rounding term for term (the library is built with -ffp-contract=off and
+// IEEE division).
+template <int VAR>
+__device__ __forceinline__ float fd_term(float C, float a, float b, float d, float one_d) {
+  if constexpr (VAR == VSE_VAR_TAICHI) return C * (a - b) * one_d;
+  else return C * (a - b) / d;
+}
+
+// Compact band index of profile index j, or -1 when j is outside the band.
+__device__ __forceinline__ int band_index(const AxisProf &P, int j) {
+  return j < P.lo ? j : (j >= P.hi ? P.lo + (j - P.hi) : -1);
+}
+
+// CPML recursion psi <- b psi + a D; returns D/k + psi (wgsl2d :854-861).
+// Outside the band the recursion is an exact no-op and D is returned as is.
+__device__ __forceinline__ float cpml_stretch(const float *a, const float *b, const float *k,
+                                              float *psi, long pidx, int j, bool in_band, float D) {
+  if (!in_band) return D;
+  float pn = b[j] * psi[pidx] + a[j] * D;
+  psi[pidx] = pn;
+  return D / k[j] + pn;
+}
+
+// Max over a 64-wide wavefront, then one atomicMax per block.  For |v|^2 >= 0
+// the IEEE bit pattern orders like the value; NaN sorts above +inf, so the
+// running max also flags NaN (the reference's `vsn2 > 1e38` test does not).
+__device__ __forceinline__ void block_max_u32(unsigned v, unsigned *dst) {
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned o = __shfl_xor(v, off, 64);
+    v = v > o ? v : o;
+  }
+  __shared__ unsigned wmax[16];
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x + threadIdx.z * blockDim.x * blockDim.y;
+  const int nthr = blockDim.x * blockDim.y * blockDim.z;
+  const int lane = tid & 63, wave = tid >> 6;
+  if (lane == 0) wmax[wave] = v;
+  __syncthreads();
+  if (tid == 0) {
+    unsigned m = 0;
+    for (int w = 0; w < (nthr + 63) / 64; w++) m = m > wmax[w] ? m : wmax[w];
+    if (m) atomicMax(dst, m);
+  }
+}

@@ -1,0 +1,93 @@
+// vse_internal.h -- device-side argument blocks shared by the 2D/3D kernels
+// and the C-ABI layer of libvse.
+//
+// Grid layout is the reference's: C order with the last axis fastest
+// (2D index y + x*ny, wgsl2d :18-22; 3D index z + y*nz + x*ny*nz, wgsl3d
+// :28-32).  Zener memory variables are stored as L contiguous planes per
+// component (plane l of component c at r[c] + l*ncell) instead of the
+// reference's (cells, L) interleave, so every load is a coalesced stream.
+// CPML memory variables (psi) are stored only over each axis's active band.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define VSE_MAX_ORD 6
+#define VSE_MAX_SLS 8
+#define VSE_VAR_WGSL 0
+#define VSE_VAR_TAICHI 1
+
+// psi slots, reference naming (wgsl2d :444-610, wgsl3d :622-997)
+enum {
+  P_DVX_DX = 0, P_DVY_DY, P_DVZ_DZ, P_DVY_DX, P_DVX_DY, P_DVZ_DX, P_DVX_DZ, P_DVZ_DY, P_DVY_DZ,
+  P_DSXX_DX, P_DSXY_DY, P_DSXZ_DZ, P_DSXY_DX, P_DSYY_DY, P_DSYZ_DZ, P_DSXZ_DX, P_DSYZ_DY, P_DSZZ_DZ,
+  P_COUNT
+};
+
+// CPML profiles of one axis.  Profile index j = cell - (ord-1) runs over
+// [0, np).  The profile is "active" (a != 0 or k != 1 on the full or half
+// grid) only for j < lo or j >= hi; there psi lives in a compact band of
+// width nb = lo + (np - hi).  Outside the band a = 0 and k = 1 exactly, so
+// psi stays 0 and the stretched derivative equals the plain one bit for bit.
+struct AxisProf {
+  const float *a, *b, *k, *ah, *bh, *kh;
+  int lo, hi, np, nb;
+};
+
+struct ZenerK {
+  float phi_p[VSE_MAX_SLS], phi_s[VSE_MAX_SLS];
+  float h_p[VSE_MAX_SLS], h_s[VSE_MAX_SLS];
+  float m_p[VSE_MAX_SLS], m_s[VSE_MAX_SLS];
+};
+
+struct StepArgs {
+  int nx, ny, nz;
+  long ncell;
+  float dx, dy, dz, dt;
+  float one_dx, one_dy, one_dz;      // f32(1/d) (Taichi variant)
+  float C[VSE_MAX_ORD];
+  AxisProf ax[3];
+  float *vx, *vy, *vz;
+  float *sxx, *syy, *szz, *sxy, *sxz, *syz;
+  float *psi[P_COUNT];
+  float *r[6];                       // r_xx, r_yy, r_zz, r_xy, r_xz, r_yz (3D); 2D: xx, yy, -, xy
+  const float *rho, *cp, *cs;
+  ZenerK zk;
+  int L;
+  unsigned *v2max;                   // running max of |v|^2 as float bits
+  const int *step;                   // device step counter (0-based)
+  // Taichi-variant source (inline in the velocity update)
+  const int *src_map;                // full grid, -1 or column
+  const float *src;                  // (n_rows, src_cols)
+  int src_cols, src_rows;
+};
+
+// Sparse per-step work: source injection (WGSL variants), receiver gather,
+// and the step counter.  One block.
+struct SparseArgs {
+  int ndim, nx, ny, nz;
+  float dt;
+  float *vx, *vy, *vz;
+  const float *rec_field;            // sigma_yy (2D) / sigma_zz (3D)
+  const float *rho;
+  unsigned *v2max;
+  int *step;
+  // sources: n_src_pts cells; value src[s*src_cols + col]
+  int n_src_pts;
+  const int *src_cell, *src_col;
+  const float *src;
+  int src_cols, src_rows;
+  // receivers: CSR over sensors
+  int n_rec;
+  const int *rec_beg, *rec_cell;     // rec_beg has n_rec+1 entries
+  const int *rec_delay;
+  int rec_gate;                      // record when s + rec_gate >= delay
+  float *traces;                     // (rows, n_rec)
+  int trace_rows;
+};
+
+// launchers (vse_k2d.hip / vse_k3d.hip)
+hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st);
+hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st);
+hipError_t vse_launch_stress3d(int visco, int ord, const StepArgs &A, hipStream_t st);
+hipError_t vse_launch_velocity3d(int visco, int ord, const StepArgs &A, hipStream_t st);
+hipError_t vse_launch_sparse(const SparseArgs &S, hipStream_t st);

@@ -1,0 +1,147 @@
+"""Synthetic workloads in the reference's configuration format.
+
+Each builder returns a configuration dict accepted by ``Simulator`` /
+``Simulator3D`` (material maps as arrays instead of .npy paths).  Grid spacing
+is 0.1 mm, every side has a 20-cell CPML, and dt sits at CFL 0.5:
+dt = 0.5 dx / (cp_max sqrt(ndim)).  Random media are seeded.
+
+* ``elastic_2d_point``: 2D isotropic elastic, aluminium, point force at the
+  centre, a line of receivers (the CPU-runnable case).
+* ``visco_2d_array``: 2D Zener (L=3, constant Q fitted over 2-8 MHz) steel
+  plate with random circular voids and a focusing linear array.
+* ``visco_3d_block``: 3D Zener two-layer block (aluminium over steel) with
+  random ellipsoidal density inclusions, a circular piston on the top face and
+  a square grid of surface receivers.
+"""
+import numpy as np
+
+AL = (6.32, 3.13, 2700.0)
+STEEL = (5.9, 3.2, 7850.0)
+PML = 20
+DX = 0.1
+
+
+def cfl_dt(cp_max, ndim, dx=DX):
+    return float(np.float32(0.5 * dx / (cp_max * np.sqrt(ndim))))
+
+
+def _roi(n_full, ndim, acc=2):
+    """ROI length giving a full grid of n_full cells: n = len + 2*PML + 2*pad."""
+    pad = acc - 1
+    n_roi = n_full - 2 * PML - 2 * pad
+    roi = {"coord_ref": [[0.0, 0.0, 0.0]], "width": n_roi * DX, "w_len": n_roi,
+           "height": n_roi * DX, "h_len": n_roi, "depth": 0.0, "d_len": 1,
+           "len_pml_xmin": PML, "len_pml_xmax": PML, "len_pml_ymin": PML if ndim == 3 else 10,
+           "len_pml_ymax": PML if ndim == 3 else 10, "len_pml_zmin": PML, "len_pml_zmax": PML,
+           "rcoef_pml": 0.0001, "npower_pml": 2.0, "k_max_pml": 1.0}
+    if ndim == 3:
+        roi["depth"] = n_roi * DX
+        roi["d_len"] = n_roi
+    return roi, n_roi
+
+
+def _configs(visco, **extra):
+    c = {"n_iter": 1, "show_anim": 0, "show_debug": 0, "save_results": 0, "save_bscan": 0,
+         "viscoelastic_attn": int(visco), "emission_laws": "", "results_dir": ""}
+    c.update(extra)
+    return c
+
+
+def elastic_2d_point(n=256, steps=1000, n_rec=128, material=AL):
+    roi, L = _roi(n, 2)
+    cp, cs, rho = material
+    w = L * DX
+    probes = [{"point": {"coord_center": [w / 2, 0.0, w / 2], "freq": 5.0, "bw": 0.5, "gain": 1.0,
+                         "emitter": "True", "receiver": "False", "t0_emission": 1.0}}]
+    xs = np.linspace(0.1 * w, 0.8 * w, n_rec)
+    for x in xs:
+        probes.append({"point": {"coord_center": [float(np.floor(x * 100) / 100), 0.0, float(np.floor(0.25 * w * 100) / 100)],
+                                 "freq": 5.0, "bw": 0.5, "gain": 1.0, "emitter": "False",
+                                 "receiver": "True", "t0_reception": 0.0}})
+    return {"roi": roi, "probes": probes,
+            "simul_params": {"time_steps": steps, "dt": cfl_dt(cp, 2), "acc": 2, "it_display": 100},
+            "simul_configs": _configs(False),
+            "specimen_params": {"cp": cp, "cs": cs, "rho": rho}}
+
+
+def _voids_2d(shape, n_voids, seed, margin_cells):
+    rng = np.random.default_rng(seed)
+    nx, nz = shape
+    mask = np.zeros(shape, dtype=bool)
+    X, Z = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    for _ in range(n_voids):
+        r = rng.uniform(0.3, 1.5) / DX
+        cx = rng.uniform(margin_cells, nx - margin_cells)
+        cz = rng.uniform(min(margin_cells + 40, nz // 3), nz - margin_cells)
+        mask |= (X - cx) ** 2 + (Z - cz) ** 2 <= r * r
+    return mask
+
+
+def visco_2d_array(n=2048, steps=4000, n_elem=64, n_voids=16, seed=2, q=50.0, emitters="all",
+                   t0_emission=None):
+    roi, L = _roi(n, 2)
+    cp, cs, rho = STEEL
+    mask = _voids_2d((L, L), n_voids, seed, margin_cells=min(20, L // 8))
+    cp_m = np.where(mask, 0.0, cp).astype(np.float32)
+    cs_m = np.where(mask, 0.0, cs).astype(np.float32)
+    rho_m = np.where(mask, 0.0, rho).astype(np.float32)
+    w = L * DX
+    pitch = 0.5
+    if t0_emission is None:
+        rng = np.random.default_rng(seed + 100)
+        fx, fz = rng.uniform(0.3 * w, 0.7 * w), rng.uniform(0.3 * w, 0.7 * w)
+        ex = w / 2 + (np.arange(n_elem) - (n_elem - 1) / 2) * pitch
+        tof = np.sqrt((ex - fx) ** 2 + fz ** 2) / cp
+        t0_emission = [float(v) for v in (tof.max() - tof + 1.0)]
+    probes = [{"linear": {"id": "array", "num_elem": n_elem, "dim_a": 0.42, "dim_p": 10.0, "inter_elem": 0.08,
+                          "freq": 5.0, "bw": 0.5, "gain": 1e2, "pulse_type": "gaussian",
+                          "coord_center": [w / 2, 0.0, 0.0], "emitters": emitters,
+                          "t0_emission": t0_emission, "receivers": "all", "t0_reception": 0.0}}]
+    return {"roi": roi, "probes": probes,
+            "simul_params": {"time_steps": steps, "dt": cfl_dt(cp, 2), "acc": 2, "it_display": 100},
+            "simul_configs": _configs(True),
+            "specimen_params": {"cp": cp, "cs": cs, "rho": rho, "cp_map": cp_m, "cs_map": cs_m, "rho_map": rho_m},
+            "attenuation": {"q_p": q, "q_s": q, "f_min": 2.0, "f_max": 8.0, "n_sls": 3}}
+
+
+def visco_3d_block(n=512, steps=2000, n_incl=24, seed=3, q=50.0, n_rec_side=32, rec_pitch=1.0):
+    roi, L = _roi(n, 3)
+    rng = np.random.default_rng(seed)
+    al_cp, al_cs, al_rho = AL
+    st_cp, st_cs, st_rho = STEEL
+    # layer interface at grid z = n/2 (ROI index n/2 - PML - pad)
+    zi = n // 2 - PML - 1
+    z = np.arange(L)
+    top = (z < zi)[None, None, :]
+    cp_m = np.broadcast_to(np.where(top, al_cp, st_cp), (L, L, L)).astype(np.float32)
+    cs_m = np.broadcast_to(np.where(top, al_cs, st_cs), (L, L, L)).astype(np.float32)
+    rho_m = np.broadcast_to(np.where(top, al_rho, st_rho), (L, L, L)).astype(np.float32).copy()
+    for _ in range(n_incl):
+        ax = rng.uniform(0.5, 2.0, size=3) / DX
+        c = rng.uniform(0.15 * L, 0.85 * L, size=3)
+        s = np.float32(rng.uniform(0.5, 2.0))
+        lo = np.maximum(np.floor(c - ax).astype(int), 0)
+        hi = np.minimum(np.ceil(c + ax).astype(int) + 1, L)
+        gx, gy, gz = np.ogrid[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        inside = ((gx - c[0]) / ax[0]) ** 2 + ((gy - c[1]) / ax[1]) ** 2 + ((gz - c[2]) / ax[2]) ** 2 <= 1.0
+        blk = rho_m[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        blk[inside] *= s
+    w = L * DX
+    radius = min(0.125, 0.3 * w / 25.4)            # inches (simul_classes.py:548)
+    rec_pitch = min(rec_pitch, 0.8 * w / n_rec_side)
+    probes = [{"mono": {"coord_center": [w / 2, w / 2, 0.0], "radius": radius, "freq": 5.0, "bw": 0.5,
+                        "gain": 1.0, "emitters": "True", "receivers": "False", "t0_emission": 1.0,
+                        "t0_reception": 0.0}}]
+    off = (n_rec_side - 1) / 2 * rec_pitch
+    for i in range(n_rec_side):
+        for j in range(n_rec_side):
+            probes.append({"point": {"coord_center": [float(np.round(w / 2 - off + i * rec_pitch, 2)),
+                                                      float(np.round(w / 2 - off + j * rec_pitch, 2)), 0.0],
+                                     "freq": 5.0, "bw": 0.5, "gain": 1.0, "emitter": "False",
+                                     "receiver": "True", "t0_reception": 0.0}})
+    return {"roi": roi, "probes": probes,
+            "simul_params": {"time_steps": steps, "dt": cfl_dt(max(al_cp, st_cp), 3), "acc": 2, "it_display": 100},
+            "simul_configs": _configs(True),
+            "specimen_params": {"cp": al_cp, "cs": al_cs, "rho": al_rho, "cp_map": cp_m, "cs_map": cs_m,
+                                "rho_map": rho_m},
+            "attenuation": {"q_p": q, "q_s": q, "f_min": 2.0, "f_max": 8.0, "n_sls": 3}}

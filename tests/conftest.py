@@ -1,0 +1,36 @@
+import ast
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU and the built libvse.so")
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    # name the case before any GPU work, so a crash can be attributed
+    if request.node.get_closest_marker("gpu"):
+        print(f"\n[gpu-test] {request.node.nodeid}", flush=True)
+    yield
+
+
+def pedro_config(n_steps=None):
+    """The reference's teste_pedro configuration with its maps loaded from the fixture."""
+    d = np.load(os.path.join(GOLDEN, "pedro_teste_aco.npz"), allow_pickle=False)
+    cfg = ast.literal_eval(str(d["config"]))
+    for k in ("rho", "cp", "cs"):
+        cfg["specimen_params"][k + "_map"] = d[k]
+    cfg["simul_configs"]["results_dir"] = ""
+    if n_steps is not None:
+        cfg["simul_params"]["time_steps"] = n_steps
+    return cfg, d["bscan"]

@@ -1,0 +1,37 @@
+"""The C-ABI library loads and exports every entry point include/vse.h declares
+(no compute call: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+from taichi_sim_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "vse.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vse_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert set(header_symbols()) == set(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in header_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_abi_version_and_arg_validation_without_gpu():
+    lib = _lib.load()
+    assert lib.vse_abi_version() == 1
+    # argument validation happens before any device call
+    d = _lib.VseDesc()
+    d.ndim = 4
+    h = ctypes.c_void_p()
+    assert lib.vse_create(ctypes.byref(d), ctypes.byref(h)) == _lib.VSE_ERR_ARG
+    assert not h.value
+    assert lib.vse_last_error(None) == b"null handle"

@@ -1,0 +1,162 @@
+"""HIP path (libvse through its C-ABI) against the CPU oracle.
+
+Same setup arrays feed both; after N steps every wavefield, memory variable
+and receiver trace must match the FP32 oracle within 1e-5 relative L2 (the
+library keeps the reference's FP32 operation order, so the observed gap is
+usually zero).  The FP64 oracle gap is asserted alongside.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from taichi_sim_amd import workloads
+from taichi_sim_amd.backend import attenuation_from_config
+from taichi_sim_amd.engine import VSESim
+from taichi_sim_amd.sim_support import StabilityError
+from taichi_sim_amd.sim_support.attenuation import AttenuationCoefficients
+from taichi_sim_amd.sim_support.simulator import Simulator
+from taichi_sim_amd.sim_support.simulator3d import Simulator3D
+from tests.conftest import pedro_config
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _compare(sim, variant, steps, att=None, visco=None, check_f64=True):
+    o = oracle.Oracle(sim, variant=variant, att=att, visco=visco).run(steps)
+    g = VSESim.from_simulator(sim, variant=variant, att=att, visco=visco)
+    g.run(steps, check_every=64)
+    assert g.step() == steps
+    names = list(o.f)
+    worst = 0.0
+    for k in names:
+        ref = o.f[k]
+        got = g.field(k)
+        if np.abs(ref).max() == 0:
+            assert np.abs(got).max() == 0, k
+            continue
+        rel = oracle.rel_l2(got, ref)
+        worst = max(worst, rel)
+        assert rel <= TOL, (k, rel, np.abs(got - ref).max())
+    if g.visco:
+        for ci, comp in ([(0, "xx"), (1, "yy"), (3, "xy")] if g.ndim == 2 else
+                         list(enumerate(("xx", "yy", "zz", "xy", "xz", "yz")))):
+            ref = o.r[ci]
+            got = g.memory(comp)
+            if np.abs(ref).max() > 0:
+                assert oracle.rel_l2(got, ref) <= TOL, comp
+    tr = g.traces()
+    assert tr.shape == o.traces[:, :tr.shape[1]].shape
+    if np.abs(o.traces).max() > 0:
+        rel = oracle.rel_l2(tr, o.traces)
+        worst = max(worst, rel)
+        assert rel <= TOL, ("traces", rel)
+    if check_f64:
+        o64 = oracle.Oracle(sim, variant=variant, att=att, visco=visco, precision="f64").run(steps)
+        key = "syy" if g.ndim == 2 else "szz"
+        assert oracle.rel_l2(g.field(key), o64.f[key]) < 1e-4
+    print(f"worst rel-L2 vs FP32 oracle: {worst:.3e}")
+    g.close()
+    return worst
+
+
+@pytest.mark.parametrize("variant", ["wgsl", "taichi"])
+def test_2d_elastic_point(variant):
+    sim = Simulator(workloads.elastic_2d_point(n=96, steps=300, n_rec=16))
+    _compare(sim, variant, 300)
+
+
+def test_2d_visco_array_voids():
+    cfg = workloads.visco_2d_array(n=128, steps=300, n_elem=8, n_voids=3)
+    _compare(Simulator(cfg), "wgsl", 300, att=attenuation_from_config(cfg))
+
+
+def test_2d_visco_reference_tau_short():
+    # the reference's own tau (seconds vs dt in us: r alternates sign each step)
+    cfg = workloads.visco_2d_array(n=80, steps=60, n_elem=4, n_voids=1)
+    _compare(Simulator(cfg), "wgsl", 60, att=AttenuationCoefficients(), check_f64=False)
+
+
+@pytest.mark.parametrize("acc", [3, 4, 6])
+def test_2d_visco_ragged_high_order(acc):
+    cfg = workloads.visco_2d_array(n=101, steps=150, n_elem=6, n_voids=2)
+    cfg["roi"]["w_len"] += 7        # nx != ny, not multiples of the block
+    cfg["roi"]["width"] = cfg["roi"]["w_len"] * 0.1
+    cfg["simul_params"]["acc"] = acc
+    _compare(Simulator(cfg), "wgsl", 150, att=attenuation_from_config(cfg))
+
+
+def test_2d_partial_emitters_wgsl_stride():
+    # only some elements emit: WGSL reads the source with stride n_src (wgsl2d :67-71)
+    cfg = workloads.visco_2d_array(n=96, steps=120, n_elem=8, n_voids=0,
+                                   emitters=["True", "False", "True", "False", "False", "True", "False", "False"])
+    _compare(Simulator(cfg), "wgsl", 120, att=attenuation_from_config(cfg))
+
+
+@pytest.mark.parametrize("visco", [True, False])
+def test_3d_block(visco):
+    cfg = workloads.visco_3d_block(n=56, steps=80, n_incl=4, n_rec_side=4)
+    _compare(Simulator3D(cfg), "wgsl", 80, att=attenuation_from_config(cfg), visco=visco)
+
+
+def test_3d_ragged_order3():
+    cfg = workloads.visco_3d_block(n=50, steps=50, n_incl=2, n_rec_side=3)
+    cfg["roi"]["w_len"] += 5
+    cfg["roi"]["width"] = cfg["roi"]["w_len"] * 0.1
+    cfg["roi"]["d_len"] += 3
+    cfg["roi"]["depth"] = cfg["roi"]["d_len"] * 0.1
+    cfg["simul_params"]["acc"] = 3
+    _compare(Simulator3D(cfg), "wgsl", 50, att=attenuation_from_config(cfg))
+
+
+@pytest.mark.parametrize("variant", ["taichi", "wgsl"])
+def test_golden_bscan_teste_pedro(variant):
+    """The reference's B-scan (Taichi FP32 on a GPU), all 10000 steps."""
+    cfg, bscan = pedro_config()
+    sim = Simulator(cfg)
+    g = VSESim.from_simulator(sim, variant=variant)
+    g.run(sim._n_steps, check_every=1000)
+    tr = g.traces()
+    g.close()
+    assert np.nonzero(np.any(tr[1:] != 0, axis=1))[0][0] + 1 == 335
+    rel = oracle.rel_l2(tr[1:], bscan[1:])
+    print(f"{variant}: rel-L2 vs golden {rel:.3e}")
+    assert rel < 2e-5
+
+
+def test_stability_error_on_overflow():
+    cfg = workloads.elastic_2d_point(n=64, steps=200, n_rec=2)
+    sim = Simulator(cfg)
+    sim._source_term = (sim._source_term * np.float32(3e37)).astype(np.float32)
+    g = VSESim.from_simulator(sim, variant="wgsl")
+    with pytest.raises(StabilityError):
+        g.run(200, check_every=50)
+    g.close()
+
+
+def test_stability_error_on_nan():
+    # the reference's `vsn2 > 1e38` test misses NaN; the device max catches it
+    cfg = workloads.elastic_2d_point(n=64, steps=60, n_rec=2)
+    sim = Simulator(cfg)
+    st = sim._source_term.copy()
+    st[10] = np.nan
+    sim._source_term = st
+    g = VSESim.from_simulator(sim, variant="wgsl")
+    with pytest.raises(StabilityError):
+        g.run(60, check_every=20)
+    g.close()
+
+
+def test_reset_and_rerun_is_deterministic():
+    cfg = workloads.visco_2d_array(n=96, steps=100, n_elem=4, n_voids=1)
+    sim = Simulator(cfg)
+    g = VSESim.from_simulator(sim, variant="wgsl", att=attenuation_from_config(cfg))
+    g.run(100)
+    a = g.field("syy"), g.traces()
+    g.reset()
+    g.run(37)
+    g.run(63, check_every=7)      # different graph chunking, same result
+    b = g.field("syy"), g.traces()
+    g.close()
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])

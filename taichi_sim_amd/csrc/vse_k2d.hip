@@ -155,7 +155,8 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A) {
         if (rho > 0.f) {
           const int is = A.src_map[id];
           float vs = 0.f;
-          if (is != -1) vs = A.src[(long)(*A.step) * A.src_cols + is] * dt / rho;
+          const int s = *A.step;
+          if (is != -1 && s < A.src_rows) vs = A.src[(long)s * A.src_cols + is] * dt / rho;
           vyn += vs + (d1 + d2) * dt / rho;
         }
       } else {

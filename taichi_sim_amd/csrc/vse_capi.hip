@@ -17,8 +17,6 @@ namespace {
 
 // axis of the derivative each psi slot belongs to
 const int kSlotAxis[P_COUNT] = {0, 1, 2, 0, 1, 0, 2, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2};
-const bool kSlot2D[P_COUNT] = {true, true, false, true, true, false, false, false, false,
-                               true, false, false, true, true, false, false, false, false};
 
 }  // namespace
 
@@ -92,7 +90,12 @@ struct vse_sim {
     if (a == 1) return (long)d.nx * ax[1].nb * d.nz;
     return (long)d.nx * d.ny * ax[2].nb;
   }
-  bool slot_used(int slot) const { return d.ndim == 3 || kSlot2D[slot]; }
+  bool slot_used(int slot) const {
+    if (d.ndim == 3) return true;
+    for (int u : kPsiSlots2D)
+      if (u == slot) return true;
+    return false;
+  }
 };
 
 static int check_handle(vse_sim *s) { return s ? 0 : VSE_ERR_ARG; }
@@ -287,6 +290,21 @@ static int finalize(vse_sim *s) {
     int rc = s->alloc(&s->psi[slot], s->psi_elems(slot));
     if (rc) return rc;
   }
+  // every slot the kernels index must exist before any launch
+  for (int slot = 0; slot < P_COUNT; slot++)
+    if (s->slot_used(slot) && !s->psi[slot]) return s->fail(VSE_ERR_STATE, "psi slot not allocated");
+  for (int a = 0; a < s->d.ndim; a++)
+    for (int i = 0; i < 6; i++)
+      if (!s->prof[a][i]) return s->fail(VSE_ERR_STATE, "CPML profile not allocated");
+  for (int f = 0; f < 9; f++) {
+    const bool need = s->d.ndim == 3 || f == VSE_F_VX || f == VSE_F_VY || f == VSE_F_SXX || f == VSE_F_SYY ||
+                      f == VSE_F_SXY;
+    if (need && !s->field[f]) return s->fail(VSE_ERR_STATE, "field not allocated");
+  }
+  if (s->d.visco && s->d.variant == VSE_VARIANT_WGSL)
+    for (int c = 0; c < 6; c++)
+      if ((s->d.ndim == 3 || c == 0 || c == 1 || c == 3) && !s->r[c])
+        return s->fail(VSE_ERR_STATE, "memory variable not allocated");
   s->psi_ready = true;
   return s->hip(hipStreamSynchronize(s->stream), "finalize");
 }

@@ -23,6 +23,11 @@ enum {
   P_COUNT
 };
 
+// psi slots each kernel family touches (the library allocates exactly these
+// and checks them before any launch)
+static const int kPsiSlots2D[] = {P_DVX_DX, P_DVY_DY, P_DVY_DX, P_DVX_DY,
+                                  P_DSXX_DX, P_DSXY_DY, P_DSXY_DX, P_DSYY_DY};
+
 // CPML profiles of one axis.  Profile index j = cell - (ord-1) runs over
 // [0, np).  The profile is "active" (a != 0 or k != 1 on the full or half
 // grid) only for j < lo or j >= hi; there psi lives in a compact band of

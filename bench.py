@@ -42,7 +42,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--profile-steps", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU-oracle sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU-oracle sample (0: skip)")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only run the plain copy kernel (1 GiB, 5 launches): rocprof counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the shot/slab sharding "
@@ -50,6 +52,13 @@ def main():
         sys.exit(2)
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    if args.copy_only:
+        import ctypes
+        from taichi_sim_amd import _lib
+        gbs = ctypes.c_float(0.0)
+        rc = _lib.load().vse_copy_bandwidth(1 << 30, 5, ctypes.byref(gbs))
+        print(json.dumps({"copy_bytes_per_launch": 2 << 30, "launches": 6, "gbs": round(gbs.value, 1), "rc": rc}))
+        sys.exit(0 if rc == 0 else 1)
     from taichi_sim_amd import workloads
     from taichi_sim_amd.backend import attenuation_from_config
     from taichi_sim_amd.engine import VSESim
@@ -81,6 +90,13 @@ def main():
     step_frac = gcell * 1e9 * b_step / (HBM_PEAK_GBS * 1e9)
     g.close()
 
+    # achievable-HBM reference: a plain float4 copy of 2 x 2 GiB
+    import ctypes
+    from taichi_sim_amd import _lib
+    copy_gbs = ctypes.c_float(0.0)
+    if _lib.load().vse_copy_bandwidth(2 << 30, 20, ctypes.byref(copy_gbs)) != 0:
+        copy_gbs.value = float("nan")
+
     cpu = None
     if args.cpu_steps > 0:
         import oracle
@@ -105,6 +121,8 @@ def main():
                                f"Al over steel with 24 seeded inclusions, piston source, 32x32 receivers",
                    "grid": [args.n] * 3, "n_sls": 3, "fd_order": 2, "parallelism": "single GPU"},
         "hbm_frac_step": round(step_frac, 4),
+        "copy_kernel_gbs": round(copy_gbs.value, 1),
+        "hbm_frac_step_vs_copy": round(gcell * b_step / copy_gbs.value, 4),
         "bytes_per_cell_step": round(b_step, 2),
         "kernel_ms": {"stress": round(k_ms[0], 4), "velocity": round(k_ms[1], 4), "sparse": round(k_ms[2], 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

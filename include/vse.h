@@ -148,6 +148,13 @@ const char *vse_device_name(vse_sim *s);
 
 const char *vse_last_error(vse_sim *s);
 
+/* Device-to-device copy throughput (read + write bytes per second, in GB/s)
+ * of a plain float4 copy kernel over two `bytes`-sized buffers, `iters`
+ * launches on the current device.  No reference counterpart: the achievable-
+ * HBM point the roofline is read against, and the known-size kernel that
+ * calibrates rocprof's memory counters. */
+int vse_copy_bandwidth(size_t bytes, int iters, float *gbs);
+
 void vse_destroy(vse_sim *s);
 
 #ifdef __cplusplus

@@ -23,7 +23,7 @@ MEMORY = {"xx": 0, "yy": 1, "zz": 2, "xy": 3, "xz": 4, "yz": 5}
 EXPORTS = ["vse_abi_version", "vse_create", "vse_set_material", "vse_set_cpml", "vse_set_sources",
            "vse_set_receivers", "vse_reset", "vse_run", "vse_profile", "vse_read_field", "vse_read_traces",
            "vse_read_memory", "vse_read_v2max", "vse_get_step", "vse_field_device_ptr",
-           "vse_device_bytes", "vse_device_name", "vse_last_error", "vse_destroy"]
+           "vse_device_bytes", "vse_device_name", "vse_last_error", "vse_destroy", "vse_copy_bandwidth"]
 
 
 class VseDesc(ctypes.Structure):
@@ -75,6 +75,7 @@ def load():
         "vse_device_name": (ctypes.c_char_p, [P]),
         "vse_last_error": (ctypes.c_char_p, [P]),
         "vse_destroy": (None, [P]),
+        "vse_copy_bandwidth": (I, [ctypes.c_size_t, I, fp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -26,18 +26,22 @@ def cfl_dt(cp_max, ndim, dx=DX):
 
 
 def _roi(n_full, ndim, acc=2):
-    """ROI length giving a full grid of n_full cells: n = len + 2*PML + 2*pad."""
+    """ROI giving a full grid of n_full cells per axis (an int, or a tuple
+    (nx, nz) in 2D / (nx, ny, nz) in 3D): n = len + 2*PML + 2*pad."""
     pad = acc - 1
-    n_roi = n_full - 2 * PML - 2 * pad
-    roi = {"coord_ref": [[0.0, 0.0, 0.0]], "width": n_roi * DX, "w_len": n_roi,
-           "height": n_roi * DX, "h_len": n_roi, "depth": 0.0, "d_len": 1,
+    if np.isscalar(n_full):
+        n_full = (n_full,) * ndim
+    lens = [int(n) - 2 * PML - 2 * pad for n in n_full]
+    lx, lz = lens[0], lens[-1]
+    roi = {"coord_ref": [[0.0, 0.0, 0.0]], "width": lx * DX, "w_len": lx,
+           "height": lz * DX, "h_len": lz, "depth": 0.0, "d_len": 1,
            "len_pml_xmin": PML, "len_pml_xmax": PML, "len_pml_ymin": PML if ndim == 3 else 10,
            "len_pml_ymax": PML if ndim == 3 else 10, "len_pml_zmin": PML, "len_pml_zmax": PML,
            "rcoef_pml": 0.0001, "npower_pml": 2.0, "k_max_pml": 1.0}
     if ndim == 3:
-        roi["depth"] = n_roi * DX
-        roi["d_len"] = n_roi
-    return roi, n_roi
+        roi["depth"] = lens[1] * DX
+        roi["d_len"] = lens[1]
+    return roi, tuple(lens)
 
 
 def _configs(visco, **extra):
@@ -48,14 +52,14 @@ def _configs(visco, **extra):
 
 
 def elastic_2d_point(n=256, steps=1000, n_rec=128, material=AL):
-    roi, L = _roi(n, 2)
+    roi, (L, Lz) = _roi(n, 2)
     cp, cs, rho = material
-    w = L * DX
-    probes = [{"point": {"coord_center": [w / 2, 0.0, w / 2], "freq": 5.0, "bw": 0.5, "gain": 1.0,
+    w, h = L * DX, Lz * DX
+    probes = [{"point": {"coord_center": [w / 2, 0.0, h / 2], "freq": 5.0, "bw": 0.5, "gain": 1.0,
                          "emitter": "True", "receiver": "False", "t0_emission": 1.0}}]
     xs = np.linspace(0.1 * w, 0.8 * w, n_rec)
     for x in xs:
-        probes.append({"point": {"coord_center": [float(np.floor(x * 100) / 100), 0.0, float(np.floor(0.25 * w * 100) / 100)],
+        probes.append({"point": {"coord_center": [float(np.floor(x * 100) / 100), 0.0, float(np.floor(0.25 * h * 100) / 100)],
                                  "freq": 5.0, "bw": 0.5, "gain": 1.0, "emitter": "False",
                                  "receiver": "True", "t0_reception": 0.0}})
     return {"roi": roi, "probes": probes,
@@ -79,9 +83,9 @@ def _voids_2d(shape, n_voids, seed, margin_cells):
 
 def visco_2d_array(n=2048, steps=4000, n_elem=64, n_voids=16, seed=2, q=50.0, emitters="all",
                    t0_emission=None):
-    roi, L = _roi(n, 2)
+    roi, (L, Lz) = _roi(n, 2)
     cp, cs, rho = STEEL
-    mask = _voids_2d((L, L), n_voids, seed, margin_cells=min(20, L // 8))
+    mask = _voids_2d((L, Lz), n_voids, seed, margin_cells=min(20, min(L, Lz) // 8))
     cp_m = np.where(mask, 0.0, cp).astype(np.float32)
     cs_m = np.where(mask, 0.0, cs).astype(np.float32)
     rho_m = np.where(mask, 0.0, rho).astype(np.float32)
@@ -89,7 +93,7 @@ def visco_2d_array(n=2048, steps=4000, n_elem=64, n_voids=16, seed=2, q=50.0, em
     pitch = 0.5
     if t0_emission is None:
         rng = np.random.default_rng(seed + 100)
-        fx, fz = rng.uniform(0.3 * w, 0.7 * w), rng.uniform(0.3 * w, 0.7 * w)
+        fx, fz = rng.uniform(0.3 * w, 0.7 * w), rng.uniform(0.3 * Lz * DX, 0.7 * Lz * DX)
         ex = w / 2 + (np.arange(n_elem) - (n_elem - 1) / 2) * pitch
         tof = np.sqrt((ex - fx) ** 2 + fz ** 2) / cp
         t0_emission = [float(v) for v in (tof.max() - tof + 1.0)]
@@ -105,38 +109,40 @@ def visco_2d_array(n=2048, steps=4000, n_elem=64, n_voids=16, seed=2, q=50.0, em
 
 
 def visco_3d_block(n=512, steps=2000, n_incl=24, seed=3, q=50.0, n_rec_side=32, rec_pitch=1.0):
-    roi, L = _roi(n, 3)
+    roi, (L, Ly, Lz) = _roi(n, 3)
     rng = np.random.default_rng(seed)
     al_cp, al_cs, al_rho = AL
     st_cp, st_cs, st_rho = STEEL
-    # layer interface at grid z = n/2 (ROI index n/2 - PML - pad)
-    zi = n // 2 - PML - 1
-    z = np.arange(L)
+    # layer interface at grid z = nz/2 (ROI index nz/2 - PML - pad)
+    zi = (Lz + 2 * PML + 2) // 2 - PML - 1
+    z = np.arange(Lz)
     top = (z < zi)[None, None, :]
-    cp_m = np.broadcast_to(np.where(top, al_cp, st_cp), (L, L, L)).astype(np.float32)
-    cs_m = np.broadcast_to(np.where(top, al_cs, st_cs), (L, L, L)).astype(np.float32)
-    rho_m = np.broadcast_to(np.where(top, al_rho, st_rho), (L, L, L)).astype(np.float32).copy()
+    shape = (L, Ly, Lz)
+    cp_m = np.broadcast_to(np.where(top, al_cp, st_cp), shape).astype(np.float32)
+    cs_m = np.broadcast_to(np.where(top, al_cs, st_cs), shape).astype(np.float32)
+    rho_m = np.broadcast_to(np.where(top, al_rho, st_rho), shape).astype(np.float32).copy()
+    ext = np.array(shape)
     for _ in range(n_incl):
         ax = rng.uniform(0.5, 2.0, size=3) / DX
-        c = rng.uniform(0.15 * L, 0.85 * L, size=3)
+        c = rng.uniform(0.15, 0.85, size=3) * ext
         s = np.float32(rng.uniform(0.5, 2.0))
         lo = np.maximum(np.floor(c - ax).astype(int), 0)
-        hi = np.minimum(np.ceil(c + ax).astype(int) + 1, L)
+        hi = np.minimum(np.ceil(c + ax).astype(int) + 1, ext)
         gx, gy, gz = np.ogrid[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
         inside = ((gx - c[0]) / ax[0]) ** 2 + ((gy - c[1]) / ax[1]) ** 2 + ((gz - c[2]) / ax[2]) ** 2 <= 1.0
         blk = rho_m[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
         blk[inside] *= s
-    w = L * DX
-    radius = min(0.125, 0.3 * w / 25.4)            # inches (simul_classes.py:548)
-    rec_pitch = min(rec_pitch, 0.8 * w / n_rec_side)
-    probes = [{"mono": {"coord_center": [w / 2, w / 2, 0.0], "radius": radius, "freq": 5.0, "bw": 0.5,
+    w, d = L * DX, Ly * DX
+    radius = min(0.125, 0.3 * min(w, d) / 25.4)            # inches (simul_classes.py:548)
+    rec_pitch = min(rec_pitch, 0.8 * min(w, d) / n_rec_side)
+    probes = [{"mono": {"coord_center": [w / 2, d / 2, 0.0], "radius": radius, "freq": 5.0, "bw": 0.5,
                         "gain": 1.0, "emitters": "True", "receivers": "False", "t0_emission": 1.0,
                         "t0_reception": 0.0}}]
     off = (n_rec_side - 1) / 2 * rec_pitch
     for i in range(n_rec_side):
         for j in range(n_rec_side):
             probes.append({"point": {"coord_center": [float(np.round(w / 2 - off + i * rec_pitch, 2)),
-                                                      float(np.round(w / 2 - off + j * rec_pitch, 2)), 0.0],
+                                                      float(np.round(d / 2 - off + j * rec_pitch, 2)), 0.0],
                                      "freq": 5.0, "bw": 0.5, "gain": 1.0, "emitter": "False",
                                      "receiver": "True", "t0_reception": 0.0}})
     return {"roi": roi, "probes": probes,

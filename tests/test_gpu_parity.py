@@ -79,9 +79,8 @@ def test_2d_visco_reference_tau_short():
 
 @pytest.mark.parametrize("acc", [3, 4, 6])
 def test_2d_visco_ragged_high_order(acc):
-    cfg = workloads.visco_2d_array(n=101, steps=150, n_elem=6, n_voids=2)
-    cfg["roi"]["w_len"] += 7        # nx != ny, not multiples of the block
-    cfg["roi"]["width"] = cfg["roi"]["w_len"] * 0.1
+    # nx != ny, neither a multiple of the block shape
+    cfg = workloads.visco_2d_array(n=(108, 101), steps=150, n_elem=6, n_voids=2)
     cfg["simul_params"]["acc"] = acc
     _compare(Simulator(cfg), "wgsl", 150, att=attenuation_from_config(cfg))
 
@@ -100,11 +99,7 @@ def test_3d_block(visco):
 
 
 def test_3d_ragged_order3():
-    cfg = workloads.visco_3d_block(n=50, steps=50, n_incl=2, n_rec_side=3)
-    cfg["roi"]["w_len"] += 5
-    cfg["roi"]["width"] = cfg["roi"]["w_len"] * 0.1
-    cfg["roi"]["d_len"] += 3
-    cfg["roi"]["depth"] = cfg["roi"]["d_len"] * 0.1
+    cfg = workloads.visco_3d_block(n=(55, 53, 50), steps=50, n_incl=2, n_rec_side=3)
     cfg["simul_params"]["acc"] = 3
     _compare(Simulator3D(cfg), "wgsl", 50, att=attenuation_from_config(cfg))
 

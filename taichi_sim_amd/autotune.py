@@ -1,0 +1,70 @@
+"""Autotuner for the 3D kernels' launch shapes.
+
+For one simulation handle it times every candidate (tile height ty, non-
+temporal streaming nt, x-chunk length) of the stress and velocity passes with
+HIP events (``VSESim.profile``), keeps the fastest per pass, and caches the
+choice per problem key (ndim, grid, FD order, visco, L) in a JSON file.  All
+shapes compute identical results, so tuning only changes speed.  The handle's
+state advances while tuning; call ``reset()`` before a measured run.
+"""
+import json
+import os
+
+DEFAULT_CACHE = os.environ.get("VSE_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                            "tuning_cache.json"))
+TY = (1, 2, 4, 8)
+NT = (0, 1)
+XCHUNK = (1, 2, 4, 8, 16, 32, 64)
+
+
+def problem_key(g):
+    return f"{g.ndim}d:{'x'.join(map(str, g.shape))}:ord{g.ord}:visco{int(g.visco)}:L{g.n_sls}"
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def tune(g, steps=3, candidates=None, cache=DEFAULT_CACHE, verbose=False, use_cache=True):
+    """Pick and apply the fastest launch shape of each 3D pass of ``g``.
+
+    Returns {"stress": (ty, nt, xchunk, ms), "velocity": (...), "table": [...]}.
+    """
+    if g.ndim != 3:
+        return {}
+    key = problem_key(g)
+    db = _load(cache) if cache else {}
+    if use_cache and key in db:
+        best = db[key]
+        for k, name in enumerate(("stress", "velocity")):
+            ty, nt, xc, _ = best[name]
+            g.set_tuning(k, ty, nt, xc)
+        return best
+    cands = candidates or [(ty, nt, xc) for ty in TY for nt in NT for xc in XCHUNK]
+    table = []
+    best = {}
+    for k, name in enumerate(("stress", "velocity")):
+        results = []
+        for ty, nt, xc in cands:
+            g.set_tuning(k, ty, nt, xc)
+            ms = g.profile(steps)[k]
+            results.append((ms, ty, nt, xc))
+            table.append((name, ty, nt, xc, ms))
+            if verbose:
+                print(f"  {name:8s} ty={ty} nt={nt} xchunk={xc:3d}: {ms:.3f} ms", flush=True)
+        ms, ty, nt, xc = min(results)
+        g.set_tuning(k, ty, nt, xc)
+        best[name] = (ty, nt, xc, ms)
+    if cache:
+        db[key] = best
+        try:
+            with open(cache, "w") as f:
+                json.dump(db, f, indent=1, sort_keys=True)
+        except OSError:
+            pass
+    best["table"] = table
+    return best

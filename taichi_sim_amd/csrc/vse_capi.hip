@@ -50,6 +50,8 @@ struct vse_sim {
   int n_rec = 0, rec_gate = 0;
   int *rec_beg = nullptr, *rec_cell = nullptr, *rec_delay = nullptr;
   float *traces = nullptr;
+  // launch shapes of the 3D stress / velocity kernels
+  KTune tune[2] = {{4, 0, 1}, {4, 0, 1}};
   // cached graph
   hipGraphExec_t gexec = nullptr;
   int gexec_steps = 0;
@@ -365,9 +367,9 @@ static hipError_t enqueue_step(vse_sim *s, const StepArgs &A, const SparseArgs &
     if (e != hipSuccess) return e;
     e = vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->stream);
   } else {
-    e = vse_launch_stress3d(d.visco, d.ord, A, s->stream);
+    e = vse_launch_stress3d(d.visco, d.ord, A, s->tune[0], s->stream);
     if (e != hipSuccess) return e;
-    e = vse_launch_velocity3d(d.visco, d.ord, A, s->stream);
+    e = vse_launch_velocity3d(d.visco, d.ord, A, s->tune[1], s->stream);
   }
   if (e != hipSuccess) return e;
   return vse_launch_sparse(S, s->stream);
@@ -464,10 +466,10 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
   for (int it = 0; it < n_steps && !rc; it++) {
     hipError_t e = hipEventRecord(ev[0], s->stream);
     if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_stress2d(d.variant, d.visco, d.ord, A, s->stream)
-                                         : vse_launch_stress3d(d.visco, d.ord, A, s->stream);
+                                         : vse_launch_stress3d(d.visco, d.ord, A, s->tune[0], s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[1], s->stream);
     if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->stream)
-                                         : vse_launch_velocity3d(d.visco, d.ord, A, s->stream);
+                                         : vse_launch_velocity3d(d.visco, d.ord, A, s->tune[1], s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
     if (e == hipSuccess) e = vse_launch_sparse(S, s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[3], s->stream);
@@ -485,6 +487,15 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
     for (int k = 0; k < 3; k++) ms[k] = (float)(acc[k] / n_steps);
   if (!rc) rc = check_stable(s);
   return rc;
+}
+
+int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk) {
+  if (check_handle(s) || kernel < 0 || kernel > 1) return VSE_ERR_ARG;
+  if (!(ty == 1 || ty == 2 || ty == 4 || ty == 8) || (nt != 0 && nt != 1) || xchunk < 1)
+    return s->fail(VSE_ERR_ARG, "tuning: ty in {1,2,4,8}, nt in {0,1}, xchunk >= 1");
+  s->tune[kernel] = KTune{ty, nt, xchunk};
+  s->drop_graph();
+  return VSE_OK;
 }
 
 int vse_read_field(vse_sim *s, int field, float *out) {

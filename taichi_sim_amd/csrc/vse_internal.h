@@ -90,9 +90,16 @@ struct SparseArgs {
   int trace_rows;
 };
 
+// Launch shape of one 3D kernel (chosen by the autotuner): tile height in y
+// (the tile is 64 cells wide in z), non-temporal streaming on/off, and the
+// number of x planes each workgroup marches through.
+struct KTune {
+  int ty, nt, xchunk;
+};
+
 // launchers (vse_k2d.hip / vse_k3d.hip)
 hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st);
 hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st);
-hipError_t vse_launch_stress3d(int visco, int ord, const StepArgs &A, hipStream_t st);
-hipError_t vse_launch_velocity3d(int visco, int ord, const StepArgs &A, hipStream_t st);
+hipError_t vse_launch_stress3d(int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st);
+hipError_t vse_launch_velocity3d(int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st);
 hipError_t vse_launch_sparse(const SparseArgs &S, hipStream_t st);

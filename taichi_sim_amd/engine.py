@@ -193,6 +193,10 @@ class VSESim:
         self._check(self._lib.vse_profile(self._h, int(n_steps), ms))
         return tuple(float(v) for v in ms)
 
+    def set_tuning(self, kernel, ty, nt, xchunk):
+        """Launch shape of the 3D stress (kernel 0) or velocity (1) pass."""
+        self._check(self._lib.vse_set_tuning(self._h, int(kernel), int(ty), int(nt), int(xchunk)))
+
     def band_cells(self):
         """Cells per psi slot-axis band: {axis: band width} from the CPML profiles."""
         return dict(self._bands)

@@ -98,6 +98,29 @@ def test_3d_block(visco):
     _compare(Simulator3D(cfg), "wgsl", 80, att=attenuation_from_config(cfg), visco=visco)
 
 
+def test_3d_launch_shapes_bit_identical():
+    """Every 3D launch shape the autotuner may pick gives the same bits."""
+    cfg = workloads.visco_3d_block(n=(70, 60, 130), steps=40, n_incl=3, n_rec_side=3)
+    sim = Simulator3D(cfg)
+    att = attenuation_from_config(cfg)
+    o = oracle.Oracle(sim, variant="wgsl", att=att).run(40)
+    g = VSESim.from_simulator(sim, variant="wgsl", att=att)
+    ref = None
+    for ty, nt, xc in [(4, 0, 1), (1, 0, 7), (2, 1, 16), (8, 1, 3), (4, 1, 64), (8, 0, 1000)]:
+        g.set_tuning(0, ty, nt, xc)
+        g.set_tuning(1, ty, 1 - nt, max(1, xc // 2))
+        g.reset()
+        g.run(40)
+        out = [g.field(k) for k in ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")] + [g.traces()]
+        if ref is None:
+            ref = out
+            assert oracle.rel_l2(out[5], o.f["szz"]) <= TOL
+        else:
+            for a, b in zip(ref, out):
+                np.testing.assert_array_equal(a, b)
+    g.close()
+
+
 def test_3d_ragged_order3():
     cfg = workloads.visco_3d_block(n=(55, 53, 50), steps=50, n_incl=2, n_rec_side=3)
     cfg["simul_params"]["acc"] = 3

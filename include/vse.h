@@ -123,12 +123,13 @@ int vse_profile(vse_sim *s, int n_steps, float *ms);
 /* Launch shape of a 3D kernel (kernel 0 = stress, 1 = velocity): workgroup
  * tile of 64 (z) x ty (y) cells, ty in {1,2,4,8}; nt = 1 streams the fields
  * touched only at the own cell with non-temporal loads/stores; each
- * workgroup marches through xchunk x-planes (1 = one cell per thread).
- * Results are identical for every shape; only speed changes.  Picked by the
+ * workgroup marches through xchunk x-planes (1 = one cell per thread);
+ * remap = 1 gives each XCD a contiguous block of tiles instead of the
+ * round-robin dispatch order.  Results are identical for every shape; only speed changes.  Picked by the
  * autotuner (taichi_sim_amd/autotune.py); no reference counterpart (the
  * reference fixes its workgroup to gcd(n, 16) per axis,
  * sim_webgpu_3D_viscoelastic.py:40-42). */
-int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk);
+int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap);
 
 /* Copy a field (VSE_F_*) to host, n cells in the reference layout.
  * Replaces the sigma_yy / sigma_zz read-back (sim_webgpu_viscoelastic.py:540-541). */

@@ -65,7 +65,7 @@ def load():
         "vse_reset": (I, [P]),
         "vse_run": (I, [P, I, I, fp]),
         "vse_profile": (I, [P, I, fp]),
-        "vse_set_tuning": (I, [P, I, I, I, I]),
+        "vse_set_tuning": (I, [P, I, I, I, I, I]),
         "vse_read_field": (I, [P, I, fp]),
         "vse_read_traces": (I, [P, fp]),
         "vse_read_memory": (I, [P, I, fp]),

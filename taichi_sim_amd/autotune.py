@@ -14,7 +14,8 @@ DEFAULT_CACHE = os.environ.get("VSE_TUNE_CACHE", os.path.join(os.path.dirname(os
                                                             "tuning_cache.json"))
 TY = (1, 2, 4, 8)
 NT = (0, 1)
-XCHUNK = (1, 2, 4, 8, 16, 32, 64)
+XCHUNK = (1, 2, 4, 8, 16, 32)
+REMAP = (0, 1)
 
 
 def problem_key(g):
@@ -32,7 +33,7 @@ def _load(path):
 def tune(g, steps=3, candidates=None, cache=DEFAULT_CACHE, verbose=False, use_cache=True):
     """Pick and apply the fastest launch shape of each 3D pass of ``g``.
 
-    Returns {"stress": (ty, nt, xchunk, ms), "velocity": (...), "table": [...]}.
+    Returns {"stress": (ty, nt, xchunk, remap, ms), "velocity": (...), "table": [...]}.
     """
     if g.ndim != 3:
         return {}
@@ -41,24 +42,24 @@ def tune(g, steps=3, candidates=None, cache=DEFAULT_CACHE, verbose=False, use_ca
     if use_cache and key in db:
         best = db[key]
         for k, name in enumerate(("stress", "velocity")):
-            ty, nt, xc, _ = best[name]
-            g.set_tuning(k, ty, nt, xc)
+            ty, nt, xc, rm, _ = best[name]
+            g.set_tuning(k, ty, nt, xc, rm)
         return best
-    cands = candidates or [(ty, nt, xc) for ty in TY for nt in NT for xc in XCHUNK]
+    cands = candidates or [(ty, nt, xc, rm) for ty in TY for nt in NT for xc in XCHUNK for rm in REMAP]
     table = []
     best = {}
     for k, name in enumerate(("stress", "velocity")):
         results = []
-        for ty, nt, xc in cands:
-            g.set_tuning(k, ty, nt, xc)
+        for ty, nt, xc, rm in cands:
+            g.set_tuning(k, ty, nt, xc, rm)
             ms = g.profile(steps)[k]
-            results.append((ms, ty, nt, xc))
-            table.append((name, ty, nt, xc, ms))
+            results.append((ms, ty, nt, xc, rm))
+            table.append((name, ty, nt, xc, rm, ms))
             if verbose:
-                print(f"  {name:8s} ty={ty} nt={nt} xchunk={xc:3d}: {ms:.3f} ms", flush=True)
-        ms, ty, nt, xc = min(results)
-        g.set_tuning(k, ty, nt, xc)
-        best[name] = (ty, nt, xc, ms)
+                print(f"  {name:8s} ty={ty} nt={nt} xchunk={xc:3d} remap={rm}: {ms:.3f} ms", flush=True)
+        ms, ty, nt, xc, rm = min(results)
+        g.set_tuning(k, ty, nt, xc, rm)
+        best[name] = (ty, nt, xc, rm, ms)
     if cache:
         db[key] = best
         try:

@@ -92,9 +92,10 @@ struct SparseArgs {
 
 // Launch shape of one 3D kernel (chosen by the autotuner): tile height in y
 // (the tile is 64 cells wide in z), non-temporal streaming on/off, and the
-// number of x planes each workgroup marches through.
+// number of x planes each workgroup marches through, and whether tiles are
+// remapped so each XCD owns a contiguous block of them.
 struct KTune {
-  int ty, nt, xchunk;
+  int ty, nt, xchunk, remap;
 };
 
 // launchers (vse_k2d.hip / vse_k3d.hip)

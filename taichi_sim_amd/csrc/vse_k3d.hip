@@ -24,27 +24,17 @@ namespace {
 
 constexpr int BZ = 64;
 
-// Streamed accesses: non-temporal when NT (bypass-friendly, keeps L2 for the
-// stencil planes), ordinary otherwise; the autotuner picks.
-template <bool NT, typename T>
-__device__ __forceinline__ T sload(const T *p) {
-  if constexpr (NT) return __builtin_nontemporal_load(p);
-  else return *p;
-}
-template <bool NT, typename T>
-__device__ __forceinline__ void sstore(T v, T *p) {
-  if constexpr (NT) __builtin_nontemporal_store(v, p);
-  else *p = v;
-}
-
-// Tile assignment: linear dispatch order bid lands on XCD bid % 8; give each
-// XCD a contiguous run of (z-tile fastest, y-tile, x-chunk) tiles.
+// Tile assignment.  Tiles are ordered z-tile fastest, then y-tile, then
+// x-chunk.  remap = 0 dispatches them in that order (consecutive workgroups,
+// which land on XCDs round-robin, work on neighbouring tiles of one plane);
+// remap = 1 gives each XCD (dispatch bid lands on XCD bid % 8) a contiguous
+// eighth of the tiles.
 struct Tile { int bz, by, bxc; };
-__device__ __forceinline__ Tile tile_of_block(int nzb, int nyb, int nxc) {
+__device__ __forceinline__ Tile tile_of_block(int nzb, int nyb, int nxc, int remap) {
   const int bid = blockIdx.x;
   const int total = nzb * nyb * nxc;
   int t = bid;
-  if ((total & 7) == 0) t = (bid & 7) * (total >> 3) + (bid >> 3);
+  if (remap && (total & 7) == 0) t = (bid & 7) * (total >> 3) + (bid >> 3);
   Tile r;
   r.bz = t % nzb;
   r.by = (t / nzb) % nyb;
@@ -52,57 +42,91 @@ __device__ __forceinline__ Tile tile_of_block(int nzb, int nyb, int nxc) {
   return r;
 }
 
-__device__ __forceinline__ float ld_plane(const float *f, long sx, int x, int nx, long col) {
-  return (x >= 0 && x < nx) ? f[x * sx + col] : 0.f;
+// One x-plane of a field as a raw buffer: a scalar (SGPR) resource
+// descriptor over the plane's bytes, addressed with a 32-bit per-lane offset.
+// The many loads in flight then cost one VGPR of address each (flat 64-bit
+// addresses cost two plus the add chains), and the hardware range-checks
+// every access against the plane (out-of-range reads return 0, writes drop).
+// Word 3 0x00020000 is the gfx9-family raw 32-bit buffer format.  CP = 2 sets
+// the non-temporal cache policy for pure streams.
+struct PB {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ PB(const float *plane_base, unsigned nbytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc((void *)plane_base, (short)0, (int)nbytes, 0x00020000)) {}
+  template <int CP = 0>
+  __device__ __forceinline__ float ld(unsigned i) const {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4u), 0, CP));
+  }
+  template <int CP = 0>
+  __device__ __forceinline__ void st(unsigned i, float v) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)(i * 4u), 0, CP);
+  }
+};
+
+// psi recursion psi <- b psi + a D, returns D/k + psi (wgsl3d :1372-1378)
+template <int CP>
+__device__ __forceinline__ float stretch(const float *a, const float *b, const float *k, const PB &P, unsigned i,
+                                         int j, float D) {
+  const float pn = b[j] * P.ld<CP>(i) + a[j] * D;
+  P.st<CP>(i, pn);
+  return D / k[j] + pn;
 }
 
 template <bool VISCO, int ORD, int TY, bool NT>
-__global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, int nzb, int nyb, int nxc) {
-  const Tile T = tile_of_block(nzb, nyb, nxc);
+__global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, int nzb, int nyb, int nxc, int remap) {
+  constexpr int CP = NT ? 2 : 0;
+  const Tile T = tile_of_block(nzb, nyb, nxc, remap);
   const int z = T.bz * BZ + threadIdx.x;
   const int y = T.by * TY + threadIdx.y;
   const int nx = A.nx, ny = A.ny, nz = A.nz;
   if (z >= nz || y >= ny) return;
   constexpr int last = ORD - 1, off = ORD - 1;
-  const long sy = nz, sx = (long)ny * nz;
-  const long col = y * sy + z;
+  const unsigned sy = nz;
+  const long sx = (long)ny * nz;
+  const unsigned pbytes = (unsigned)(sx * 4);
+  const unsigned col = y * sy + z;
   const float dt = A.dt;
   const int x0 = T.bxc * xchunk, x1 = min(nx, x0 + xchunk);
   const int jy = y - off, jz = z - off;
   const int by = band_index(A.ax[1], jy), bz = band_index(A.ax[2], jz);
   const int nby = A.ax[1].nb, nbz = A.ax[2].nb;
+  const unsigned py_col = by * nz + z;      // offset in a psi_y plane (nby * nz per x)
+  const unsigned pz_col = y * nbz + bz;     // offset in a psi_z plane (ny * nbz per x)
+  const unsigned pyb = (unsigned)nby * nz * 4u, pzb = (unsigned)ny * nbz * 4u;
   const bool yp = y >= last && y < ny - last - 1;
   const bool zp = z >= last && z < nz - last - 1;
   const bool ym = y >= last + 1 && y < ny - last;
   const bool zm = z >= last + 1 && z < nz - last;
+  auto P = [&](const float *f, int x) { return PB(f + (long)(x < 0 ? 0 : (x >= nx ? nx - 1 : x)) * sx, pbytes); };
 
   // x windows of the own column: vx at x-(ORD-1)..x+ORD, vy/vz at x-ORD..x+ORD-1
+  // (planes outside [0,nx) read clamped values that no update uses)
   float wvx[2 * ORD], wvy[2 * ORD], wvz[2 * ORD];
 #pragma unroll
   for (int i = 0; i < 2 * ORD - 1; i++) {
-    wvx[i + 1] = ld_plane(A.vx, sx, x0 - (ORD - 1) + i, nx, col);
-    wvy[i + 1] = ld_plane(A.vy, sx, x0 - ORD + i, nx, col);
-    wvz[i + 1] = ld_plane(A.vz, sx, x0 - ORD + i, nx, col);
+    wvx[i + 1] = P(A.vx, x0 - (ORD - 1) + i).ld(col);
+    wvy[i + 1] = P(A.vy, x0 - ORD + i).ld(col);
+    wvz[i + 1] = P(A.vz, x0 - ORD + i).ld(col);
   }
-  float rho_n = ld_plane(A.rho, sx, x0, nx, col), cs_n = ld_plane(A.cs, sx, x0, nx, col);
+  float rho_n = P(A.rho, x0).ld(col), cs_n = P(A.cs, x0).ld(col);
 
+#pragma unroll 1
   for (int x = x0; x < x1; x++) {
 #pragma unroll
     for (int i = 0; i < 2 * ORD - 1; i++) { wvx[i] = wvx[i + 1]; wvy[i] = wvy[i + 1]; wvz[i] = wvz[i + 1]; }
-    wvx[2 * ORD - 1] = ld_plane(A.vx, sx, x + ORD, nx, col);
-    wvy[2 * ORD - 1] = ld_plane(A.vy, sx, x + ORD - 1, nx, col);
-    wvz[2 * ORD - 1] = ld_plane(A.vz, sx, x + ORD - 1, nx, col);
+    wvx[2 * ORD - 1] = P(A.vx, x + ORD).ld(col);
+    wvy[2 * ORD - 1] = P(A.vy, x + ORD - 1).ld(col);
+    wvz[2 * ORD - 1] = P(A.vz, x + ORD - 1).ld(col);
     const float rho0 = rho_n, cs0 = cs_n;
-    rho_n = ld_plane(A.rho, sx, x + 1, nx, col);
-    cs_n = ld_plane(A.cs, sx, x + 1, nx, col);
-    const long id = x * sx + col;
+    const PB rho1P = P(A.rho, x + 1), cs1P = P(A.cs, x + 1);
+    rho_n = rho1P.ld(col);
+    cs_n = cs1P.ld(col);
     const int jx = x - off;
     const int bx = band_index(A.ax[0], jx);
-    const long px = (long)bx * sx + col;
-    const long py = (long)x * nby * nz + (long)by * nz + z;
-    const long pz = ((long)x * ny + y) * nbz + bz;
     const bool xp = x >= last && x < nx - last - 1;
     const bool xm = x >= last + 1 && x < nx - last;
+    const PB vxP = P(A.vx, x), vyP = P(A.vy, x), vzP = P(A.vz, x), rhoP = P(A.rho, x), csP = P(A.cs, x);
+    const long rofs = (long)x * sx;
 
     // ---- normal stresses (wgsl3d :1356-1453): x D+, y D-, z D-
     if (xp && ym && zm) {
@@ -110,52 +134,38 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
 #pragma unroll
       for (int c = 0; c < ORD; c++) {
         dvx += A.C[c] * (wvx[ORD + c] - wvx[ORD - 1 - c]) / A.dx;            // vx[x+c+1] - vx[x-c]
-        dvy += A.C[c] * (A.vy[id + c * sy] - A.vy[id - (c + 1) * sy]) / A.dy;
-        dvz += A.C[c] * (A.vz[id + c] - A.vz[id - c - 1]) / A.dz;
+        dvy += A.C[c] * (vyP.ld(col + c * sy) - vyP.ld(col - (c + 1) * sy)) / A.dy;
+        dvz += A.C[c] * (vzP.ld(col + c) - vzP.ld(col - c - 1)) / A.dz;
       }
-      if (bx >= 0) {
-        float *p = A.psi[P_DVX_DX] + px;
-        const float pn = A.ax[0].bh[jx] * sload<NT>(p) + A.ax[0].ah[jx] * dvx;
-        sstore<NT>(pn, p);
-        dvx = dvx / A.ax[0].kh[jx] + pn;
-      }
-      if (by >= 0) {
-        float *p = A.psi[P_DVY_DY] + py;
-        const float pn = A.ax[1].b[jy] * sload<NT>(p) + A.ax[1].a[jy] * dvy;
-        sstore<NT>(pn, p);
-        dvy = dvy / A.ax[1].k[jy] + pn;
-      }
-      if (bz >= 0) {
-        float *p = A.psi[P_DVZ_DZ] + pz;
-        const float pn = A.ax[2].b[jz] * sload<NT>(p) + A.ax[2].a[jz] * dvz;
-        sstore<NT>(pn, p);
-        dvz = dvz / A.ax[2].k[jz] + pn;
-      }
-      const float cp = sload<NT>(A.cp + id);
+      if (bx >= 0) dvx = stretch<CP>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, PB(A.psi[P_DVX_DX] + (long)bx * sx, pbytes), col, jx, dvx);
+      if (by >= 0) dvy = stretch<CP>(A.ax[1].a, A.ax[1].b, A.ax[1].k, PB(A.psi[P_DVY_DY] + (long)x * nby * nz, pyb), py_col, jy, dvy);
+      if (bz >= 0) dvz = stretch<CP>(A.ax[2].a, A.ax[2].b, A.ax[2].k, PB(A.psi[P_DVZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, dvz);
+      const float cp = PB(A.cp + rofs, pbytes).ld<CP>(col);
       const float lambda = rho0 * (cp * cp - 2.0f * cs0 * cs0);
       const float mu = rho0 * (cs0 * cs0);
       const float lp2m = lambda + 2.0f * mu;
-      const float sxx = sload<NT>(A.sxx + id), syy = sload<NT>(A.syy + id), szz = sload<NT>(A.szz + id);
+      const PB sxxP(A.sxx + rofs, pbytes), syyP(A.syy + rofs, pbytes), szzP(A.szz + rofs, pbytes);
       if constexpr (VISCO) {
         float Sxx = 0.f, Syy = 0.f, Szz = 0.f;
         const float div = dvx + dvy + dvz;
         for (int l = 0; l < A.L; l++) {
-          float *pxx = A.r[0] + l * A.ncell + id, *pyy = A.r[1] + l * A.ncell + id, *pzz = A.r[2] + l * A.ncell + id;
-          const float oxx = sload<NT>(pxx), oyy = sload<NT>(pyy), ozz = sload<NT>(pzz);
+          const long lo = (long)l * A.ncell + rofs;
+          const PB pxx(A.r[0] + lo, pbytes), pyy(A.r[1] + lo, pbytes), pzz(A.r[2] + lo, pbytes);
+          const float oxx = pxx.ld<CP>(col), oyy = pyy.ld<CP>(col), ozz = pzz.ld<CP>(col);
           const float nxx = ((oxx + div * A.zk.phi_p[l] - oxx * A.zk.h_p[l]) * A.zk.m_p[l]);
           const float nyy = ((oyy + (dvx - div / 3.0f) * A.zk.phi_s[l] - oyy * A.zk.h_s[l]) * A.zk.m_s[l]);
           const float nzz = ((ozz + (dvy - div / 3.0f) * A.zk.phi_s[l] - ozz * A.zk.h_s[l]) * A.zk.m_s[l]);
           Sxx += oxx + nxx; Syy += oyy + nyy; Szz += ozz + nzz;
-          sstore<NT>(nxx, pxx); sstore<NT>(nyy, pyy); sstore<NT>(nzz, pzz);
+          pxx.st<CP>(col, nxx); pyy.st<CP>(col, nyy); pzz.st<CP>(col, nzz);
         }
         const float l23 = lambda + 2.0f * mu / 3.0f;
-        sstore<NT>(sxx + (lp2m * dvx + lambda * (dvy + dvz) + l23 * 0.5f * Sxx + mu * Syy) * dt, A.sxx + id);
-        sstore<NT>(syy + (lambda * (dvx + dvz) + lp2m * dvy + l23 * 0.5f * Sxx + mu * Szz) * dt, A.syy + id);
-        sstore<NT>(szz + (lambda * (dvx + dvy) + lp2m * dvz + lp2m * 0.5f * Sxx - (mu / 3.0f) * (Syy + Szz)) * dt, A.szz + id);
+        sxxP.st<CP>(col, sxxP.ld<CP>(col) + (lp2m * dvx + lambda * (dvy + dvz) + l23 * 0.5f * Sxx + mu * Syy) * dt);
+        syyP.st<CP>(col, syyP.ld<CP>(col) + (lambda * (dvx + dvz) + lp2m * dvy + l23 * 0.5f * Sxx + mu * Szz) * dt);
+        szzP.st<CP>(col, szzP.ld<CP>(col) + (lambda * (dvx + dvy) + lp2m * dvz + lp2m * 0.5f * Sxx - (mu / 3.0f) * (Syy + Szz)) * dt);
       } else {
-        sstore<NT>(sxx + (lp2m * dvx + lambda * (dvy + dvz)) * dt, A.sxx + id);
-        sstore<NT>(syy + (lambda * (dvx + dvz) + lp2m * dvy) * dt, A.syy + id);
-        sstore<NT>(szz + (lambda * (dvx + dvy) + lp2m * dvz) * dt, A.szz + id);
+        sxxP.st<CP>(col, sxxP.ld<CP>(col) + (lp2m * dvx + lambda * (dvy + dvz)) * dt);
+        syyP.st<CP>(col, syyP.ld<CP>(col) + (lambda * (dvx + dvz) + lp2m * dvy) * dt);
+        szzP.st<CP>(col, szzP.ld<CP>(col) + (lambda * (dvx + dvy) + lp2m * dvz) * dt);
       }
     }
 
@@ -165,108 +175,85 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
       const bool ok = (sh < 2 ? xm : xp) && yp && zp;
       if (!ok) continue;
       float d1 = 0.f, d2 = 0.f, rho4, cs4;
-      float *sig, *rr;
+      float *sig;
+      float *rr;
       if (sh == 0) {            // sigma_xy: dvy/dx (D-), dvx/dy (D+)
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
           d1 += A.C[c] * (wvy[ORD + c] - wvy[ORD - 1 - c]) / A.dx;         // vy[x+c] - vy[x-c-1]
-          d2 += A.C[c] * (A.vx[id + (c + 1) * sy] - A.vx[id - c * sy]) / A.dy;
+          d2 += A.C[c] * (vxP.ld(col + (c + 1) * sy) - vxP.ld(col - c * sy)) / A.dy;
         }
-        if (bx >= 0) {
-          float *p = A.psi[P_DVY_DX] + px;
-          const float pn = A.ax[0].b[jx] * sload<NT>(p) + A.ax[0].a[jx] * d1;
-          sstore<NT>(pn, p);
-          d1 = d1 / A.ax[0].k[jx] + pn;
-        }
-        if (by >= 0) {
-          float *p = A.psi[P_DVX_DY] + py;
-          const float pn = A.ax[1].bh[jy] * sload<NT>(p) + A.ax[1].ah[jy] * d2;
-          sstore<NT>(pn, p);
-          d2 = d2 / A.ax[1].kh[jy] + pn;
-        }
-        rho4 = 0.25f * (rho_n + rho0 + A.rho[id + sy] + A.rho[id + sx + sy]);
-        cs4 = 0.25f * (cs_n + cs0 + A.cs[id + sy] + A.cs[id + sx + sy]);
+        if (bx >= 0) d1 = stretch<CP>(A.ax[0].a, A.ax[0].b, A.ax[0].k, PB(A.psi[P_DVY_DX] + (long)bx * sx, pbytes), col, jx, d1);
+        if (by >= 0) d2 = stretch<CP>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, PB(A.psi[P_DVX_DY] + (long)x * nby * nz, pyb), py_col, jy, d2);
+        rho4 = 0.25f * (rho_n + rho0 + rhoP.ld(col + sy) + rho1P.ld(col + sy));
+        cs4 = 0.25f * (cs_n + cs0 + csP.ld(col + sy) + cs1P.ld(col + sy));
         sig = A.sxy; rr = A.r[3];
       } else if (sh == 1) {     // sigma_xz: dvz/dx (D-), dvx/dz (D+)
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
           d1 += A.C[c] * (wvz[ORD + c] - wvz[ORD - 1 - c]) / A.dx;         // vz[x+c] - vz[x-c-1]
-          d2 += A.C[c] * (A.vx[id + c + 1] - A.vx[id - c]) / A.dz;
+          d2 += A.C[c] * (vxP.ld(col + c + 1) - vxP.ld(col - c)) / A.dz;
         }
-        if (bx >= 0) {
-          float *p = A.psi[P_DVZ_DX] + px;
-          const float pn = A.ax[0].b[jx] * sload<NT>(p) + A.ax[0].a[jx] * d1;
-          sstore<NT>(pn, p);
-          d1 = d1 / A.ax[0].k[jx] + pn;
-        }
-        if (bz >= 0) {
-          float *p = A.psi[P_DVX_DZ] + pz;
-          const float pn = A.ax[2].bh[jz] * sload<NT>(p) + A.ax[2].ah[jz] * d2;
-          sstore<NT>(pn, p);
-          d2 = d2 / A.ax[2].kh[jz] + pn;
-        }
-        rho4 = 0.25f * (rho_n + rho0 + A.rho[id + 1] + A.rho[id + sx + 1]);
-        cs4 = 0.25f * (cs_n + cs0 + A.cs[id + 1] + A.cs[id + sx + 1]);
+        if (bx >= 0) d1 = stretch<CP>(A.ax[0].a, A.ax[0].b, A.ax[0].k, PB(A.psi[P_DVZ_DX] + (long)bx * sx, pbytes), col, jx, d1);
+        if (bz >= 0) d2 = stretch<CP>(A.ax[2].ah, A.ax[2].bh, A.ax[2].kh, PB(A.psi[P_DVX_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, d2);
+        rho4 = 0.25f * (rho_n + rho0 + rhoP.ld(col + 1) + rho1P.ld(col + 1));
+        cs4 = 0.25f * (cs_n + cs0 + csP.ld(col + 1) + cs1P.ld(col + 1));
         sig = A.sxz; rr = A.r[4];
       } else {                  // sigma_yz: dvz/dy (D+), dvy/dz (D+)
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
-          d1 += A.C[c] * (A.vz[id + (c + 1) * sy] - A.vz[id - c * sy]) / A.dy;
-          d2 += A.C[c] * (A.vy[id + c + 1] - A.vy[id - c]) / A.dz;
+          d1 += A.C[c] * (vzP.ld(col + (c + 1) * sy) - vzP.ld(col - c * sy)) / A.dy;
+          d2 += A.C[c] * (vyP.ld(col + c + 1) - vyP.ld(col - c)) / A.dz;
         }
-        if (by >= 0) {
-          float *p = A.psi[P_DVZ_DY] + py;
-          const float pn = A.ax[1].bh[jy] * sload<NT>(p) + A.ax[1].ah[jy] * d1;
-          sstore<NT>(pn, p);
-          d1 = d1 / A.ax[1].kh[jy] + pn;
-        }
-        if (bz >= 0) {
-          float *p = A.psi[P_DVY_DZ] + pz;
-          const float pn = A.ax[2].bh[jz] * sload<NT>(p) + A.ax[2].ah[jz] * d2;
-          sstore<NT>(pn, p);
-          d2 = d2 / A.ax[2].kh[jz] + pn;
-        }
-        rho4 = 0.25f * (A.rho[id + sy] + rho0 + A.rho[id + 1] + A.rho[id + sy + 1]);
-        cs4 = 0.25f * (A.cs[id + sy] + cs0 + A.cs[id + 1] + A.cs[id + sy + 1]);
+        if (by >= 0) d1 = stretch<CP>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, PB(A.psi[P_DVZ_DY] + (long)x * nby * nz, pyb), py_col, jy, d1);
+        if (bz >= 0) d2 = stretch<CP>(A.ax[2].ah, A.ax[2].bh, A.ax[2].kh, PB(A.psi[P_DVY_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, d2);
+        rho4 = 0.25f * (rhoP.ld(col + sy) + rho0 + rhoP.ld(col + 1) + rhoP.ld(col + sy + 1));
+        cs4 = 0.25f * (csP.ld(col + sy) + cs0 + csP.ld(col + 1) + csP.ld(col + sy + 1));
         sig = A.syz; rr = A.r[5];
       }
       const float mu = rho4 * (cs4 * cs4);
-      const float so = sload<NT>(sig + id);
+      const PB sigP(sig + rofs, pbytes);
       if constexpr (VISCO) {
         float S = 0.f;
         const float X = d1 + d2;
         for (int l = 0; l < A.L; l++) {
-          float *pr = rr + l * A.ncell + id;
-          const float o = sload<NT>(pr);
+          const PB pr(rr + (long)l * A.ncell + rofs, pbytes);
+          const float o = pr.ld<CP>(col);
           const float n = (o + X * A.zk.phi_s[l] - o * A.zk.h_s[l]) * A.zk.m_s[l];
           S += o + n;
-          sstore<NT>(n, pr);
+          pr.st<CP>(col, n);
         }
-        sstore<NT>(so + (mu * (d1 + d2) + mu * 0.5f * S) * dt, sig + id);
+        sigP.st<CP>(col, sigP.ld<CP>(col) + (mu * (d1 + d2) + mu * 0.5f * S) * dt);
       } else {
-        sstore<NT>(so + (d2 + d1) * mu * dt, sig + id);
+        sigP.st<CP>(col, sigP.ld<CP>(col) + (d2 + d1) * mu * dt);
       }
     }
   }
 }
 
 template <int ORD, int TY, bool NT>
-__global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, int nzb, int nyb, int nxc) {
-  const Tile T = tile_of_block(nzb, nyb, nxc);
+__global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, int nzb, int nyb, int nxc, int remap) {
+  constexpr int CP = NT ? 2 : 0;
+  const Tile T = tile_of_block(nzb, nyb, nxc, remap);
   const int z = T.bz * BZ + threadIdx.x;
   const int y = T.by * TY + threadIdx.y;
   const int nx = A.nx, ny = A.ny, nz = A.nz;
   const bool valid = z < nz && y < ny;
   constexpr int last = ORD - 1, off = ORD - 1;
-  const long sy = nz, sx = (long)ny * nz;
-  const long col = (long)y * sy + z;
+  const unsigned sy = nz;
+  const long sx = (long)ny * nz;
+  const unsigned pbytes = (unsigned)(sx * 4);
+  const unsigned col = y * sy + z;
   const float dt = A.dt;
   const int x0 = T.bxc * xchunk, x1 = min(nx, x0 + xchunk);
+  auto P = [&](const float *f, int x) { return PB(f + (long)(x < 0 ? 0 : (x >= nx ? nx - 1 : x)) * sx, pbytes); };
   unsigned v2bits = 0;
   if (valid) {
     const int jy = y - off, jz = z - off;
     const int by = band_index(A.ax[1], jy), bz = band_index(A.ax[2], jz);
     const int nby = A.ax[1].nb, nbz = A.ax[2].nb;
+    const unsigned py_col = by * nz + z, pz_col = y * nbz + bz;
+    const unsigned pyb = (unsigned)nby * nz * 4u, pzb = (unsigned)ny * nbz * 4u;
     const bool yi_vx = y >= last + 1 && y < ny - last, zi_vx = z >= last + 1 && z < nz - last;
     const bool yi_vy = y >= last && y < ny - last - 1, zi_vy = z >= last + 1 && z < nz - last;
     const bool yi_vz = y >= last + 1 && y < ny - last, zi_vz = z >= last && z < nz - last - 1;
@@ -274,53 +261,39 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
     float wxx[2 * ORD], wxy[2 * ORD], wxz[2 * ORD];
 #pragma unroll
     for (int i = 0; i < 2 * ORD - 1; i++) {
-      wxx[i + 1] = ld_plane(A.sxx, sx, x0 - ORD + i, nx, col);
-      wxy[i + 1] = ld_plane(A.sxy, sx, x0 - (ORD - 1) + i, nx, col);
-      wxz[i + 1] = ld_plane(A.sxz, sx, x0 - (ORD - 1) + i, nx, col);
+      wxx[i + 1] = P(A.sxx, x0 - ORD + i).ld(col);
+      wxy[i + 1] = P(A.sxy, x0 - (ORD - 1) + i).ld(col);
+      wxz[i + 1] = P(A.sxz, x0 - (ORD - 1) + i).ld(col);
     }
-    float rho_n = ld_plane(A.rho, sx, x0, nx, col);
+    float rho_n = P(A.rho, x0).ld(col);
+#pragma unroll 1
     for (int x = x0; x < x1; x++) {
 #pragma unroll
       for (int i = 0; i < 2 * ORD - 1; i++) { wxx[i] = wxx[i + 1]; wxy[i] = wxy[i + 1]; wxz[i] = wxz[i + 1]; }
-      wxx[2 * ORD - 1] = ld_plane(A.sxx, sx, x + ORD - 1, nx, col);
-      wxy[2 * ORD - 1] = ld_plane(A.sxy, sx, x + ORD, nx, col);
-      wxz[2 * ORD - 1] = ld_plane(A.sxz, sx, x + ORD, nx, col);
+      wxx[2 * ORD - 1] = P(A.sxx, x + ORD - 1).ld(col);
+      wxy[2 * ORD - 1] = P(A.sxy, x + ORD).ld(col);
+      wxz[2 * ORD - 1] = P(A.sxz, x + ORD).ld(col);
       const float rho0 = rho_n;
-      rho_n = ld_plane(A.rho, sx, x + 1, nx, col);
-      const long id = x * sx + col;
+      rho_n = P(A.rho, x + 1).ld(col);
       const int jx = x - off;
       const int bx = band_index(A.ax[0], jx);
-      const long px = (long)bx * sx + col;
-      const long py = (long)x * nby * nz + (long)by * nz + z;
-      const long pz = ((long)x * ny + y) * nbz + bz;
-      float vxn = sload<NT>(A.vx + id), vyn = sload<NT>(A.vy + id), vzn = sload<NT>(A.vz + id);
+      const long rofs = (long)x * sx;
+      const PB sxyP(A.sxy + rofs, pbytes), sxzP(A.sxz + rofs, pbytes), syyP(A.syy + rofs, pbytes);
+      const PB syzP(A.syz + rofs, pbytes), szzP(A.szz + rofs, pbytes), rhoP(A.rho + rofs, pbytes);
+      const PB vxP(A.vx + rofs, pbytes), vyP(A.vy + rofs, pbytes), vzP(A.vz + rofs, pbytes);
+      float vxn = vxP.ld<CP>(col), vyn = vyP.ld<CP>(col), vzn = vzP.ld<CP>(col);
       // vx (wgsl3d :1647-1681)
       if (x >= last + 1 && x < nx - last && yi_vx && zi_vx) {
         float a = 0.f, b = 0.f, c_ = 0.f;
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
           a += A.C[c] * (wxx[ORD + c] - wxx[ORD - 1 - c]) / A.dx;          // sxx[x+c] - sxx[x-c-1]
-          b += A.C[c] * (A.sxy[id + c * sy] - A.sxy[id - (c + 1) * sy]) / A.dy;
-          c_ += A.C[c] * (A.sxz[id + c] - A.sxz[id - c - 1]) / A.dz;
+          b += A.C[c] * (sxyP.ld(col + c * sy) - sxyP.ld(col - (c + 1) * sy)) / A.dy;
+          c_ += A.C[c] * (sxzP.ld(col + c) - sxzP.ld(col - c - 1)) / A.dz;
         }
-        if (bx >= 0) {
-          float *p = A.psi[P_DSXX_DX] + px;
-          const float pn = A.ax[0].b[jx] * sload<NT>(p) + A.ax[0].a[jx] * a;
-          sstore<NT>(pn, p);
-          a = a / A.ax[0].k[jx] + pn;
-        }
-        if (by >= 0) {
-          float *p = A.psi[P_DSXY_DY] + py;
-          const float pn = A.ax[1].b[jy] * sload<NT>(p) + A.ax[1].a[jy] * b;
-          sstore<NT>(pn, p);
-          b = b / A.ax[1].k[jy] + pn;
-        }
-        if (bz >= 0) {
-          float *p = A.psi[P_DSXZ_DZ] + pz;
-          const float pn = A.ax[2].b[jz] * sload<NT>(p) + A.ax[2].a[jz] * c_;
-          sstore<NT>(pn, p);
-          c_ = c_ / A.ax[2].k[jz] + pn;
-        }
+        if (bx >= 0) a = stretch<CP>(A.ax[0].a, A.ax[0].b, A.ax[0].k, PB(A.psi[P_DSXX_DX] + (long)bx * sx, pbytes), col, jx, a);
+        if (by >= 0) b = stretch<CP>(A.ax[1].a, A.ax[1].b, A.ax[1].k, PB(A.psi[P_DSXY_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
+        if (bz >= 0) c_ = stretch<CP>(A.ax[2].a, A.ax[2].b, A.ax[2].k, PB(A.psi[P_DSXZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
         const float rho = 0.5f * (rho_n + rho0);
         if (rho > 0.f) vxn = (a + b + c_) * dt / rho + vxn;
       }
@@ -331,28 +304,13 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
           a += A.C[c] * (wxy[ORD + c] - wxy[ORD - 1 - c]) / A.dx;          // sxy[x+c+1] - sxy[x-c]
-          b += A.C[c] * (A.syy[id + (c + 1) * sy] - A.syy[id - c * sy]) / A.dy;
-          c_ += A.C[c] * (A.syz[id + c] - A.syz[id - c - 1]) / A.dz;
+          b += A.C[c] * (syyP.ld(col + (c + 1) * sy) - syyP.ld(col - c * sy)) / A.dy;
+          c_ += A.C[c] * (syzP.ld(col + c) - syzP.ld(col - c - 1)) / A.dz;
         }
-        if (bx >= 0) {
-          float *p = A.psi[P_DSXY_DX] + px;
-          const float pn = A.ax[0].bh[jx] * sload<NT>(p) + A.ax[0].ah[jx] * a;
-          sstore<NT>(pn, p);
-          a = a / A.ax[0].kh[jx] + pn;
-        }
-        if (by >= 0) {
-          float *p = A.psi[P_DSYY_DY] + py;
-          const float pn = A.ax[1].bh[jy] * sload<NT>(p) + A.ax[1].ah[jy] * b;
-          sstore<NT>(pn, p);
-          b = b / A.ax[1].kh[jy] + pn;
-        }
-        if (bz >= 0) {
-          float *p = A.psi[P_DSYZ_DZ] + pz;
-          const float pn = A.ax[2].b[jz] * sload<NT>(p) + A.ax[2].a[jz] * c_;
-          sstore<NT>(pn, p);
-          c_ = c_ / A.ax[2].k[jz] + pn;
-        }
-        const float rho = 0.5f * (A.rho[id + sy] + rho0);
+        if (bx >= 0) a = stretch<CP>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, PB(A.psi[P_DSXY_DX] + (long)bx * sx, pbytes), col, jx, a);
+        if (by >= 0) b = stretch<CP>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, PB(A.psi[P_DSYY_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
+        if (bz >= 0) c_ = stretch<CP>(A.ax[2].a, A.ax[2].b, A.ax[2].k, PB(A.psi[P_DSYZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
+        const float rho = 0.5f * (rhoP.ld(col + sy) + rho0);
         if (rho > 0.f) vyn = (a + b + c_) * dt / rho + vyn;
       }
       // vz, with the vy density (wgsl3d :1719-1758)
@@ -361,35 +319,20 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
           a += A.C[c] * (wxz[ORD + c] - wxz[ORD - 1 - c]) / A.dx;          // sxz[x+c+1] - sxz[x-c]
-          b += A.C[c] * (A.syz[id + c * sy] - A.syz[id - (c + 1) * sy]) / A.dy;
-          c_ += A.C[c] * (A.szz[id + c + 1] - A.szz[id - c]) / A.dz;
+          b += A.C[c] * (syzP.ld(col + c * sy) - syzP.ld(col - (c + 1) * sy)) / A.dy;
+          c_ += A.C[c] * (szzP.ld(col + c + 1) - szzP.ld(col - c)) / A.dz;
         }
-        if (bx >= 0) {
-          float *p = A.psi[P_DSXZ_DX] + px;
-          const float pn = A.ax[0].bh[jx] * sload<NT>(p) + A.ax[0].ah[jx] * a;
-          sstore<NT>(pn, p);
-          a = a / A.ax[0].kh[jx] + pn;
-        }
-        if (by >= 0) {
-          float *p = A.psi[P_DSYZ_DY] + py;
-          const float pn = A.ax[1].b[jy] * sload<NT>(p) + A.ax[1].a[jy] * b;
-          sstore<NT>(pn, p);
-          b = b / A.ax[1].k[jy] + pn;
-        }
-        if (bz >= 0) {
-          float *p = A.psi[P_DSZZ_DZ] + pz;
-          const float pn = A.ax[2].bh[jz] * sload<NT>(p) + A.ax[2].ah[jz] * c_;
-          sstore<NT>(pn, p);
-          c_ = c_ / A.ax[2].kh[jz] + pn;
-        }
-        const float rho = 0.5f * (A.rho[id + sy] + rho0);
+        if (bx >= 0) a = stretch<CP>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, PB(A.psi[P_DSXZ_DX] + (long)bx * sx, pbytes), col, jx, a);
+        if (by >= 0) b = stretch<CP>(A.ax[1].a, A.ax[1].b, A.ax[1].k, PB(A.psi[P_DSYZ_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
+        if (bz >= 0) c_ = stretch<CP>(A.ax[2].ah, A.ax[2].bh, A.ax[2].kh, PB(A.psi[P_DSZZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
+        const float rho = 0.5f * (rhoP.ld(col + sy) + rho0);
         if (rho > 0.f) vzn = (a + b + c_) * dt / rho + vzn;
       } else {
         vxn = 0.f; vyn = 0.f; vzn = 0.f;
       }
-      sstore<NT>(vxn, A.vx + id);
-      sstore<NT>(vyn, A.vy + id);
-      sstore<NT>(vzn, A.vz + id);
+      vxP.st<CP>(col, vxn);
+      vyP.st<CP>(col, vyn);
+      vzP.st<CP>(col, vzn);
       const unsigned u = __float_as_uint(vxn * vxn + vyn * vyn + vzn * vzn);
       v2bits = v2bits > u ? v2bits : u;
     }
@@ -413,14 +356,14 @@ Launch launch_shape(const StepArgs &A, int ty, int xchunk) {
 }
 
 template <int ORD, int TY, bool NT>
-hipError_t launch_one(bool stress, int visco, const StepArgs &A, int xchunk, hipStream_t st) {
+hipError_t launch_one(bool stress, int visco, const StepArgs &A, int xchunk, int remap, hipStream_t st) {
   const Launch L = launch_shape(A, TY, xchunk);
   const dim3 grid((unsigned)(L.nzb * L.nyb * L.nxc)), block(BZ, TY);
   if (stress) {
-    if (visco) hipLaunchKernelGGL((k_stress3d<true, ORD, TY, NT>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc);
-    else hipLaunchKernelGGL((k_stress3d<false, ORD, TY, NT>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc);
+    if (visco) hipLaunchKernelGGL((k_stress3d<true, ORD, TY, NT>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
+    else hipLaunchKernelGGL((k_stress3d<false, ORD, TY, NT>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
   } else {
-    hipLaunchKernelGGL((k_velocity3d<ORD, TY, NT>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc);
+    hipLaunchKernelGGL((k_velocity3d<ORD, TY, NT>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
   }
   return hipGetLastError();
 }
@@ -430,8 +373,8 @@ hipError_t launch_tuned(bool stress, int visco, const StepArgs &A, const KTune &
   // shapes compiled for every order: (TY, NT) in {1,2,4,8} x {0,1}
 #define VSE_TY(TYV)                                                                          \
   case TYV:                                                                                  \
-    return t.nt ? launch_one<ORD, TYV, true>(stress, visco, A, t.xchunk, st)                 \
-                : launch_one<ORD, TYV, false>(stress, visco, A, t.xchunk, st);
+    return t.nt ? launch_one<ORD, TYV, true>(stress, visco, A, t.xchunk, t.remap, st)        \
+                : launch_one<ORD, TYV, false>(stress, visco, A, t.xchunk, t.remap, st);
   switch (t.ty) {
     VSE_TY(1) VSE_TY(2) VSE_TY(4) VSE_TY(8)
     default: return hipErrorInvalidValue;

@@ -193,9 +193,9 @@ class VSESim:
         self._check(self._lib.vse_profile(self._h, int(n_steps), ms))
         return tuple(float(v) for v in ms)
 
-    def set_tuning(self, kernel, ty, nt, xchunk):
+    def set_tuning(self, kernel, ty, nt, xchunk, remap=0):
         """Launch shape of the 3D stress (kernel 0) or velocity (1) pass."""
-        self._check(self._lib.vse_set_tuning(self._h, int(kernel), int(ty), int(nt), int(xchunk)))
+        self._check(self._lib.vse_set_tuning(self._h, int(kernel), int(ty), int(nt), int(xchunk), int(remap)))
 
     def band_cells(self):
         """Cells per psi slot-axis band: {axis: band width} from the CPML profiles."""

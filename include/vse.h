@@ -131,6 +131,15 @@ int vse_profile(vse_sim *s, int n_steps, float *ms);
  * sim_webgpu_3D_viscoelastic.py:40-42). */
 int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap);
 
+/* Arithmetic mode of the 3D kernels.  VSE_MATH_STRICT (default) keeps the
+ * reference's FP32 operation order term for term (coef*(a-b)/dx as in
+ * shader_webgpu_3D_viscoelastic.wgsl:1367-1378).  VSE_MATH_FAST multiplies
+ * by f32(coef/dx), 1/k and 1/3 instead of dividing: one extra rounding per
+ * term, within the FP32 noise of the reference. */
+#define VSE_MATH_STRICT 0
+#define VSE_MATH_FAST 1
+int vse_set_math(vse_sim *s, int mode);
+
 /* Copy a field (VSE_F_*) to host, n cells in the reference layout.
  * Replaces the sigma_yy / sigma_zz read-back (sim_webgpu_viscoelastic.py:540-541). */
 int vse_read_field(vse_sim *s, int field, float *out);

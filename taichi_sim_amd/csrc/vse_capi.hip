@@ -33,7 +33,8 @@ struct vse_sim {
   float *field[9] = {};
   float *rho = nullptr, *cp = nullptr, *cs = nullptr;
   bool have_material = false;
-  float *prof[3][6] = {};
+  float *prof[3][8] = {};     // a, b, k, a_h, b_h, k_h, 1/k, 1/k_h
+  int math = VSE_MATH_STRICT;
   AxisProf ax[3]{};
   bool have_cpml[3] = {false, false, false};
   float *psi[P_COUNT] = {};
@@ -209,15 +210,23 @@ int vse_set_cpml(vse_sim *s, int axis, const float *a, const float *b, const flo
     if (o.lo != lo || o.hi != hi) return s->fail(VSE_ERR_STATE, "CPML band changed after state allocation");
   }
   int rc = 0;
-  for (int i = 0; i < 6 && !rc; i++) {
+  std::vector<float> inv_k(np), inv_kh(np);
+  for (int j = 0; j < np; j++) {
+    inv_k[j] = 1.0f / k[j];
+    inv_kh[j] = 1.0f / k_h[j];
+  }
+  const float *all[8] = {a, b, k, a_h, b_h, k_h, inv_k.data(), inv_kh.data()};
+  for (int i = 0; i < 8 && !rc; i++) {
     if (!s->prof[axis][i]) rc = s->alloc(&s->prof[axis][i], np);
-    if (!rc) rc = s->hip(hipMemcpyAsync(s->prof[axis][i], src[i], np * sizeof(float), hipMemcpyHostToDevice,
+    if (!rc) rc = s->hip(hipMemcpyAsync(s->prof[axis][i], all[i], np * sizeof(float), hipMemcpyHostToDevice,
                                         s->stream), "copy cpml");
   }
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "set_cpml copy");
   if (rc) return rc;
   AxisProf &P = s->ax[axis];
   P.a = s->prof[axis][0]; P.b = s->prof[axis][1]; P.k = s->prof[axis][2];
   P.ah = s->prof[axis][3]; P.bh = s->prof[axis][4]; P.kh = s->prof[axis][5];
+  P.ki = s->prof[axis][6]; P.khi = s->prof[axis][7];
   P.lo = lo; P.hi = hi; P.np = np; P.nb = lo + (np - hi);
   s->have_cpml[axis] = true;
   s->drop_graph();
@@ -296,7 +305,7 @@ static int finalize(vse_sim *s) {
   for (int slot = 0; slot < P_COUNT; slot++)
     if (s->slot_used(slot) && !s->psi[slot]) return s->fail(VSE_ERR_STATE, "psi slot not allocated");
   for (int a = 0; a < s->d.ndim; a++)
-    for (int i = 0; i < 6; i++)
+    for (int i = 0; i < 8; i++)
       if (!s->prof[a][i]) return s->fail(VSE_ERR_STATE, "CPML profile not allocated");
   for (int f = 0; f < 9; f++) {
     const bool need = s->d.ndim == 3 || f == VSE_F_VX || f == VSE_F_VY || f == VSE_F_SXX || f == VSE_F_SYY ||
@@ -333,7 +342,12 @@ static void make_args(vse_sim *s, StepArgs &A, SparseArgs &S) {
   A.nx = d.nx; A.ny = d.ny; A.nz = d.nz; A.ncell = s->ncell;
   A.dx = d.dx; A.dy = d.dy; A.dz = d.ndim == 3 ? d.dz : 1.f; A.dt = d.dt;
   A.one_dx = 1.0f / d.dx; A.one_dy = 1.0f / d.dy; A.one_dz = d.ndim == 3 ? 1.0f / d.dz : 1.f;
-  for (int c = 0; c < d.ord; c++) A.C[c] = d.coefs[c];
+  for (int c = 0; c < d.ord; c++) {
+    A.C[c] = d.coefs[c];
+    A.Cd[0][c] = d.coefs[c] / A.dx;
+    A.Cd[1][c] = d.coefs[c] / A.dy;
+    A.Cd[2][c] = d.coefs[c] / A.dz;
+  }
   for (int a = 0; a < 3; a++) A.ax[a] = s->ax[a];
   A.vx = s->field[VSE_F_VX]; A.vy = s->field[VSE_F_VY]; A.vz = s->field[VSE_F_VZ];
   A.sxx = s->field[VSE_F_SXX]; A.syy = s->field[VSE_F_SYY]; A.szz = s->field[VSE_F_SZZ];
@@ -367,9 +381,9 @@ static hipError_t enqueue_step(vse_sim *s, const StepArgs &A, const SparseArgs &
     if (e != hipSuccess) return e;
     e = vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->stream);
   } else {
-    e = vse_launch_stress3d(d.visco, d.ord, A, s->tune[0], s->stream);
+    e = vse_launch_stress3d(d.visco, d.ord, s->math, A, s->tune[0], s->stream);
     if (e != hipSuccess) return e;
-    e = vse_launch_velocity3d(d.visco, d.ord, A, s->tune[1], s->stream);
+    e = vse_launch_velocity3d(d.visco, d.ord, s->math, A, s->tune[1], s->stream);
   }
   if (e != hipSuccess) return e;
   return vse_launch_sparse(S, s->stream);
@@ -466,10 +480,10 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
   for (int it = 0; it < n_steps && !rc; it++) {
     hipError_t e = hipEventRecord(ev[0], s->stream);
     if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_stress2d(d.variant, d.visco, d.ord, A, s->stream)
-                                         : vse_launch_stress3d(d.visco, d.ord, A, s->tune[0], s->stream);
+                                         : vse_launch_stress3d(d.visco, d.ord, s->math, A, s->tune[0], s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[1], s->stream);
     if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->stream)
-                                         : vse_launch_velocity3d(d.visco, d.ord, A, s->tune[1], s->stream);
+                                         : vse_launch_velocity3d(d.visco, d.ord, s->math, A, s->tune[1], s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
     if (e == hipSuccess) e = vse_launch_sparse(S, s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[3], s->stream);
@@ -491,9 +505,16 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
 
 int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap) {
   if (check_handle(s) || kernel < 0 || kernel > 1) return VSE_ERR_ARG;
-  if (!(ty == 1 || ty == 2 || ty == 4 || ty == 8) || (nt != 0 && nt != 1) || xchunk < 1 || (remap != 0 && remap != 1))
-    return s->fail(VSE_ERR_ARG, "tuning: ty in {1,2,4,8}, nt in {0,1}, xchunk >= 1, remap in {0,1}");
+  if (!(ty == 1 || ty == 4) || (nt != 0 && nt != 1) || xchunk < 1 || (remap != 0 && remap != 1))
+    return s->fail(VSE_ERR_ARG, "tuning: ty in {1,4}, nt in {0,1}, xchunk >= 1, remap in {0,1}");
   s->tune[kernel] = KTune{ty, nt, xchunk, remap};
+  s->drop_graph();
+  return VSE_OK;
+}
+
+int vse_set_math(vse_sim *s, int mode) {
+  if (check_handle(s) || (mode != VSE_MATH_STRICT && mode != VSE_MATH_FAST)) return VSE_ERR_ARG;
+  s->math = mode;
   s->drop_graph();
   return VSE_OK;
 }

@@ -35,6 +35,7 @@ static const int kPsiSlots2D[] = {P_DVX_DX, P_DVY_DY, P_DVY_DX, P_DVX_DY,
 // psi stays 0 and the stretched derivative equals the plain one bit for bit.
 struct AxisProf {
   const float *a, *b, *k, *ah, *bh, *kh;
+  const float *ki, *khi;             // 1/k, 1/k_half (fast arithmetic)
   int lo, hi, np, nb;
 };
 
@@ -50,6 +51,7 @@ struct StepArgs {
   float dx, dy, dz, dt;
   float one_dx, one_dy, one_dz;      // f32(1/d) (Taichi variant)
   float C[VSE_MAX_ORD];
+  float Cd[3][VSE_MAX_ORD];          // f32(C[c] / d_axis) (fast arithmetic)
   AxisProf ax[3];
   float *vx, *vy, *vz;
   float *sxx, *syy, *szz, *sxy, *sxz, *syz;
@@ -98,9 +100,16 @@ struct KTune {
   int ty, nt, xchunk, remap;
 };
 
+// Arithmetic mode.  STRICT keeps the reference's FP32 operation order term
+// for term (coef*(a-b)/d, D/k, div/3): bit-compatible with the CPU oracle.
+// FAST multiplies by precomputed f32(coef/d), 1/k and 1/3 instead of
+// dividing: one extra rounding per term, results within the FP32 noise of the
+// reference (checked against the oracle at 1e-5 rel-L2), far fewer VALU ops.
+enum { VSE_MATH_STRICT = 0, VSE_MATH_FAST = 1 };
+
 // launchers (vse_k2d.hip / vse_k3d.hip)
 hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st);
 hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st);
-hipError_t vse_launch_stress3d(int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st);
-hipError_t vse_launch_velocity3d(int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st);
+hipError_t vse_launch_stress3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
+hipError_t vse_launch_velocity3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
 hipError_t vse_launch_sparse(const SparseArgs &S, hipStream_t st);

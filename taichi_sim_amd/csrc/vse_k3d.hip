@@ -63,16 +63,26 @@ struct PB {
   }
 };
 
-// psi recursion psi <- b psi + a D, returns D/k + psi (wgsl3d :1372-1378)
-template <int CP>
-__device__ __forceinline__ float stretch(const float *a, const float *b, const float *k, const PB &P, unsigned i,
-                                         int j, float D) {
+// psi recursion psi <- b psi + a D, returns D/k + psi (wgsl3d :1372-1378);
+// FAST multiplies by the precomputed 1/k.
+template <int CP, bool FAST>
+__device__ __forceinline__ float stretch(const float *a, const float *b, const float *k, const float *ki,
+                                         const PB &P, unsigned i, int j, float D) {
   const float pn = b[j] * P.ld<CP>(i) + a[j] * D;
   P.st<CP>(i, pn);
-  return D / k[j] + pn;
+  if constexpr (FAST) return D * ki[j] + pn;
+  else return D / k[j] + pn;
 }
 
-template <bool VISCO, int ORD, int TY, bool NT>
+// One staggered-difference term along axis AX: C*(a-b)/d (reference order)
+// or (a-b)*f32(C/d) (fast).
+template <bool FAST, int AX>
+__device__ __forceinline__ float dterm(const StepArgs &A, int c, float a, float b) {
+  if constexpr (FAST) return (a - b) * A.Cd[AX][c];
+  else return A.C[c] * (a - b) / (AX == 0 ? A.dx : (AX == 1 ? A.dy : A.dz));
+}
+
+template <bool VISCO, int ORD, int TY, bool NT, bool FAST>
 __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, int nzb, int nyb, int nxc, int remap) {
   constexpr int CP = NT ? 2 : 0;
   const Tile T = tile_of_block(nzb, nyb, nxc, remap);
@@ -133,13 +143,13 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
       float dvx = 0.f, dvy = 0.f, dvz = 0.f;
 #pragma unroll
       for (int c = 0; c < ORD; c++) {
-        dvx += A.C[c] * (wvx[ORD + c] - wvx[ORD - 1 - c]) / A.dx;            // vx[x+c+1] - vx[x-c]
-        dvy += A.C[c] * (vyP.ld(col + c * sy) - vyP.ld(col - (c + 1) * sy)) / A.dy;
-        dvz += A.C[c] * (vzP.ld(col + c) - vzP.ld(col - c - 1)) / A.dz;
+        dvx += dterm<FAST, 0>(A, c, wvx[ORD + c], wvx[ORD - 1 - c]);            // vx[x+c+1] - vx[x-c]
+        dvy += dterm<FAST, 1>(A, c, vyP.ld(col + c * sy), vyP.ld(col - (c + 1) * sy));
+        dvz += dterm<FAST, 2>(A, c, vzP.ld(col + c), vzP.ld(col - c - 1));
       }
-      if (bx >= 0) dvx = stretch<CP>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, PB(A.psi[P_DVX_DX] + (long)bx * sx, pbytes), col, jx, dvx);
-      if (by >= 0) dvy = stretch<CP>(A.ax[1].a, A.ax[1].b, A.ax[1].k, PB(A.psi[P_DVY_DY] + (long)x * nby * nz, pyb), py_col, jy, dvy);
-      if (bz >= 0) dvz = stretch<CP>(A.ax[2].a, A.ax[2].b, A.ax[2].k, PB(A.psi[P_DVZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, dvz);
+      if (bx >= 0) dvx = stretch<CP, FAST>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, A.ax[0].khi, PB(A.psi[P_DVX_DX] + (long)bx * sx, pbytes), col, jx, dvx);
+      if (by >= 0) dvy = stretch<CP, FAST>(A.ax[1].a, A.ax[1].b, A.ax[1].k, A.ax[1].ki, PB(A.psi[P_DVY_DY] + (long)x * nby * nz, pyb), py_col, jy, dvy);
+      if (bz >= 0) dvz = stretch<CP, FAST>(A.ax[2].a, A.ax[2].b, A.ax[2].k, A.ax[2].ki, PB(A.psi[P_DVZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, dvz);
       const float cp = PB(A.cp + rofs, pbytes).ld<CP>(col);
       const float lambda = rho0 * (cp * cp - 2.0f * cs0 * cs0);
       const float mu = rho0 * (cs0 * cs0);
@@ -148,13 +158,14 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
       if constexpr (VISCO) {
         float Sxx = 0.f, Syy = 0.f, Szz = 0.f;
         const float div = dvx + dvy + dvz;
+        const float div3 = FAST ? div * (1.0f / 3.0f) : div / 3.0f;
         for (int l = 0; l < A.L; l++) {
           const long lo = (long)l * A.ncell + rofs;
           const PB pxx(A.r[0] + lo, pbytes), pyy(A.r[1] + lo, pbytes), pzz(A.r[2] + lo, pbytes);
           const float oxx = pxx.ld<CP>(col), oyy = pyy.ld<CP>(col), ozz = pzz.ld<CP>(col);
           const float nxx = ((oxx + div * A.zk.phi_p[l] - oxx * A.zk.h_p[l]) * A.zk.m_p[l]);
-          const float nyy = ((oyy + (dvx - div / 3.0f) * A.zk.phi_s[l] - oyy * A.zk.h_s[l]) * A.zk.m_s[l]);
-          const float nzz = ((ozz + (dvy - div / 3.0f) * A.zk.phi_s[l] - ozz * A.zk.h_s[l]) * A.zk.m_s[l]);
+          const float nyy = ((oyy + (dvx - div3) * A.zk.phi_s[l] - oyy * A.zk.h_s[l]) * A.zk.m_s[l]);
+          const float nzz = ((ozz + (dvy - div3) * A.zk.phi_s[l] - ozz * A.zk.h_s[l]) * A.zk.m_s[l]);
           Sxx += oxx + nxx; Syy += oyy + nyy; Szz += ozz + nzz;
           pxx.st<CP>(col, nxx); pyy.st<CP>(col, nyy); pzz.st<CP>(col, nzz);
         }
@@ -180,33 +191,33 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
       if (sh == 0) {            // sigma_xy: dvy/dx (D-), dvx/dy (D+)
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
-          d1 += A.C[c] * (wvy[ORD + c] - wvy[ORD - 1 - c]) / A.dx;         // vy[x+c] - vy[x-c-1]
-          d2 += A.C[c] * (vxP.ld(col + (c + 1) * sy) - vxP.ld(col - c * sy)) / A.dy;
+          d1 += dterm<FAST, 0>(A, c, wvy[ORD + c], wvy[ORD - 1 - c]);         // vy[x+c] - vy[x-c-1]
+          d2 += dterm<FAST, 1>(A, c, vxP.ld(col + (c + 1) * sy), vxP.ld(col - c * sy));
         }
-        if (bx >= 0) d1 = stretch<CP>(A.ax[0].a, A.ax[0].b, A.ax[0].k, PB(A.psi[P_DVY_DX] + (long)bx * sx, pbytes), col, jx, d1);
-        if (by >= 0) d2 = stretch<CP>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, PB(A.psi[P_DVX_DY] + (long)x * nby * nz, pyb), py_col, jy, d2);
+        if (bx >= 0) d1 = stretch<CP, FAST>(A.ax[0].a, A.ax[0].b, A.ax[0].k, A.ax[0].ki, PB(A.psi[P_DVY_DX] + (long)bx * sx, pbytes), col, jx, d1);
+        if (by >= 0) d2 = stretch<CP, FAST>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, A.ax[1].khi, PB(A.psi[P_DVX_DY] + (long)x * nby * nz, pyb), py_col, jy, d2);
         rho4 = 0.25f * (rho_n + rho0 + rhoP.ld(col + sy) + rho1P.ld(col + sy));
         cs4 = 0.25f * (cs_n + cs0 + csP.ld(col + sy) + cs1P.ld(col + sy));
         sig = A.sxy; rr = A.r[3];
       } else if (sh == 1) {     // sigma_xz: dvz/dx (D-), dvx/dz (D+)
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
-          d1 += A.C[c] * (wvz[ORD + c] - wvz[ORD - 1 - c]) / A.dx;         // vz[x+c] - vz[x-c-1]
-          d2 += A.C[c] * (vxP.ld(col + c + 1) - vxP.ld(col - c)) / A.dz;
+          d1 += dterm<FAST, 0>(A, c, wvz[ORD + c], wvz[ORD - 1 - c]);         // vz[x+c] - vz[x-c-1]
+          d2 += dterm<FAST, 2>(A, c, vxP.ld(col + c + 1), vxP.ld(col - c));
         }
-        if (bx >= 0) d1 = stretch<CP>(A.ax[0].a, A.ax[0].b, A.ax[0].k, PB(A.psi[P_DVZ_DX] + (long)bx * sx, pbytes), col, jx, d1);
-        if (bz >= 0) d2 = stretch<CP>(A.ax[2].ah, A.ax[2].bh, A.ax[2].kh, PB(A.psi[P_DVX_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, d2);
+        if (bx >= 0) d1 = stretch<CP, FAST>(A.ax[0].a, A.ax[0].b, A.ax[0].k, A.ax[0].ki, PB(A.psi[P_DVZ_DX] + (long)bx * sx, pbytes), col, jx, d1);
+        if (bz >= 0) d2 = stretch<CP, FAST>(A.ax[2].ah, A.ax[2].bh, A.ax[2].kh, A.ax[2].khi, PB(A.psi[P_DVX_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, d2);
         rho4 = 0.25f * (rho_n + rho0 + rhoP.ld(col + 1) + rho1P.ld(col + 1));
         cs4 = 0.25f * (cs_n + cs0 + csP.ld(col + 1) + cs1P.ld(col + 1));
         sig = A.sxz; rr = A.r[4];
       } else {                  // sigma_yz: dvz/dy (D+), dvy/dz (D+)
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
-          d1 += A.C[c] * (vzP.ld(col + (c + 1) * sy) - vzP.ld(col - c * sy)) / A.dy;
-          d2 += A.C[c] * (vyP.ld(col + c + 1) - vyP.ld(col - c)) / A.dz;
+          d1 += dterm<FAST, 1>(A, c, vzP.ld(col + (c + 1) * sy), vzP.ld(col - c * sy));
+          d2 += dterm<FAST, 2>(A, c, vyP.ld(col + c + 1), vyP.ld(col - c));
         }
-        if (by >= 0) d1 = stretch<CP>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, PB(A.psi[P_DVZ_DY] + (long)x * nby * nz, pyb), py_col, jy, d1);
-        if (bz >= 0) d2 = stretch<CP>(A.ax[2].ah, A.ax[2].bh, A.ax[2].kh, PB(A.psi[P_DVY_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, d2);
+        if (by >= 0) d1 = stretch<CP, FAST>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, A.ax[1].khi, PB(A.psi[P_DVZ_DY] + (long)x * nby * nz, pyb), py_col, jy, d1);
+        if (bz >= 0) d2 = stretch<CP, FAST>(A.ax[2].ah, A.ax[2].bh, A.ax[2].kh, A.ax[2].khi, PB(A.psi[P_DVY_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, d2);
         rho4 = 0.25f * (rhoP.ld(col + sy) + rho0 + rhoP.ld(col + 1) + rhoP.ld(col + sy + 1));
         cs4 = 0.25f * (csP.ld(col + sy) + cs0 + csP.ld(col + 1) + csP.ld(col + sy + 1));
         sig = A.syz; rr = A.r[5];
@@ -231,7 +242,7 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
   }
 }
 
-template <int ORD, int TY, bool NT>
+template <int ORD, int TY, bool NT, bool FAST>
 __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, int nzb, int nyb, int nxc, int remap) {
   constexpr int CP = NT ? 2 : 0;
   const Tile T = tile_of_block(nzb, nyb, nxc, remap);
@@ -287,13 +298,13 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
         float a = 0.f, b = 0.f, c_ = 0.f;
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
-          a += A.C[c] * (wxx[ORD + c] - wxx[ORD - 1 - c]) / A.dx;          // sxx[x+c] - sxx[x-c-1]
-          b += A.C[c] * (sxyP.ld(col + c * sy) - sxyP.ld(col - (c + 1) * sy)) / A.dy;
-          c_ += A.C[c] * (sxzP.ld(col + c) - sxzP.ld(col - c - 1)) / A.dz;
+          a += dterm<FAST, 0>(A, c, wxx[ORD + c], wxx[ORD - 1 - c]);          // sxx[x+c] - sxx[x-c-1]
+          b += dterm<FAST, 1>(A, c, sxyP.ld(col + c * sy), sxyP.ld(col - (c + 1) * sy));
+          c_ += dterm<FAST, 2>(A, c, sxzP.ld(col + c), sxzP.ld(col - c - 1));
         }
-        if (bx >= 0) a = stretch<CP>(A.ax[0].a, A.ax[0].b, A.ax[0].k, PB(A.psi[P_DSXX_DX] + (long)bx * sx, pbytes), col, jx, a);
-        if (by >= 0) b = stretch<CP>(A.ax[1].a, A.ax[1].b, A.ax[1].k, PB(A.psi[P_DSXY_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
-        if (bz >= 0) c_ = stretch<CP>(A.ax[2].a, A.ax[2].b, A.ax[2].k, PB(A.psi[P_DSXZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
+        if (bx >= 0) a = stretch<CP, FAST>(A.ax[0].a, A.ax[0].b, A.ax[0].k, A.ax[0].ki, PB(A.psi[P_DSXX_DX] + (long)bx * sx, pbytes), col, jx, a);
+        if (by >= 0) b = stretch<CP, FAST>(A.ax[1].a, A.ax[1].b, A.ax[1].k, A.ax[1].ki, PB(A.psi[P_DSXY_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
+        if (bz >= 0) c_ = stretch<CP, FAST>(A.ax[2].a, A.ax[2].b, A.ax[2].k, A.ax[2].ki, PB(A.psi[P_DSXZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
         const float rho = 0.5f * (rho_n + rho0);
         if (rho > 0.f) vxn = (a + b + c_) * dt / rho + vxn;
       }
@@ -303,13 +314,13 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
         float a = 0.f, b = 0.f, c_ = 0.f;
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
-          a += A.C[c] * (wxy[ORD + c] - wxy[ORD - 1 - c]) / A.dx;          // sxy[x+c+1] - sxy[x-c]
-          b += A.C[c] * (syyP.ld(col + (c + 1) * sy) - syyP.ld(col - c * sy)) / A.dy;
-          c_ += A.C[c] * (syzP.ld(col + c) - syzP.ld(col - c - 1)) / A.dz;
+          a += dterm<FAST, 0>(A, c, wxy[ORD + c], wxy[ORD - 1 - c]);          // sxy[x+c+1] - sxy[x-c]
+          b += dterm<FAST, 1>(A, c, syyP.ld(col + (c + 1) * sy), syyP.ld(col - c * sy));
+          c_ += dterm<FAST, 2>(A, c, syzP.ld(col + c), syzP.ld(col - c - 1));
         }
-        if (bx >= 0) a = stretch<CP>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, PB(A.psi[P_DSXY_DX] + (long)bx * sx, pbytes), col, jx, a);
-        if (by >= 0) b = stretch<CP>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, PB(A.psi[P_DSYY_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
-        if (bz >= 0) c_ = stretch<CP>(A.ax[2].a, A.ax[2].b, A.ax[2].k, PB(A.psi[P_DSYZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
+        if (bx >= 0) a = stretch<CP, FAST>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, A.ax[0].khi, PB(A.psi[P_DSXY_DX] + (long)bx * sx, pbytes), col, jx, a);
+        if (by >= 0) b = stretch<CP, FAST>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, A.ax[1].khi, PB(A.psi[P_DSYY_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
+        if (bz >= 0) c_ = stretch<CP, FAST>(A.ax[2].a, A.ax[2].b, A.ax[2].k, A.ax[2].ki, PB(A.psi[P_DSYZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
         const float rho = 0.5f * (rhoP.ld(col + sy) + rho0);
         if (rho > 0.f) vyn = (a + b + c_) * dt / rho + vyn;
       }
@@ -318,13 +329,13 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
         float a = 0.f, b = 0.f, c_ = 0.f;
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
-          a += A.C[c] * (wxz[ORD + c] - wxz[ORD - 1 - c]) / A.dx;          // sxz[x+c+1] - sxz[x-c]
-          b += A.C[c] * (syzP.ld(col + c * sy) - syzP.ld(col - (c + 1) * sy)) / A.dy;
-          c_ += A.C[c] * (szzP.ld(col + c + 1) - szzP.ld(col - c)) / A.dz;
+          a += dterm<FAST, 0>(A, c, wxz[ORD + c], wxz[ORD - 1 - c]);          // sxz[x+c+1] - sxz[x-c]
+          b += dterm<FAST, 1>(A, c, syzP.ld(col + c * sy), syzP.ld(col - (c + 1) * sy));
+          c_ += dterm<FAST, 2>(A, c, szzP.ld(col + c + 1), szzP.ld(col - c));
         }
-        if (bx >= 0) a = stretch<CP>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, PB(A.psi[P_DSXZ_DX] + (long)bx * sx, pbytes), col, jx, a);
-        if (by >= 0) b = stretch<CP>(A.ax[1].a, A.ax[1].b, A.ax[1].k, PB(A.psi[P_DSYZ_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
-        if (bz >= 0) c_ = stretch<CP>(A.ax[2].ah, A.ax[2].bh, A.ax[2].kh, PB(A.psi[P_DSZZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
+        if (bx >= 0) a = stretch<CP, FAST>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, A.ax[0].khi, PB(A.psi[P_DSXZ_DX] + (long)bx * sx, pbytes), col, jx, a);
+        if (by >= 0) b = stretch<CP, FAST>(A.ax[1].a, A.ax[1].b, A.ax[1].k, A.ax[1].ki, PB(A.psi[P_DSYZ_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
+        if (bz >= 0) c_ = stretch<CP, FAST>(A.ax[2].ah, A.ax[2].bh, A.ax[2].kh, A.ax[2].khi, PB(A.psi[P_DSZZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
         const float rho = 0.5f * (rhoP.ld(col + sy) + rho0);
         if (rho > 0.f) vzn = (a + b + c_) * dt / rho + vzn;
       } else {
@@ -355,50 +366,47 @@ Launch launch_shape(const StepArgs &A, int ty, int xchunk) {
   return L;
 }
 
-template <int ORD, int TY, bool NT>
+template <int ORD, int TY, bool NT, bool FAST>
 hipError_t launch_one(bool stress, int visco, const StepArgs &A, int xchunk, int remap, hipStream_t st) {
   const Launch L = launch_shape(A, TY, xchunk);
   const dim3 grid((unsigned)(L.nzb * L.nyb * L.nxc)), block(BZ, TY);
   if (stress) {
-    if (visco) hipLaunchKernelGGL((k_stress3d<true, ORD, TY, NT>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
-    else hipLaunchKernelGGL((k_stress3d<false, ORD, TY, NT>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
+    if (visco) hipLaunchKernelGGL((k_stress3d<true, ORD, TY, NT, FAST>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
+    else hipLaunchKernelGGL((k_stress3d<false, ORD, TY, NT, FAST>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
   } else {
-    hipLaunchKernelGGL((k_velocity3d<ORD, TY, NT>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
+    hipLaunchKernelGGL((k_velocity3d<ORD, TY, NT, FAST>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
   }
   return hipGetLastError();
 }
 
-template <int ORD>
+template <int ORD, bool FAST>
 hipError_t launch_tuned(bool stress, int visco, const StepArgs &A, const KTune &t, hipStream_t st) {
-  // shapes compiled for every order: (TY, NT) in {1,2,4,8} x {0,1}
-#define VSE_TY(TYV)                                                                          \
-  case TYV:                                                                                  \
-    return t.nt ? launch_one<ORD, TYV, true>(stress, visco, A, t.xchunk, t.remap, st)        \
-                : launch_one<ORD, TYV, false>(stress, visco, A, t.xchunk, t.remap, st);
-  switch (t.ty) {
-    VSE_TY(1) VSE_TY(2) VSE_TY(4) VSE_TY(8)
-    default: return hipErrorInvalidValue;
-  }
-#undef VSE_TY
+  // compiled shapes: TY in {1, 4} x non-temporal on/off
+  if (t.ty == 1) return t.nt ? launch_one<ORD, 1, true, FAST>(stress, visco, A, t.xchunk, t.remap, st)
+                             : launch_one<ORD, 1, false, FAST>(stress, visco, A, t.xchunk, t.remap, st);
+  if (t.ty == 4) return t.nt ? launch_one<ORD, 4, true, FAST>(stress, visco, A, t.xchunk, t.remap, st)
+                             : launch_one<ORD, 4, false, FAST>(stress, visco, A, t.xchunk, t.remap, st);
+  return hipErrorInvalidValue;
 }
 
+template <bool FAST>
 hipError_t dispatch(bool stress, int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st) {
   switch (ord) {
-    case 2: return launch_tuned<2>(stress, visco, A, t, st);
-    case 3: return launch_tuned<3>(stress, visco, A, t, st);
-    case 4: return launch_tuned<4>(stress, visco, A, t, st);
-    case 5: return launch_tuned<5>(stress, visco, A, t, st);
-    case 6: return launch_tuned<6>(stress, visco, A, t, st);
+    case 2: return launch_tuned<2, FAST>(stress, visco, A, t, st);
+    case 3: return launch_tuned<3, FAST>(stress, visco, A, t, st);
+    case 4: return launch_tuned<4, FAST>(stress, visco, A, t, st);
+    case 5: return launch_tuned<5, FAST>(stress, visco, A, t, st);
+    case 6: return launch_tuned<6, FAST>(stress, visco, A, t, st);
     default: return hipErrorInvalidValue;
   }
 }
 
 }  // namespace
 
-hipError_t vse_launch_stress3d(int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st) {
-  return dispatch(true, visco, ord, A, t, st);
+hipError_t vse_launch_stress3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st) {
+  return math == VSE_MATH_FAST ? dispatch<true>(true, visco, ord, A, t, st) : dispatch<false>(true, visco, ord, A, t, st);
 }
 
-hipError_t vse_launch_velocity3d(int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st) {
-  return dispatch(false, visco, ord, A, t, st);
+hipError_t vse_launch_velocity3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st) {
+  return math == VSE_MATH_FAST ? dispatch<true>(false, visco, ord, A, t, st) : dispatch<false>(false, visco, ord, A, t, st);
 }

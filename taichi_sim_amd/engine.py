@@ -197,6 +197,10 @@ class VSESim:
         """Launch shape of the 3D stress (kernel 0) or velocity (1) pass."""
         self._check(self._lib.vse_set_tuning(self._h, int(kernel), int(ty), int(nt), int(xchunk), int(remap)))
 
+    def set_math(self, mode):
+        """'strict' (reference operation order, default) or 'fast' (3D kernels)."""
+        self._check(self._lib.vse_set_math(self._h, {"strict": 0, "fast": 1}[mode]))
+
     def band_cells(self):
         """Cells per psi slot-axis band: {axis: band width} from the CPML profiles."""
         return dict(self._bands)

@@ -59,7 +59,7 @@ def main():
         rc = _lib.load().vse_copy_bandwidth(1 << 30, 5, ctypes.byref(gbs))
         print(json.dumps({"copy_bytes_per_launch": 2 << 30, "launches": 6, "gbs": round(gbs.value, 1), "rc": rc}))
         sys.exit(0 if rc == 0 else 1)
-    from taichi_sim_amd import workloads
+    from taichi_sim_amd import workloads, autotune
     from taichi_sim_amd.backend import attenuation_from_config
     from taichi_sim_amd.engine import VSESim
     from taichi_sim_amd.sim_support.simulator3d import Simulator3D
@@ -73,11 +73,32 @@ def main():
     print(f"# setup done: grid {g.shape}, {g.device_bytes() / 2**30:.1f} GiB on {g.device_name()}",
           file=sys.stderr, flush=True)
 
-    if args.warmup:
-        g.run(args.warmup, check_every=args.warmup)
-    t0 = time.perf_counter()
-    dev_ms = g.run(args.steps, check_every=500, timed=True)      # synchronises before returning
-    wall = time.perf_counter() - t0
+    # launch shapes from the autotuner (before the timed region; it advances
+    # the state, so the simulation restarts from zero afterwards)
+    cands = [autotune.PLAIN] + [(ty, nt, xc, rm) for ty in (1, 4) for nt in (0, 1) for xc in (4, 8, 16)
+                                for rm in (0, 1)]
+    tuned = {}
+    for math in ("fast", "strict"):
+        g.set_math(math)
+        best = autotune.tune(g, steps=2, candidates=cands, cache=None)
+        tuned[math] = {k: best[k] for k in ("stress", "velocity")}
+    print(f"# tuned launch shapes (ty, nt, xchunk, remap, ms): {tuned}", file=sys.stderr, flush=True)
+
+    def timed_run(math, steps):
+        g.set_math(math)
+        for k, name in enumerate(("stress", "velocity")):
+            g.set_tuning(k, *tuned[math][name][:4])
+        g.reset()
+        if args.warmup:
+            g.run(args.warmup, check_every=args.warmup)
+        t0 = time.perf_counter()
+        dev_ms = g.run(steps, check_every=500, timed=True)      # synchronises before returning
+        wall = time.perf_counter() - t0
+        return wall, dev_ms
+
+    wall_fast, _ = timed_run("fast", max(1, args.steps // 4))
+    gcell_fast = ncell * max(1, args.steps // 4) / wall_fast / 1e9
+    wall, dev_ms = timed_run("strict", args.steps)              # headline: reference operation order
     ms_per_step = 1e3 * wall / args.steps
     gcell = ncell * args.steps / wall / 1e9
     print(f"# timed {args.steps} steps: wall {wall:.3f} s, device {dev_ms / 1e3:.3f} s",
@@ -120,6 +141,9 @@ def main():
         "config": {"workload": f"3D viscoelastic Zener FD {args.n}^3 (incl. 20-cell CPML), L=3 Q=50, "
                                f"Al over steel with 24 seeded inclusions, piston source, 32x32 receivers",
                    "grid": [args.n] * 3, "n_sls": 3, "fd_order": 2, "parallelism": "single GPU"},
+        "math": "strict (reference FP32 operation order)",
+        "value_fast_math": round(gcell_fast, 4),
+        "launch_shapes": tuned,
         "hbm_frac_step": round(step_frac, 4),
         "copy_kernel_gbs": round(copy_gbs.value, 1),
         "hbm_frac_step_vs_copy": round(gcell * b_step / copy_gbs.value, 4),

@@ -15,6 +15,7 @@ DEFAULT_CACHE = os.environ.get("VSE_TUNE_CACHE", os.path.join(os.path.dirname(os
 TY = (1, 4)
 NT = (0, 1)
 XCHUNK = (1, 2, 4, 8, 16, 32)
+PLAIN = (4, 0, 0, 0)      # xchunk 0: one cell per thread
 REMAP = (0, 1)
 
 
@@ -45,7 +46,7 @@ def tune(g, steps=3, candidates=None, cache=DEFAULT_CACHE, verbose=False, use_ca
             ty, nt, xc, rm, _ = best[name]
             g.set_tuning(k, ty, nt, xc, rm)
         return best
-    cands = candidates or [(ty, nt, xc, rm) for ty in TY for nt in NT for xc in XCHUNK for rm in REMAP]
+    cands = candidates or ([PLAIN] + [(ty, nt, xc, rm) for ty in TY for nt in NT for xc in XCHUNK for rm in REMAP])
     table = []
     best = {}
     for k, name in enumerate(("stress", "velocity")):

@@ -52,7 +52,7 @@ struct vse_sim {
   int *rec_beg = nullptr, *rec_cell = nullptr, *rec_delay = nullptr;
   float *traces = nullptr;
   // launch shapes of the 3D stress / velocity kernels
-  KTune tune[2] = {{4, 0, 1, 0}, {4, 0, 1, 0}};
+  KTune tune[2] = {{4, 0, 0, 0}, {4, 0, 8, 1}};
   // cached graph
   hipGraphExec_t gexec = nullptr;
   int gexec_steps = 0;
@@ -505,8 +505,8 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
 
 int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap) {
   if (check_handle(s) || kernel < 0 || kernel > 1) return VSE_ERR_ARG;
-  if (!(ty == 1 || ty == 4) || (nt != 0 && nt != 1) || xchunk < 1 || (remap != 0 && remap != 1))
-    return s->fail(VSE_ERR_ARG, "tuning: ty in {1,4}, nt in {0,1}, xchunk >= 1, remap in {0,1}");
+  if (!(ty == 1 || ty == 4) || (nt != 0 && nt != 1) || xchunk < 0 || (remap != 0 && remap != 1))
+    return s->fail(VSE_ERR_ARG, "tuning: ty in {1,4}, nt in {0,1}, xchunk >= 0, remap in {0,1}");
   s->tune[kernel] = KTune{ty, nt, xchunk, remap};
   s->drop_graph();
   return VSE_OK;

@@ -82,7 +82,7 @@ def main():
         g.set_math(math)
         best = autotune.tune(g, steps=2, candidates=cands, cache=None)
         tuned[math] = {k: best[k] for k in ("stress", "velocity")}
-    print(f"# tuned launch shapes (ty, nt, xchunk, remap, ms): {tuned}", file=sys.stderr, flush=True)
+    print(f"# tuned launch shapes (ty, nt, xchunk, remap, step ms): {tuned}", file=sys.stderr, flush=True)
 
     def timed_run(math, steps):
         g.set_math(math)

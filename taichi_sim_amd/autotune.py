@@ -1,8 +1,10 @@
 """Autotuner for the 3D kernels' launch shapes.
 
 For one simulation handle it times every candidate (tile height ty, non-
-temporal streaming nt, x-chunk length) of the stress and velocity passes with
-HIP events (``VSESim.profile``), keeps the fastest per pass, and caches the
+temporal streaming nt, x-chunk length, tile order) of the stress and velocity
+passes with HIP events (``VSESim.profile``) and keeps the one with the fastest
+*step* (stress + velocity back to back: one pass's speed depends on what the
+other leaves in L2 / MALL), tuning stress first, then velocity, and caches the
 choice per problem key (ndim, grid, FD order, visco, L) in a JSON file.  All
 shapes compute identical results, so tuning only changes speed.  The handle's
 state advances while tuning; call ``reset()`` before a measured run.
@@ -34,7 +36,7 @@ def _load(path):
 def tune(g, steps=3, candidates=None, cache=DEFAULT_CACHE, verbose=False, use_cache=True):
     """Pick and apply the fastest launch shape of each 3D pass of ``g``.
 
-    Returns {"stress": (ty, nt, xchunk, remap, ms), "velocity": (...), "table": [...]}.
+    Returns {"stress": (ty, nt, xchunk, remap, step_ms), "velocity": (...), "table": [...]}.
     """
     if g.ndim != 3:
         return {}
@@ -53,7 +55,8 @@ def tune(g, steps=3, candidates=None, cache=DEFAULT_CACHE, verbose=False, use_ca
         results = []
         for ty, nt, xc, rm in cands:
             g.set_tuning(k, ty, nt, xc, rm)
-            ms = g.profile(steps)[k]
+            t = g.profile(steps)
+            ms = t[0] + t[1]
             results.append((ms, ty, nt, xc, rm))
             table.append((name, ty, nt, xc, rm, ms))
             if verbose:

@@ -1,11 +1,18 @@
 """Headline benchmark: 3D viscoelastic (Zener L=3) FP32 time loop, 512^3 cells.
 
-Workload (BASELINE config "3D viscoelastic Zener FD 512^3"): 512^3 full grid
+Workload "3D viscoelastic Zener FD 512^3": 512^3 full grid
 incl. 20-cell CPML on every face, 0.1 mm cells, aluminium over steel with
 seeded ellipsoidal density inclusions, L=3 mechanisms fitted to Q=50 over
 2-8 MHz, 5 MHz circular piston on the top face, 32x32 surface receivers,
 dt at CFL 0.5.  A step = stress pass + velocity pass + source/receiver pass
 over the whole grid; inputs are resident in HBM before the timed region.
+
+``--workload shots`` measures the multi-shot path instead: a full-matrix
+capture on 4 steel plates with voids (512^2, 1024^2, 1536^2, 2048^2, Zener
+L=3, 16-element array, shot k fires element k), all 64 simulations advanced
+as one batch (one launch per pass per step), against the same 64 simulations
+run one after another as the reference runs its emission laws.
+value = 3000-step shots per second.
 
 Prints one JSON line.  value = Gcell-updates/s of the whole job (one GPU).
 roofline: the dominant kernel (stress pass) -- algorithmic bytes per launch /
@@ -43,6 +50,8 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU-oracle sample (0: skip)")
+    ap.add_argument("--workload", choices=("3d", "shots"), default="3d")
+    ap.add_argument("--shots", type=int, default=16, help="array elements = FMC shots per specimen (--workload shots)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 5 launches): rocprof counter calibration")
     args = ap.parse_args()
@@ -59,6 +68,8 @@ def main():
         rc = _lib.load().vse_copy_bandwidth(1 << 30, 5, ctypes.byref(gbs))
         print(json.dumps({"copy_bytes_per_launch": 2 << 30, "launches": 6, "gbs": round(gbs.value, 1), "rc": rc}))
         sys.exit(0 if rc == 0 else 1)
+    if args.workload == "shots":
+        return bench_shots(args)
     from taichi_sim_amd import workloads, autotune
     from taichi_sim_amd.backend import attenuation_from_config
     from taichi_sim_amd.engine import VSESim
@@ -153,6 +164,83 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "stress3d", "bytes_per_cell": round(b_stress, 2)},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_shots(args):
+    """Batched multi-shot 2D throughput vs the per-shot loop (see module doc)."""
+    from taichi_sim_amd import workloads
+    from taichi_sim_amd.backend import attenuation_from_config
+    from taichi_sim_amd.engine import VSEBatch, VSESim
+    from taichi_sim_amd.sim_support.simulator import Simulator
+
+    shot_len = 3000                                   # steps of one shot
+    sizes = (512, 1024, 1536, 2048)
+    steps = args.steps
+    t_setup = time.perf_counter()
+    sims, cells = [], 0
+    for cfg, laws in workloads.fmc_2d(sizes=sizes, n_elem=args.shots, steps=shot_len):
+        sim = Simulator(cfg)
+        sim._emission_laws = laws
+        att = attenuation_from_config(cfg)
+        for law in range(args.shots):
+            sim.select_law(law)
+            sims.append(VSESim.from_simulator(sim, variant="wgsl", att=att))
+            cells += int(np.prod(sims[-1].shape))
+    gib = sum(g.device_bytes() for g in sims) / 2**30
+    print(f"# setup: {len(sims)} simulations, {cells / 1e6:.0f} Mcell, {gib:.1f} GiB, "
+          f"{time.perf_counter() - t_setup:.0f} s", file=sys.stderr, flush=True)
+
+    batch = VSEBatch(sims)
+    groups = batch.groups()
+    k_batch = batch.profile(max(1, args.profile_steps))        # all members in one launch per pass
+    k_solo = [0.0, 0.0, 0.0]
+    for g in sims:
+        t = g.profile(max(1, args.profile_steps))
+        k_solo = [a + b for a, b in zip(k_solo, t)]
+
+    def timed(fn):
+        # both paths run hipGraphs of min(steps, 256) steps; an untimed call of
+        # that length captures them first
+        for g in sims:
+            g.reset()
+        fn(min(steps, 256))
+        for g in sims:
+            g.reset()
+        t0 = time.perf_counter()
+        dev = fn(steps)
+        return time.perf_counter() - t0, dev
+
+    check = min(steps, 500)
+    wall_b, dev_b = timed(lambda n: batch.run(n, check_every=check, timed=True))
+    print(f"# batch ({len(groups)} groups): {steps} steps in {wall_b:.3f} s (device {dev_b / 1e3:.3f} s)",
+          file=sys.stderr, flush=True)
+    wall_l, _ = timed(lambda n: [g.run(n, check_every=check) for g in sims])
+    print(f"# per-shot loop: {wall_l:.3f} s", file=sys.stderr, flush=True)
+    batch.close()
+    name = sims[0].device_name()
+    for g in sims:
+        g.close()
+
+    shots = len(sims) * steps / shot_len
+    line = {
+        "metric": "shots/s (2D viscoelastic Zener L=3 FP32 FMC, 3000-step shots, batched)",
+        "value": round(shots / wall_b, 4), "unit": "shots/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall_b / steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"FMC: {len(sizes)} steel plates with voids {'/'.join(f'{n}^2' for n in sizes)} "
+                               f"(incl. CPML) x {args.shots} shots ({args.shots}-element array, all receiving)",
+                   "shots": len(sims), "cells_per_step": cells, "shot_steps": shot_len,
+                   "parallelism": "single GPU, one batch", "cache_groups": len(groups)},
+        "per_shot_loop_shots_per_s": round(shots / wall_l, 4),
+        "batch_speedup": round(wall_l / wall_b, 3),
+        "gcell_per_s_batched": round(cells * steps / wall_b / 1e9, 3),
+        "gcell_per_s_loop": round(cells * steps / wall_l / 1e9, 3),
+        "kernel_ms_all_in_one_launch": {k: round(v, 4) for k, v in zip(("stress", "velocity", "sparse"), k_batch)},
+        "kernel_ms_loop_sum": {k: round(v, 4) for k, v in zip(("stress", "velocity", "sparse"), k_solo)},
+        "device": name,
     }
     print(json.dumps(line), flush=True)
 

@@ -178,6 +178,47 @@ int vse_copy_bandwidth(size_t bytes, int iters, float *gbs);
 
 void vse_destroy(vse_sim *s);
 
+/* ---- batches: many 2D simulations per launch ---------------------------
+ * The reference runs one simulation per emission law, one after another
+ * (sim_support/simulator.py:328-337 loops over laws around implementation()).
+ * A batch advances up to VSE_BATCH_MAX 2D handles (shots of one specimen, or
+ * specimens of different sizes).  Members are packed, in the given order,
+ * into groups whose device bytes fit a cache budget (vse_batch_set_group_bytes;
+ * default VSE_BATCH_GROUP_BYTES_DEFAULT, sized for the 256 MB Infinity Cache
+ * of MI355X).  Each group runs all its steps before the next group starts,
+ * as hipGraphs of up to check_every steps in which every step is one stress,
+ * one velocity and one sparse launch over the tiles of all the group's
+ * members; a group of one member runs vse_run on it.  Members must share
+ * variant, viscoelastic flag, FD order and device; they keep their own state,
+ * so vse_read_* on a member reads its results, and every member's result is
+ * identical to vse_run on it alone.  A batch does not own its members:
+ * destroy it before them, and do not call a member while a batch call is in
+ * flight on another thread.  No reference counterpart. */
+#define VSE_BATCH_MAX 256
+#define VSE_BATCH_GROUP_BYTES_DEFAULT ((size_t)192 << 20)
+typedef struct vse_batch vse_batch;
+int vse_batch_create(vse_sim *const *sims, int n, vse_batch **out);
+/* Advance every member n_steps; VSE_ERR_UNSTABLE names the first unstable
+ * member in vse_batch_last_error() (and in that member's vse_last_error()).
+ * elapsed_ms (optional) sums the device time of every group. */
+int vse_batch_run(vse_batch *b, int n_steps, int check_every, float *elapsed_ms);
+/* Average (stress, velocity, sparse) durations in ms of one step of all
+ * members in one launch per pass (ignores the grouping), over n_steps steps,
+ * each launch bracketed by HIP events.  Advances the members. */
+int vse_batch_profile(vse_batch *b, int n_steps, float *ms);
+/* Each workgroup of a batched launch covers 64 (y) x 4*rows (x) cells of one
+ * member, walking its `rows` groups of 4 x-rows in turn (default 4, 1..64).
+ * Results are identical for every value; only speed changes. */
+int vse_batch_set_rows(vse_batch *b, int rows);
+/* Cache budget of a group in bytes of member device memory; 0 puts every
+ * member in one group (all members advance in lockstep).  Speed only. */
+int vse_batch_set_group_bytes(vse_batch *b, size_t bytes);
+/* Number of groups; writes the first member index of each (up to max_groups). */
+int vse_batch_groups(vse_batch *b, int *first, int max_groups);
+int vse_batch_size(vse_batch *b);
+const char *vse_batch_last_error(vse_batch *b);
+void vse_batch_destroy(vse_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

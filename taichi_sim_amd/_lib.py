@@ -23,7 +23,10 @@ MEMORY = {"xx": 0, "yy": 1, "zz": 2, "xy": 3, "xz": 4, "yz": 5}
 EXPORTS = ["vse_abi_version", "vse_create", "vse_set_material", "vse_set_cpml", "vse_set_sources",
            "vse_set_receivers", "vse_reset", "vse_run", "vse_profile", "vse_set_tuning", "vse_set_math", "vse_read_field", "vse_read_traces",
            "vse_read_memory", "vse_read_v2max", "vse_get_step", "vse_field_device_ptr",
-           "vse_device_bytes", "vse_device_name", "vse_last_error", "vse_destroy", "vse_copy_bandwidth"]
+           "vse_device_bytes", "vse_device_name", "vse_last_error", "vse_destroy", "vse_copy_bandwidth",
+           "vse_batch_create", "vse_batch_run", "vse_batch_profile", "vse_batch_set_rows", "vse_batch_set_group_bytes", "vse_batch_groups",
+           "vse_batch_size", "vse_batch_last_error", "vse_batch_destroy"]
+BATCH_MAX = 256
 
 
 class VseDesc(ctypes.Structure):
@@ -78,6 +81,15 @@ def load():
         "vse_last_error": (ctypes.c_char_p, [P]),
         "vse_destroy": (None, [P]),
         "vse_copy_bandwidth": (I, [ctypes.c_size_t, I, fp]),
+        "vse_batch_create": (I, [ctypes.POINTER(P), I, ctypes.POINTER(P)]),
+        "vse_batch_run": (I, [P, I, I, fp]),
+        "vse_batch_profile": (I, [P, I, fp]),
+        "vse_batch_set_rows": (I, [P, I]),
+        "vse_batch_set_group_bytes": (I, [P, ctypes.c_size_t]),
+        "vse_batch_groups": (I, [P, ip, I]),
+        "vse_batch_size": (I, [P]),
+        "vse_batch_last_error": (ctypes.c_char_p, [P]),
+        "vse_batch_destroy": (None, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -14,7 +14,9 @@ against the reference's own ``Simulator``.
 """
 from time import time
 
-from .engine import VSESim
+import numpy as np
+
+from .engine import VSEBatch, VSESim
 from .sim_support.attenuation import AttenuationCoefficients, fit_constant_q
 from .sim_support.simulator import Simulator
 from .sim_support.simulator3d import Simulator3D
@@ -50,9 +52,38 @@ class HIPBackend:
             gpu = sim.device_name()
         finally:
             sim.close()
+        return self._result(field, traces, gpu, ms / 1000.0, f"libvse {self._variant} variant, {sim.ndim}D, hipGraph step loop\n")
+
+    @staticmethod
+    def _result(field, traces, gpu, sim_time, msg):
         return {"stress": field, "sens_stress": traces, "pressure": field, "sens_pressure": traces,
-                "gpu_str": gpu, "sim_time": ms / 1000.0,
-                "msg_impl": f"libvse {self._variant} variant, {sim.ndim}D, hipGraph step loop\n"}
+                "gpu_str": gpu, "sim_time": sim_time, "msg_impl": msg}
+
+    def implementation_many(self, laws):
+        """Several emission laws of a 2D run simulated together: one libvse
+        simulation per law, advanced as one batch (one launch per pass per
+        step for all of them).  Each law's result equals implementation()'s;
+        "sim_time" is the batch's device time divided by the number of laws."""
+        if len(laws) < 2 or np.asarray(self._rho_grid_vx).ndim != 2:
+            return super().implementation_many(laws)
+        att = attenuation_from_config(self._configs)
+        sims = []
+        try:
+            for law in laws:
+                self.select_law(law)
+                print(f'Simulacao {self._name}')
+                sims.append(VSESim.from_simulator(self, variant=self._variant, att=att))
+            batch = VSEBatch(sims)
+            try:
+                ms = batch.run(self._n_steps, check_every=self.check_every, timed=True)
+            finally:
+                batch.close()
+            gpu = sims[0].device_name()
+            msg = f"libvse {self._variant} variant, 2D, batch of {len(sims)} laws, hipGraph step loop\n"
+            return [self._result(s.field("syy"), s.traces(), gpu, ms / 1000.0 / len(sims), msg) for s in sims]
+        finally:
+            for s in sims:
+                s.close()
 
 
 class SimulatorHIP(HIPBackend, Simulator):

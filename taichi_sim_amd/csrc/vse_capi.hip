@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#define VSE_HOST_SIDE
 #include "vse_internal.h"
 #include "../../include/vse.h"
 
@@ -130,7 +131,9 @@ int vse_create(const vse_desc *desc, vse_sim **out) {
   if (!rc) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, s->device) == hipSuccess)
-      s->devname = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+      s->devname = (prop.name[0] ? std::string(prop.name) : std::string("AMD GPU")) + " (" + prop.gcnArchName +
+                   ", " + std::to_string(prop.multiProcessorCount) + " CUs, " +
+                   std::to_string(prop.totalGlobalMem >> 30) + " GiB)";
   }
   for (int f = 0; f < 9 && !rc; f++) {
     const bool used = d.ndim == 3 || f == VSE_F_VX || f == VSE_F_VY || f == VSE_F_SXX || f == VSE_F_SYY ||
@@ -585,5 +588,333 @@ void vse_destroy(vse_sim *s) {
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches
+// Many 2D simulations (shots of one specimen, or specimens of different
+// sizes) advanced together.  Members are packed, in order, into groups whose
+// device state fits a cache budget (the 256 MB Infinity Cache of MI355X by
+// default); each group runs all its steps before the next one starts, with
+// one stress, one velocity and one sparse launch per step covering every
+// member of the group.  Small grids thus fill the chip together, and a
+// group's state stays cache-resident from step to step, as a lone
+// simulation's does.  A group of one member runs that member's own kernels.
+struct BatchGroup {
+  int first = 0, n = 0;
+  BatchArgs B{};
+  int total_tiles = 0;
+  hipGraphExec_t gexec = nullptr;
+  int gexec_steps = 0;
+};
+
+struct vse_batch {
+  std::vector<vse_sim *> sims;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int variant = 0, visco = 0, ord = 0;
+  int rows = 4;
+  size_t group_bytes = 0;
+  std::vector<BatchGroup> groups;
+  BatchGroup all;                                   // every member in one launch (vse_batch_profile)
+  StepArgs *d_args = nullptr;
+  SparseArgs *d_sparse = nullptr;
+  unsigned *d_ctrl = nullptr, *h_ctrl = nullptr;   // (v2max bits, step) per member
+  std::string err;
+
+  int fail(int code, const std::string &m) { err = m; return code; }
+  int hip(hipError_t e, const char *what) {
+    if (e == hipSuccess) return 0;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return VSE_ERR_HIP;
+  }
+  void drop_graphs() {
+    for (auto &g : groups)
+      if (g.gexec) { (void)hipGraphExecDestroy(g.gexec); g.gexec = nullptr; g.gexec_steps = 0; }
+  }
+};
+
+static int build_group(vse_batch *b, BatchGroup &g, int first, int n) {
+  g.first = first;
+  g.n = n;
+  g.B = BatchArgs{};
+  g.B.args = b->d_args + first;
+  g.B.sparse = b->d_sparse + first;
+  g.B.n = n;
+  g.B.rows = b->rows;
+  long tiles = 0;
+  for (int i = 0; i < n; i++) {
+    g.B.tile_beg[i] = (int)tiles;
+    tiles += vse_tiles2d(b->sims[first + i]->d.nx, b->sims[first + i]->d.ny, b->rows);
+    if (tiles > 0x7fffffffL) return b->fail(VSE_ERR_ARG, "batch too large for one launch");
+  }
+  g.B.tile_beg[n] = (int)tiles;
+  g.total_tiles = (int)tiles;
+  return 0;
+}
+
+// Greedy packing in member order: a group closes when the next member would
+// take its device bytes past group_bytes (0: one group of every member).
+static int regroup(vse_batch *b) {
+  b->drop_graphs();
+  b->groups.clear();
+  const int n = (int)b->sims.size();
+  int rc = build_group(b, b->all, 0, n);
+  int first = 0;
+  size_t bytes = 0;
+  for (int i = 0; i < n && !rc; i++) {
+    const size_t mb = b->sims[i]->bytes;
+    if (b->group_bytes && i > first && bytes + mb > b->group_bytes) {
+      b->groups.emplace_back();
+      rc = build_group(b, b->groups.back(), first, i - first);
+      first = i;
+      bytes = 0;
+    }
+    bytes += mb;
+  }
+  if (!rc) {
+    b->groups.emplace_back();
+    rc = build_group(b, b->groups.back(), first, n - first);
+  }
+  return rc;
+}
+
+static hipError_t enqueue_group_step(vse_batch *b, const BatchGroup &g) {
+  hipError_t e = vse_launch_batch2d(true, b->variant, b->visco, b->ord, g.B, g.total_tiles, b->stream);
+  if (e == hipSuccess) e = vse_launch_batch2d(false, b->variant, b->visco, b->ord, g.B, g.total_tiles, b->stream);
+  if (e == hipSuccess) e = vse_launch_sparse_batch(g.B, b->stream);
+  return e;
+}
+
+// Members' argument blocks as of now (their device buffers may have been
+// replaced by vse_set_sources / vse_set_receivers since the last run).
+static int upload_batch_args(vse_batch *b) {
+  const int n = (int)b->sims.size();
+  std::vector<StepArgs> A(n);
+  std::vector<SparseArgs> S(n);
+  for (int i = 0; i < n; i++) {
+    vse_sim *s = b->sims[i];
+    int rc = finalize(s);
+    if (rc) return b->fail(rc, "member " + std::to_string(i) + ": " + s->err);
+    rc = s->hip(hipStreamSynchronize(s->stream), "member sync");
+    if (rc) return b->fail(rc, "member " + std::to_string(i) + ": " + s->err);
+    make_args(s, A[i], S[i]);
+  }
+  int rc = b->hip(hipMemcpyAsync(b->d_args, A.data(), n * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream), "upload args");
+  if (!rc) rc = b->hip(hipMemcpyAsync(b->d_sparse, S.data(), n * sizeof(SparseArgs), hipMemcpyHostToDevice, b->stream), "upload args");
+  if (!rc) rc = b->hip(hipStreamSynchronize(b->stream), "upload args");
+  return rc;
+}
+
+// Stability of the members of g (device max of |v|^2 reached +inf or NaN).
+static int check_group(vse_batch *b, const BatchGroup &g) {
+  unsigned *dc = b->d_ctrl + 2 * g.first, *hc = b->h_ctrl + 2 * g.first;
+  int rc = b->hip(vse_launch_batch_ctrl(g.B, dc, b->stream), "ctrl gather");
+  if (!rc) rc = b->hip(hipMemcpyAsync(hc, dc, 2 * g.n * sizeof(unsigned), hipMemcpyDeviceToHost, b->stream), "read ctrl");
+  if (!rc) rc = b->hip(hipStreamSynchronize(b->stream), "sync");
+  if (rc) return rc;
+  for (int i = 0; i < g.n; i++)
+    if (hc[2 * i] >= 0x7f800000u) {
+      float v;
+      memcpy(&v, &hc[2 * i], 4);
+      char buf[160];
+      snprintf(buf, sizeof buf, "Simulacao tornando-se instavel [member %d: |v|^2 = %g at step %u]", g.first + i, v,
+               hc[2 * i + 1]);
+      b->sims[g.first + i]->err = buf;
+      return b->fail(VSE_ERR_UNSTABLE, buf);
+    }
+  return 0;
+}
+
+// n_steps of one multi-member group on the batch stream, in hipGraphs of up
+// to check_every steps.
+static int run_group(vse_batch *b, BatchGroup &g, int n_steps, int check_every, float *elapsed_ms) {
+  int rc = 0;
+  const int chunk = std::max(1, std::min(check_every > 0 ? check_every : 256, std::min(n_steps, 256)));
+  if (!(g.gexec && g.gexec_steps == chunk)) {
+    if (g.gexec) (void)hipGraphExecDestroy(g.gexec);
+    g.gexec = nullptr;
+    g.gexec_steps = 0;
+    hipGraph_t gr = nullptr;
+    rc = b->hip(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < chunk && e == hipSuccess; i++) e = enqueue_group_step(b, g);
+    hipError_t e2 = hipStreamEndCapture(b->stream, &gr);
+    if (e != hipSuccess) { if (gr) (void)hipGraphDestroy(gr); return b->hip(e, "capture launch"); }
+    rc = b->hip(e2, "end capture");
+    if (!rc) rc = b->hip(hipGraphInstantiate(&g.gexec, gr, nullptr, nullptr, 0), "graph instantiate");
+    if (gr) (void)hipGraphDestroy(gr);
+    if (rc) return rc;
+    g.gexec_steps = chunk;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (elapsed_ms) {
+    rc = b->hip(hipEventCreate(&e0), "event");
+    if (!rc) rc = b->hip(hipEventCreate(&e1), "event");
+    if (!rc) rc = b->hip(hipEventRecord(e0, b->stream), "event record");
+  }
+  int done = 0, since_check = 0;
+  while (!rc && done < n_steps) {
+    const int left = n_steps - done;
+    if (left >= chunk) {
+      rc = b->hip(hipGraphLaunch(g.gexec, b->stream), "graph launch");
+      done += chunk;
+      since_check += chunk;
+    } else {
+      for (int i = 0; i < left && !rc; i++) rc = b->hip(enqueue_group_step(b, g), "step launch");
+      done += left;
+      since_check += left;
+    }
+    if (!rc && check_every > 0 && since_check >= check_every && done < n_steps) {
+      rc = check_group(b, g);
+      since_check = 0;
+    }
+  }
+  if (elapsed_ms && !rc) {
+    rc = b->hip(hipEventRecord(e1, b->stream), "event record");
+    if (!rc) rc = b->hip(hipEventSynchronize(e1), "event sync");
+    if (!rc) rc = b->hip(hipEventElapsedTime(elapsed_ms, e0, e1), "elapsed");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (!rc) rc = check_group(b, g);
+  else (void)hipStreamSynchronize(b->stream);
+  return rc;
+}
+
+extern "C" {
+
+void vse_batch_destroy(vse_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  b->drop_graphs();
+  if (b->d_args) (void)hipFree(b->d_args);
+  if (b->d_sparse) (void)hipFree(b->d_sparse);
+  if (b->d_ctrl) (void)hipFree(b->d_ctrl);
+  if (b->h_ctrl) (void)hipHostFree(b->h_ctrl);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int vse_batch_create(vse_sim *const *sims, int n, vse_batch **out) {
+  static_assert(VSE_MAX_BATCH == VSE_BATCH_MAX, "batch limit");
+  if (!sims || !out || n < 1 || n > VSE_BATCH_MAX) return VSE_ERR_ARG;
+  *out = nullptr;
+  vse_sim *s0 = sims[0];
+  if (!s0) return VSE_ERR_ARG;
+  for (int i = 0; i < n; i++) {
+    vse_sim *s = sims[i];
+    if (!s || s->d.ndim != 2 || s->d.variant != s0->d.variant || s->d.visco != s0->d.visco ||
+        s->d.ord != s0->d.ord || s->device != s0->device)
+      return VSE_ERR_ARG;   // members: 2D, one device, one kernel instantiation
+    for (int j = 0; j < i; j++)
+      if (sims[j] == s) return VSE_ERR_ARG;
+  }
+  vse_batch *b = new vse_batch();
+  b->sims.assign(sims, sims + n);
+  b->device = s0->device;
+  b->variant = s0->d.variant;
+  b->visco = s0->d.visco;
+  b->ord = s0->d.ord;
+  b->group_bytes = VSE_BATCH_GROUP_BYTES_DEFAULT;
+  int cur = 0;
+  int rc = b->hip(hipGetDevice(&cur), "hipGetDevice");
+  if (!rc && cur != b->device) rc = b->fail(VSE_ERR_ARG, "batch must be created on its members' device");
+  if (!rc) rc = b->hip(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+  if (!rc) rc = b->hip(hipMalloc((void **)&b->d_args, n * sizeof(StepArgs)), "hipMalloc");
+  if (!rc) rc = b->hip(hipMalloc((void **)&b->d_sparse, n * sizeof(SparseArgs)), "hipMalloc");
+  if (!rc) rc = b->hip(hipMalloc((void **)&b->d_ctrl, 2 * n * sizeof(unsigned)), "hipMalloc");
+  if (!rc) rc = b->hip(hipHostMalloc((void **)&b->h_ctrl, 2 * n * sizeof(unsigned), hipHostMallocDefault), "hipHostMalloc");
+  if (!rc) rc = upload_batch_args(b);   // finalizes the members: their device bytes are complete
+  if (!rc) rc = regroup(b);
+  if (rc) {
+    vse_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return VSE_OK;
+}
+
+int vse_batch_run(vse_batch *b, int n_steps, int check_every, float *elapsed_ms) {
+  if (!b || n_steps < 0) return VSE_ERR_ARG;
+  int rc = upload_batch_args(b);
+  if (rc) return rc;
+  double total = 0.0;
+  for (size_t k = 0; k < b->groups.size() && !rc && n_steps > 0; k++) {
+    BatchGroup &g = b->groups[k];
+    float ms = 0.f;
+    if (g.n == 1) {
+      vse_sim *s = b->sims[g.first];
+      rc = vse_run(s, n_steps, check_every, elapsed_ms ? &ms : nullptr);
+      if (rc) {
+        b->err = "member " + std::to_string(g.first) + ": " + s->err;
+        if (rc == VSE_ERR_UNSTABLE)
+          b->err = "Simulacao tornando-se instavel [member " + std::to_string(g.first) + ": " + s->err + "]";
+      }
+    } else {
+      rc = run_group(b, g, n_steps, check_every, elapsed_ms ? &ms : nullptr);
+    }
+    total += ms;
+  }
+  if (elapsed_ms) *elapsed_ms = rc ? 0.f : (float)total;
+  return rc;
+}
+
+int vse_batch_profile(vse_batch *b, int n_steps, float *ms) {
+  if (!b || n_steps < 1 || !ms) return VSE_ERR_ARG;
+  int rc = upload_batch_args(b);
+  if (rc) return rc;
+  const BatchGroup &g = b->all;
+  hipEvent_t ev[4] = {};
+  for (int i = 0; i < 4 && !rc; i++) rc = b->hip(hipEventCreate(&ev[i]), "event");
+  double acc[3] = {0, 0, 0};
+  for (int it = 0; it < n_steps && !rc; it++) {
+    hipError_t e = hipEventRecord(ev[0], b->stream);
+    if (e == hipSuccess) e = vse_launch_batch2d(true, b->variant, b->visco, b->ord, g.B, g.total_tiles, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], b->stream);
+    if (e == hipSuccess) e = vse_launch_batch2d(false, b->variant, b->visco, b->ord, g.B, g.total_tiles, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[2], b->stream);
+    if (e == hipSuccess) e = vse_launch_sparse_batch(g.B, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[3], b->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
+    rc = b->hip(e, "profile step");
+    for (int k = 0; k < 3 && !rc; k++) {
+      float t = 0.f;
+      rc = b->hip(hipEventElapsedTime(&t, ev[k], ev[k + 1]), "elapsed");
+      acc[k] += t;
+    }
+  }
+  for (int i = 0; i < 4; i++)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
+  if (!rc)
+    for (int k = 0; k < 3; k++) ms[k] = (float)(acc[k] / n_steps);
+  if (!rc) rc = check_group(b, g);
+  return rc;
+}
+
+int vse_batch_set_rows(vse_batch *b, int rows) {
+  if (!b || rows < 1 || rows > 64) return VSE_ERR_ARG;
+  b->rows = rows;
+  return regroup(b);
+}
+
+int vse_batch_set_group_bytes(vse_batch *b, size_t bytes) {
+  if (!b) return VSE_ERR_ARG;
+  b->group_bytes = bytes;
+  return regroup(b);
+}
+
+int vse_batch_groups(vse_batch *b, int *first, int max_groups) {
+  if (!b) return 0;
+  const int ng = (int)b->groups.size();
+  if (first)
+    for (int k = 0; k < ng && k < max_groups; k++) first[k] = b->groups[k].first;
+  return ng;
+}
+
+int vse_batch_size(vse_batch *b) { return b ? (int)b->sims.size() : 0; }
+
+const char *vse_batch_last_error(vse_batch *b) { return b ? b->err.c_str() : "null handle"; }
 
 }  // extern "C"

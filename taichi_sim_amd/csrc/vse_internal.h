@@ -11,6 +11,18 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Device code sees the argument blocks' pointers as global-memory pointers
+// (address space 1): a block read from device memory (batched launches) then
+// still compiles to global_load/global_store instead of flat accesses.  Host
+// code sees plain pointers; the layout is the same.  Files without device
+// code (the C-ABI layer) define VSE_HOST_SIDE to see plain pointers in both
+// compilation passes.
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(VSE_HOST_SIDE)
+#define VSE_G __attribute__((address_space(1)))
+#else
+#define VSE_G
+#endif
+
 #define VSE_MAX_ORD 6
 #define VSE_MAX_SLS 8
 #define VSE_VAR_WGSL 0
@@ -34,8 +46,8 @@ static const int kPsiSlots2D[] = {P_DVX_DX, P_DVY_DY, P_DVY_DX, P_DVX_DY,
 // width nb = lo + (np - hi).  Outside the band a = 0 and k = 1 exactly, so
 // psi stays 0 and the stretched derivative equals the plain one bit for bit.
 struct AxisProf {
-  const float *a, *b, *k, *ah, *bh, *kh;
-  const float *ki, *khi;             // 1/k, 1/k_half (fast arithmetic)
+  const VSE_G float *a, *b, *k, *ah, *bh, *kh;
+  const VSE_G float *ki, *khi;       // 1/k, 1/k_half (fast arithmetic)
   int lo, hi, np, nb;
 };
 
@@ -53,18 +65,18 @@ struct StepArgs {
   float C[VSE_MAX_ORD];
   float Cd[3][VSE_MAX_ORD];          // f32(C[c] / d_axis) (fast arithmetic)
   AxisProf ax[3];
-  float *vx, *vy, *vz;
-  float *sxx, *syy, *szz, *sxy, *sxz, *syz;
-  float *psi[P_COUNT];
-  float *r[6];                       // r_xx, r_yy, r_zz, r_xy, r_xz, r_yz (3D); 2D: xx, yy, -, xy
-  const float *rho, *cp, *cs;
+  VSE_G float *vx, *vy, *vz;
+  VSE_G float *sxx, *syy, *szz, *sxy, *sxz, *syz;
+  VSE_G float *psi[P_COUNT];
+  VSE_G float *r[6];                 // r_xx, r_yy, r_zz, r_xy, r_xz, r_yz (3D); 2D: xx, yy, -, xy
+  const VSE_G float *rho, *cp, *cs;
   ZenerK zk;
   int L;
   unsigned *v2max;                   // running max of |v|^2 as float bits
-  const int *step;                   // device step counter (0-based)
+  const VSE_G int *step;             // device step counter (0-based)
   // Taichi-variant source (inline in the velocity update)
-  const int *src_map;                // full grid, -1 or column
-  const float *src;                  // (n_rows, src_cols)
+  const VSE_G int *src_map;          // full grid, -1 or column
+  const VSE_G float *src;            // (n_rows, src_cols)
   int src_cols, src_rows;
 };
 
@@ -107,9 +119,25 @@ struct KTune {
 // reference (checked against the oracle at 1e-5 rel-L2), far fewer VALU ops.
 enum { VSE_MATH_STRICT = 0, VSE_MATH_FAST = 1 };
 
+// Batched 2D step: per-instance argument blocks in device memory and the
+// prefix of per-instance workgroup (tile) counts.
+#define VSE_MAX_BATCH 256
+struct BatchArgs {
+  const StepArgs *__restrict__ args;        // device array [n]
+  const SparseArgs *__restrict__ sparse;    // device array [n]
+  int n;
+  int rows;                    // BX-row groups each workgroup walks in x
+  int tile_beg[VSE_MAX_BATCH + 1];
+};
+
 // launchers (vse_k2d.hip / vse_k3d.hip)
 hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st);
 hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st);
 hipError_t vse_launch_stress3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
 hipError_t vse_launch_velocity3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
 hipError_t vse_launch_sparse(const SparseArgs &S, hipStream_t st);
+hipError_t vse_launch_batch2d(bool stress, int variant, int visco, int ord, const BatchArgs &B, int total_tiles,
+                              hipStream_t st);
+hipError_t vse_launch_sparse_batch(const BatchArgs &B, hipStream_t st);
+hipError_t vse_launch_batch_ctrl(const BatchArgs &B, unsigned *out, hipStream_t st);
+long vse_tiles2d(int nx, int ny, int rows);

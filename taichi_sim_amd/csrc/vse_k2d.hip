@@ -13,9 +13,7 @@ namespace {
 constexpr int BY = 64, BX = 4;
 
 template <int VAR, bool VISCO, int ORD>
-__global__ __launch_bounds__(BX * BY) void k_stress2d(StepArgs A) {
-  const int y = blockIdx.x * BY + threadIdx.x;
-  const int x = blockIdx.y * BX + threadIdx.y;
+__device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
   const int nx = A.nx, ny = A.ny;
   if (x >= nx || y >= ny) return;
   constexpr int last = ORD - 1, off = ORD - 1;
@@ -104,9 +102,7 @@ __global__ __launch_bounds__(BX * BY) void k_stress2d(StepArgs A) {
 }
 
 template <int VAR, bool VISCO, int ORD>
-__global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A) {
-  const int y = blockIdx.x * BY + threadIdx.x;
-  const int x = blockIdx.y * BX + threadIdx.y;
+__device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, int y) {
   const int nx = A.nx, ny = A.ny;
   const bool valid = x < nx && y < ny;
   constexpr int last = ORD - 1, off = ORD - 1;
@@ -171,7 +167,60 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A) {
     A.vy[id] = vyn;
     v2bits = __float_as_uint(vxn * vxn + vyn * vyn);
   }
-  block_max_u32(v2bits, A.v2max);
+  return v2bits;
+}
+
+template <int VAR, bool VISCO, int ORD>
+__global__ __launch_bounds__(BX * BY) void k_stress2d(StepArgs A) {
+  stress2d_cell<VAR, VISCO, ORD>(A, blockIdx.y * BX + threadIdx.y, blockIdx.x * BY + threadIdx.x);
+}
+
+template <int VAR, bool VISCO, int ORD>
+__global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A) {
+  const unsigned v2 = velocity2d_cell<VAR, VISCO, ORD>(A, blockIdx.y * BX + threadIdx.y, blockIdx.x * BY + threadIdx.x);
+  block_max_u32(v2, A.v2max);
+}
+
+// ---- batched: one launch over the tiles of many simulations --------------
+// A batch tile is BY (y) x BX*B.rows (x) cells: the workgroup walks its
+// B.rows row-groups of BX rows one after another, so the instance lookup and
+// the argument loads at its start are paid once per B.rows row-groups.
+// Workgroup b belongs to instance i with B.tile_beg[i] <= b < B.tile_beg[i+1];
+// its tile inside that instance is b - tile_beg[i], y-tiles fastest.
+__device__ __forceinline__ int batch_instance(const BatchArgs &B, int b) {
+  int lo = 0, hi = B.n;          // tile_beg is increasing; find the last i with tile_beg[i] <= b
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (B.tile_beg[mid] <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <int VAR, bool VISCO, int ORD>
+__global__ __launch_bounds__(BX * BY) void k_stress2d_batch(BatchArgs B) {
+  const int i = batch_instance(B, blockIdx.x);
+  const StepArgs &A = B.args[i];
+  const int t = blockIdx.x - B.tile_beg[i];
+  const int nyb = (A.ny + BY - 1) / BY;
+  const int y = (t % nyb) * BY + threadIdx.x;
+  const int x0 = (t / nyb) * BX * B.rows + threadIdx.y;
+  for (int k = 0; k < B.rows; k++) stress2d_cell<VAR, VISCO, ORD>(A, x0 + k * BX, y);
+}
+
+template <int VAR, bool VISCO, int ORD>
+__global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B) {
+  const int i = batch_instance(B, blockIdx.x);
+  const StepArgs &A = B.args[i];
+  const int t = blockIdx.x - B.tile_beg[i];
+  const int nyb = (A.ny + BY - 1) / BY;
+  const int y = (t % nyb) * BY + threadIdx.x;
+  const int x0 = (t / nyb) * BX * B.rows + threadIdx.y;
+  unsigned v2 = 0;
+  for (int k = 0; k < B.rows; k++) {
+    const unsigned u = velocity2d_cell<VAR, VISCO, ORD>(A, x0 + k * BX, y);
+    v2 = v2 > u ? v2 : u;
+  }
+  block_max_u32(v2, A.v2max);
 }
 
 template <int VAR, bool VISCO>
@@ -191,6 +240,22 @@ hipError_t launch_ord(bool stress, int ord, const StepArgs &A, hipStream_t st) {
   return hipGetLastError();
 }
 
+template <int VAR, bool VISCO>
+hipError_t launch_batch_ord(bool stress, int ord, const BatchArgs &B, int total_tiles, hipStream_t st) {
+  dim3 block(BY, BX), grid(total_tiles);
+#define VSE_CASE(O)                                                                              \
+  case O:                                                                                        \
+    if (stress) hipLaunchKernelGGL((k_stress2d_batch<VAR, VISCO, O>), grid, block, 0, st, B);    \
+    else hipLaunchKernelGGL((k_velocity2d_batch<VAR, VISCO, O>), grid, block, 0, st, B);         \
+    break;
+  switch (ord) {
+    VSE_CASE(2) VSE_CASE(3) VSE_CASE(4) VSE_CASE(5) VSE_CASE(6)
+    default: return hipErrorInvalidValue;
+  }
+#undef VSE_CASE
+  return hipGetLastError();
+}
+
 hipError_t dispatch(bool stress, int variant, int visco, int ord, const StepArgs &A, hipStream_t st) {
   if (variant == VSE_VAR_TAICHI) return launch_ord<VSE_VAR_TAICHI, false>(stress, ord, A, st);
   if (visco) return launch_ord<VSE_VAR_WGSL, true>(stress, ord, A, st);
@@ -198,6 +263,17 @@ hipError_t dispatch(bool stress, int variant, int visco, int ord, const StepArgs
 }
 
 }  // namespace
+
+hipError_t vse_launch_batch2d(bool stress, int variant, int visco, int ord, const BatchArgs &B, int total_tiles,
+                              hipStream_t st) {
+  if (variant == VSE_VAR_TAICHI) return launch_batch_ord<VSE_VAR_TAICHI, false>(stress, ord, B, total_tiles, st);
+  if (visco) return launch_batch_ord<VSE_VAR_WGSL, true>(stress, ord, B, total_tiles, st);
+  return launch_batch_ord<VSE_VAR_WGSL, false>(stress, ord, B, total_tiles, st);
+}
+
+long vse_tiles2d(int nx, int ny, int rows) {
+  return (long)((ny + BY - 1) / BY) * (((nx + BX - 1) / BX + rows - 1) / rows);
+}
 
 hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st) {
   return dispatch(true, variant, visco, ord, A, st);

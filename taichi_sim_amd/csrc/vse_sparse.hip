@@ -14,7 +14,7 @@
 
 namespace {
 
-__global__ __launch_bounds__(1024) void k_sparse(SparseArgs S) {
+__device__ __forceinline__ void sparse_body(const SparseArgs &S) {
   __shared__ int s_shared;
   if (threadIdx.x == 0) s_shared = *S.step;
   __syncthreads();
@@ -65,7 +65,30 @@ __global__ __launch_bounds__(1024) void k_sparse(SparseArgs S) {
   if (threadIdx.x == 0) *S.step = s + 1;
 }
 
+__global__ __launch_bounds__(1024) void k_sparse(SparseArgs S) { sparse_body(S); }
+
+// one block per simulation of a batch
+__global__ __launch_bounds__(1024) void k_sparse_batch(BatchArgs B) { sparse_body(B.sparse[blockIdx.x]); }
+
+// (v2max bits, step) of every simulation of a batch into out[2i], out[2i+1]
+__global__ __launch_bounds__(256) void k_batch_ctrl(BatchArgs B, unsigned *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B.n) return;
+  out[2 * i] = *B.sparse[i].v2max;
+  out[2 * i + 1] = (unsigned)*B.sparse[i].step;
+}
+
 }  // namespace
+
+hipError_t vse_launch_batch_ctrl(const BatchArgs &B, unsigned *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_ctrl, dim3((B.n + 255) / 256), dim3(256), 0, st, B, out);
+  return hipGetLastError();
+}
+
+hipError_t vse_launch_sparse_batch(const BatchArgs &B, hipStream_t st) {
+  hipLaunchKernelGGL(k_sparse_batch, dim3(B.n), dim3(1024), 0, st, B);
+  return hipGetLastError();
+}
 
 hipError_t vse_launch_sparse(const SparseArgs &S, hipStream_t st) {
   hipLaunchKernelGGL(k_sparse, dim3(1), dim3(1024), 0, st, S);

@@ -295,3 +295,76 @@ class VSESim:
             keep = cols < src.shape[1]          # columns >= n_src read 0 in the reference
             cells, cols = cells[keep], cols[keep]
         self.set_sources(cells, cols, src)
+
+
+class VSEBatch:
+    """Several 2D ``VSESim`` advanced together, one launch per pass per step
+    over all of them (shots of one specimen, or specimens of different
+    sizes).  Members keep their own state and results; each member ends
+    exactly where ``member.run(n)`` alone would.  The batch does not own its
+    members: close it before them."""
+
+    def __init__(self, sims):
+        self._lib = _lib.load()
+        self.sims = list(sims)
+        if not 1 <= len(self.sims) <= _lib.BATCH_MAX:
+            raise ValueError(f"a batch holds 1..{_lib.BATCH_MAX} simulations, got {len(self.sims)}")
+        arr = (ctypes.c_void_p * len(self.sims))(*[s._h.value for s in self.sims])
+        h = ctypes.c_void_p()
+        rc = self._lib.vse_batch_create(arr, len(self.sims), ctypes.byref(h))
+        if rc != 0 or not h.value:
+            raise _lib.VSEError(rc, "vse_batch_create failed: members must be open 2D simulations on one device "
+                                    "with the same variant, viscoelastic flag and FD order")
+        self._h = h
+
+    def run(self, n_steps, check_every=100, timed=False):
+        ms = ctypes.c_float(0.0)
+        rc = self._lib.vse_batch_run(self._h, int(n_steps), int(check_every), ctypes.byref(ms) if timed else None)
+        if rc != 0:
+            msg = self._lib.vse_batch_last_error(self._h).decode(errors="replace")
+            if rc == _lib.VSE_ERR_UNSTABLE:
+                raise StabilityError("Simulacao tornando-se instavel", msg)
+            raise _lib.VSEError(rc, msg)
+        return ms.value if timed else None
+
+    def set_rows(self, rows):
+        """x row-groups (of 4 rows) each workgroup walks; speed only."""
+        rc = self._lib.vse_batch_set_rows(self._h, int(rows))
+        if rc != 0:
+            raise _lib.VSEError(rc, self._lib.vse_batch_last_error(self._h).decode(errors="replace"))
+
+    def set_group_bytes(self, nbytes):
+        """Cache budget per group of members (0: all members in lockstep); speed only."""
+        rc = self._lib.vse_batch_set_group_bytes(self._h, int(nbytes))
+        if rc != 0:
+            raise _lib.VSEError(rc, self._lib.vse_batch_last_error(self._h).decode(errors="replace"))
+
+    def groups(self):
+        """First member index of each group."""
+        n = self._lib.vse_batch_groups(self._h, None, 0)
+        out = (ctypes.c_int * max(n, 1))()
+        self._lib.vse_batch_groups(self._h, out, n)
+        return list(out[:n])
+
+    def profile(self, n_steps):
+        """Average (stress, velocity, sparse) launch durations in ms of one step of
+        all members in one launch per pass."""
+        ms = (ctypes.c_float * 3)()
+        rc = self._lib.vse_batch_profile(self._h, int(n_steps), ms)
+        if rc != 0:
+            raise _lib.VSEError(rc, self._lib.vse_batch_last_error(self._h).decode(errors="replace"))
+        return tuple(float(v) for v in ms)
+
+    def __len__(self):
+        return len(self.sims)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.vse_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -274,44 +274,29 @@ class Simulator:
     def _trace_key(self, res):
         return res["sens_stress"] if "sens_stress" in res else res["sens_pressure"]
 
-    def run(self):
-        """Iterations x emission laws -> implementation(); timing, energy, saves."""
+    def run(self, laws_per_call=1):
+        """Iterations x emission laws -> implementation(); timing, energy, saves.
+
+        ``laws_per_call`` > 1 hands that many laws at a time to
+        ``implementation_many()`` (a backend may simulate them together); the
+        results, their order and the saved files are the same as with 1."""
         now = datetime.now()
         sim_times, total_times, results = [], [], []
         for n in range(self._n_iter):
             print(f'Iteracao {n}')
             n_laws = self._emission_laws.shape[0] if hasattr(self, "_emission_laws") else 1
-            for law in range(n_laws):
-                print(f'\tLaw {law} of {n_laws}')
-                if hasattr(self, "_emission_laws"):
-                    for p in self._probes:
-                        p.set_t0(self._emission_laws[law])
-                    self.rebuild_sources()
+            step = max(1, int(laws_per_call))
+            for first in range(0, n_laws, step):
+                laws = list(range(first, min(n_laws, first + step)))
                 t0 = time()
-                res = self.implementation()
-                sim_total = time() - t0
-                sim_times.append(res["sim_time"])
-                total_times.append(sim_total)
-                print(res["gpu_str"])
-                print(f'{sim_times[-1]:.3}s')
-                print(f'Tempo total (inclui transferencia de dados): {total_times[-1]:.3}s')
-                result_dir = getattr(self, "_results_dir", ".")
-                name = os.path.join(result_dir, f'result_{self._name}_{now.strftime("%Y%m%d-%H%M%S")}_'
-                                                f'{self._nx}x{self._ny}_{self._n_steps}_iter_{n}_law_{law}')
-                traces = self._trace_key(res)
-                field = self._field_key(res)
-                row = max(0, min(traces.shape[0] - 1, self._n_steps - self.cpml_time))
-                reflected = traces[row, :]
-                self._reflected_energy = (reflected ** 2).sum() * self._dt
-                print(f"Energia refletida (sensor): {self._reflected_energy:.2}")
-                self._field_energy = (field ** 2).sum() * self._dx * self._dy
-                print(f"Energia ultimo frame: {self._field_energy:.2}")
-                tag = "pressure" if self._sim_type == "acoustic" else "stress"
-                if self._save_results and self._save_field:
-                    np.save(name + ('_field_pressure' if tag == "pressure" else '_field_stress_yy'), field)
-                if self._save_bscan:
-                    np.save(name + f'_bscan_{tag}', traces)
-                results.append(res)
+                chunk = self.implementation_many(laws)
+                sim_total = (time() - t0) / len(laws)
+                for law, res in zip(laws, chunk):
+                    print(f'\tLaw {law} of {n_laws}')
+                    sim_times.append(res["sim_time"])
+                    total_times.append(sim_total)
+                    self._report(res, n, law, now, sim_times[-1], total_times[-1])
+                    results.append(res)
         self._sim_times = np.array(sim_times)
         self._sim_total_times = np.array(total_times)
         print(f'TEMPO - {self._n_steps} pontos de tempo')
@@ -319,6 +304,44 @@ class Simulator:
             np.save(os.path.join(getattr(self, "_results_dir", "."),
                                  f'sources_{self._name}_{now.strftime("%Y%m%d-%H%M%S")}'), self._source_term)
         return results
+
+    def select_law(self, law):
+        """Apply emission law ``law`` to the probes and rebuild the source term
+        (simulator.py:331-333, plus the rebuild the reference leaves out)."""
+        if hasattr(self, "_emission_laws"):
+            for p in self._probes:
+                p.set_t0(self._emission_laws[law])
+            self.rebuild_sources()
+
+    def implementation_many(self, laws):
+        """Results of ``laws``, in order: one ``implementation()`` per law."""
+        out = []
+        for law in laws:
+            self.select_law(law)
+            out.append(self.implementation())
+        return out
+
+    def _report(self, res, n, law, now, sim_time, total_time):
+        """Per-law console lines, energies and np.save outputs (simulator.py:338-605)."""
+        print(res["gpu_str"])
+        print(f'{sim_time:.3}s')
+        print(f'Tempo total (inclui transferencia de dados): {total_time:.3}s')
+        result_dir = getattr(self, "_results_dir", ".")
+        name = os.path.join(result_dir, f'result_{self._name}_{now.strftime("%Y%m%d-%H%M%S")}_'
+                                        f'{self._nx}x{self._ny}_{self._n_steps}_iter_{n}_law_{law}')
+        traces = self._trace_key(res)
+        field = self._field_key(res)
+        row = max(0, min(traces.shape[0] - 1, self._n_steps - self.cpml_time))
+        reflected = traces[row, :]
+        self._reflected_energy = (reflected ** 2).sum() * self._dt
+        print(f"Energia refletida (sensor): {self._reflected_energy:.2}")
+        self._field_energy = (field ** 2).sum() * self._dx * self._dy
+        print(f"Energia ultimo frame: {self._field_energy:.2}")
+        tag = "pressure" if self._sim_type == "acoustic" else "stress"
+        if self._save_results and self._save_field:
+            np.save(name + ('_field_pressure' if tag == "pressure" else '_field_stress_yy'), field)
+        if self._save_bscan:
+            np.save(name + f'_bscan_{tag}', traces)
 
     def implementation(self):
         """Backend hook: returns {"stress"/"pressure", "sens_stress"/"sens_pressure",

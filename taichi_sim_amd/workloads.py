@@ -9,6 +9,11 @@ dt = 0.5 dx / (cp_max sqrt(ndim)).  Random media are seeded.
   centre, a line of receivers (the CPU-runnable case).
 * ``visco_2d_array``: 2D Zener (L=3, constant Q fitted over 2-8 MHz) steel
   plate with random circular voids and a focusing linear array.
+* ``focal_laws`` / ``multi_shot_2d``: emission laws (one focal point per
+  shot) for ``visco_2d_array`` specimens of several sizes -- the multi-shot
+  workload of a full focal-law scan.
+* ``fmc_2d``: full-matrix capture (law k fires element k only) on
+  ``visco_2d_array`` specimens of several sizes.
 * ``visco_3d_block``: 3D Zener two-layer block (aluminium over steel) with
   random ellipsoidal density inclusions, a circular piston on the top face and
   a square grid of surface receivers.
@@ -106,6 +111,54 @@ def visco_2d_array(n=2048, steps=4000, n_elem=64, n_voids=16, seed=2, q=50.0, em
             "simul_configs": _configs(True),
             "specimen_params": {"cp": cp, "cs": cs, "rho": rho, "cp_map": cp_m, "cs_map": cs_m, "rho_map": rho_m},
             "attenuation": {"q_p": q, "q_s": q, "f_min": 2.0, "f_max": 8.0, "n_sls": 3}}
+
+
+def focal_laws(n_elem, n_shots, width, height, cp, pitch=0.5, seed=7):
+    """(n_shots, n_elem) emission delays in microseconds: shot i focuses the
+    array (centred on the top face, element pitch ``pitch`` mm) at a seeded
+    point in the middle of the specimen (the retE column of a .law file)."""
+    rng = np.random.default_rng(seed)
+    ex = width / 2 + (np.arange(n_elem) - (n_elem - 1) / 2) * pitch
+    laws = np.empty((n_shots, n_elem))
+    for i in range(n_shots):
+        fx = rng.uniform(0.25 * width, 0.75 * width)
+        fz = rng.uniform(0.3 * height, 0.7 * height)
+        tof = np.sqrt((ex - fx) ** 2 + fz ** 2) / cp
+        laws[i] = tof.max() - tof + 1.0
+    return laws
+
+
+def multi_shot_2d(sizes=(512, 1024, 1536, 2048), n_shots=64, steps=3000, n_elem=32, seed=2):
+    """[(config, laws)] per specimen: a ``visco_2d_array`` steel plate of each
+    size (full grid incl. CPML) and ``n_shots`` focal laws for its array."""
+    out = []
+    for k, n in enumerate(sizes):
+        cfg = visco_2d_array(n=n, steps=steps, n_elem=n_elem, n_voids=max(2, n // 128), seed=seed + k)
+        pr = cfg["probes"][0]["linear"]
+        w = cfg["roi"]["width"]
+        h = cfg["roi"]["height"]
+        pitch = pr["dim_a"] + pr["inter_elem"]
+        out.append((cfg, focal_laws(n_elem, n_shots, w, h, STEEL[0], pitch=pitch, seed=seed + 100 + k)))
+    return out
+
+
+def fmc_2d(sizes=(512, 1024, 1536, 2048), n_elem=16, steps=3000, seeds=(40, 41, 42, 43), n_voids=16,
+           silent_t0=1.0e6):
+    """[(config, laws)] per specimen for a full-matrix capture: a
+    ``visco_2d_array`` steel plate of each size whose ``n_elem`` elements all
+    emit and receive, and ``n_elem`` emission laws where law k fires element k
+    at t0 = 1 us and delays every other element by ``silent_t0`` us, past any
+    simulated time, so its pulse is exactly zero.  (A per-shot emitter mask
+    would instead leave a single emitting element, which the WGSL source
+    indexing reads as column 0 only: shots k > 0 would be silent.)"""
+    out = []
+    for n, seed in zip(sizes, seeds):
+        cfg = visco_2d_array(n=n, steps=steps, n_elem=n_elem, n_voids=n_voids, seed=seed,
+                             t0_emission=[1.0] * n_elem)
+        laws = np.full((n_elem, n_elem), float(silent_t0))
+        np.fill_diagonal(laws, 1.0)
+        out.append((cfg, laws))
+    return out
 
 
 def visco_3d_block(n=512, steps=2000, n_incl=24, seed=3, q=50.0, n_rec_side=32, rec_pitch=1.0):

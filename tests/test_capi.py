@@ -35,3 +35,14 @@ def test_abi_version_and_arg_validation_without_gpu():
     assert lib.vse_create(ctypes.byref(d), ctypes.byref(h)) == _lib.VSE_ERR_ARG
     assert not h.value
     assert lib.vse_last_error(None) == b"null handle"
+    # batches: member validation precedes any device call
+    bh = ctypes.c_void_p()
+    assert lib.vse_batch_create(None, 1, ctypes.byref(bh)) == _lib.VSE_ERR_ARG
+    arr = (ctypes.c_void_p * 1)(None)
+    assert lib.vse_batch_create(arr, 1, ctypes.byref(bh)) == _lib.VSE_ERR_ARG
+    assert lib.vse_batch_create(arr, 0, ctypes.byref(bh)) == _lib.VSE_ERR_ARG
+    assert lib.vse_batch_create(arr, _lib.BATCH_MAX + 1, ctypes.byref(bh)) == _lib.VSE_ERR_ARG
+    assert not bh.value
+    assert lib.vse_batch_size(None) == 0
+    assert lib.vse_batch_last_error(None) == b"null handle"
+    assert lib.vse_batch_run(None, 1, 1, None) == _lib.VSE_ERR_ARG

@@ -14,6 +14,12 @@ as one batch (one launch per pass per step), against the same 64 simulations
 run one after another as the reference runs its emission laws.
 value = 3000-step shots per second.
 
+``--workload adjoint`` measures the gradient: 2D viscoelastic 1024^2, L=3,
+3000 steps, 8 FMC shots, observed traces from a true model with 8 seeded
+Gaussian velocity anomalies (+-5%, 4-30 mm under the array), gradient at the homogeneous starting
+model with sqrt(N) checkpointing.  value = time(checkpointed forward +
+recompute + adjoint) / time(plain forward), per shot, device time.
+
 Prints one JSON line.  value = Gcell-updates/s of the whole job (one GPU).
 roofline: the dominant kernel (stress pass) -- algorithmic bytes per launch /
 its average duration measured here with HIP events on the library's stream.
@@ -50,7 +56,9 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU-oracle sample (0: skip)")
-    ap.add_argument("--workload", choices=("3d", "shots"), default="3d")
+    ap.add_argument("--workload", choices=("3d", "shots", "adjoint"), default="3d")
+    ap.add_argument("--fd-dirs", type=int, default=2, help="finite-difference directions (--workload adjoint)")
+    ap.add_argument("--adj-shots", type=int, default=8, help="shots (--workload adjoint)")
     ap.add_argument("--shots", type=int, default=16, help="array elements = FMC shots per specimen (--workload shots)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 5 launches): rocprof counter calibration")
@@ -70,6 +78,8 @@ def main():
         sys.exit(0 if rc == 0 else 1)
     if args.workload == "shots":
         return bench_shots(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
     from taichi_sim_amd import workloads, autotune
     from taichi_sim_amd.backend import attenuation_from_config
     from taichi_sim_amd.engine import VSESim
@@ -241,6 +251,122 @@ def bench_shots(args):
         "kernel_ms_all_in_one_launch": {k: round(v, 4) for k, v in zip(("stress", "velocity", "sparse"), k_batch)},
         "kernel_ms_loop_sum": {k: round(v, 4) for k, v in zip(("stress", "velocity", "sparse"), k_solo)},
         "device": name,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_adjoint(args):
+    """Adjoint/forward time ratio of the C-ABI gradient (see module doc)."""
+    from taichi_sim_amd import workloads
+    from taichi_sim_amd.adjoint import VSEGrad, chain_to_cells
+    from taichi_sim_amd.backend import attenuation_from_config
+    from taichi_sim_amd.engine import VSESim
+    from taichi_sim_amd.sim_support.simulator import Simulator
+
+    n, steps, n_shots = 1024, 3000, args.adj_shots
+    cfg, laws, true = workloads.adjoint_2d(n=n, steps=steps, n_shots=n_shots)
+    sim = Simulator(cfg)
+    sim._emission_laws = laws
+    att = attenuation_from_config(cfg)
+    g = VSESim.from_simulator(sim, variant="wgsl", visco=True, att=att)
+    start = [np.asarray(a, np.float32).copy() for a in (sim._rho_grid_vx, sim._cp_grid_vx, sim._cs_grid_vx)]
+    eng = VSEGrad(g)
+    print(f"# setup: grid {g.shape}, segment {eng.seg_len} steps, {eng.device_bytes() / 2**30:.1f} GiB "
+          f"of checkpoints/adjoint state", file=sys.stderr, flush=True)
+    ncell = int(np.prod(g.shape))
+
+    # observed data: the true model through the product forward (vse_run)
+    obs = []
+    g.set_material(*true)
+    for k in range(n_shots):
+        sim.select_law(k)
+        g.load_sources(sim)
+        g.reset()
+        g.run(steps, check_every=500)
+        obs.append(g.traces().copy())
+
+    t_fwd, t_gfwd, t_bwd, J = [], [], [], 0.0
+    grad = None
+    for k in range(n_shots):
+        sim.select_law(k)
+        g.load_sources(sim)
+        # the plain forward of the same shot at the starting model
+        g.set_material(*start)
+        g.reset()
+        t_fwd.append(g.run(steps, check_every=500, timed=True))
+        # checkpointed forward + FP64 misfit + recompute/adjoint
+        eng.set_model(*start)
+        eng.forward(steps, check_every=500)
+        t_gfwd.append(eng.ms["forward"])
+        J += eng.misfit(obs[k])
+        coef = eng.backward()
+        t_bwd.append(eng.ms["backward"])
+        grad = coef if grad is None else {key: grad[key] + coef[key] for key in grad}
+        print(f"# shot {k}: forward {t_fwd[-1]:.1f} ms, checkpointed forward {t_gfwd[-1]:.1f} ms, "
+              f"recompute+adjoint {t_bwd[-1]:.1f} ms", file=sys.stderr, flush=True)
+    g_rho, g_cp, g_cs = (t.numpy() for t in chain_to_cells(grad, *start))
+
+    # finite-difference spot check at full size on shot 0.  The misfit of the
+    # inversion is tiny at the starting model (smooth anomalies reflect
+    # weakly), where a central difference measures its cubic term; the check
+    # therefore differentiates J = 0.5*||traces||^2 (zero data), which the
+    # FP32 forward resolves to ~1e-7, along directions under the array
+    # moving cp and cs, with a fourth-order central quotient.
+    zero = np.zeros_like(obs[0])
+    sim.select_law(0)
+    g.load_sources(sim)
+    eng.set_model(*start)
+    eng.forward(steps, check_every=500)
+    energy = eng.misfit(zero)
+    e_coef = eng.backward()
+    e_rho, e_cp, e_cs = (t.numpy() for t in chain_to_cells(e_coef, *start))
+
+    def misfit_all(model):
+        eng.set_model(*model)
+        eng.forward(steps, check_every=500)
+        return eng.misfit(zero)
+
+    fd_rel = []
+    if args.fd_dirs > 0:
+        rng = np.random.default_rng(5)
+        y_top = int(np.min(np.nonzero(np.asarray(sim._pos_sources) != -1)[1]))
+        X, Y = np.meshgrid(np.arange(g.shape[0]), np.arange(g.shape[1]), indexing="ij")
+        for d in range(args.fd_dirs):
+            f = np.zeros(g.shape)
+            for _ in range(4):
+                cx, cy = rng.uniform(0.3, 0.7) * g.shape[0], rng.uniform(y_top + 2, y_top + 25)
+                sg = rng.uniform(3.0, 8.0)
+                f += rng.uniform(-1, 1) * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * sg * sg))
+            f /= np.abs(f).max()
+            h = 2e-2
+            dirs = [np.zeros(g.shape), f * float(start[1].max()), -f * float(start[2].max())]
+            models = {t: [(start[p] + t * h * dirs[p]).astype(np.float32) for p in range(3)] for t in (-2, -1, 1, 2)}
+            Jt = {t: misfit_all(m) for t, m in models.items()}
+            fd = (8.0 * (Jt[1] - Jt[-1]) - (Jt[2] - Jt[-2])) / 12.0
+            step = lambda p, t: models[t][p].astype(np.float64) - start[p].astype(np.float64)
+            ad = sum(float((gp * (8.0 * (step(p, 1) - step(p, -1)) - (step(p, 2) - step(p, -2))) / 12.0).sum())
+                     for p, gp in enumerate((e_rho, e_cp, e_cs)))
+            fd_rel.append(abs(fd - ad) / max(abs(fd), 1e-30))
+            print(f"# FD direction {d}: FD {fd:.6e} adjoint {ad:.6e} rel {fd_rel[-1]:.1e}", file=sys.stderr, flush=True)
+    eng.close()
+    g.close()
+
+    fwd, gf, bw = (float(np.mean(t)) for t in (t_fwd, t_gfwd, t_bwd))
+    ratio = (gf + bw) / fwd
+    line = {
+        "metric": "adjoint/forward time (2D viscoelastic 1024^2 L=3, 3000 steps, sqrt(N) checkpoints)",
+        "value": round(ratio, 3), "unit": "x", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round((gf + bw) / steps, 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (FP64 misfit and gradient accumulators)", "data": "synthetic",
+        "config": {"workload": f"gradient of the L2 trace misfit, {n_shots} FMC shots, true model with 8 Gaussian "
+                               f"velocity anomalies (+-5%), homogeneous start", "grid": [n, n], "n_sls": 3,
+                   "segment": eng.seg_len, "parallelism": "single GPU"},
+        "forward_ms_per_shot": round(fwd, 2), "checkpointed_forward_ms_per_shot": round(gf, 2),
+        "recompute_adjoint_ms_per_shot": round(bw, 2), "misfit": J, "trace_energy_shot0": energy,
+        "gradient_norms": {"rho": float(np.linalg.norm(g_rho)), "cp": float(np.linalg.norm(g_cp)),
+                           "cs": float(np.linalg.norm(g_cs)), "tau": float(np.linalg.norm(grad["tau"]))},
+        "gcell_per_s_forward": round(ncell * steps / fwd / 1e6, 3),
+        "fd_rel_err_trace_energy": [float(f"{e:.2e}") for e in fd_rel],
     }
     print(json.dumps(line), flush=True)
 

@@ -219,6 +219,53 @@ int vse_batch_size(vse_batch *b);
 const char *vse_batch_last_error(vse_batch *b);
 void vse_batch_destroy(vse_batch *b);
 
+/* ---- gradient of the receiver misfit (2D viscoelastic, WGSL variant) ----
+ * No reference counterpart: the reference only runs the forward time loop.
+ * A vse_grad handle differentiates the traces of one 2D viscoelastic
+ * simulation (its material, CPML, sources and receivers as set on `s`) with
+ * respect to the coefficients the kernels use, per cell:
+ *   VSE_GRAD_LAMBDA, VSE_GRAD_MU  lambda and mu at the normal-stress point
+ *                                 (formed from rho, cp, cs at x and x+1,
+ *                                 shader_webgpu_viscoelastic.wgsl:863-868)
+ *   VSE_GRAD_MU_XY                mu at the shear point (y and y+1, :948-951)
+ *   VSE_GRAD_RHO_VX               rho at the vx point (the cell, :1019)
+ *   VSE_GRAD_RHO_VY               the 4-point rho at the vy point (:1035-1036),
+ *                                 including its use by the source term
+ *   VSE_GRAD_TAU                  a per-cell scale of every relaxation time
+ *                                 (1 = the simulation's own tau)
+ * The chain rule to rho, cp, cs (or rho, lambda, mu) is left to the caller
+ * (taichi_sim_amd/adjoint.py does it in FP64 torch).
+ * vse_grad_forward runs the forward with the tau scale (vse_grad_set_tau_scale)
+ * keeping a checkpoint every seg_len steps (0: ceil(sqrt(n_rows))); its traces
+ * equal vse_run's when the scale is 1.  vse_grad_misfit takes the observed
+ * traces, returns 0.5*sum((traces - observed)^2) accumulated in FP64 and
+ * makes the residual the adjoint source; vse_grad_backward (grad_traces NULL)
+ * then runs the reverse sweep -- segment recompute from the checkpoints plus
+ * the hand-written adjoint kernels -- accumulating the gradient in FP64.
+ * grad_traces non-NULL supplies the adjoint source (d misfit / d traces)
+ * directly.  Wavefields are FP32; only the misfit and the gradient
+ * accumulators are FP64. */
+#define VSE_GRAD_LAMBDA 0
+#define VSE_GRAD_MU 1
+#define VSE_GRAD_MU_XY 2
+#define VSE_GRAD_RHO_VX 3
+#define VSE_GRAD_RHO_VY 4
+#define VSE_GRAD_TAU 5
+typedef struct vse_grad vse_grad;
+int vse_grad_create(vse_sim *s, int seg_len, vse_grad **out);
+int vse_grad_set_tau_scale(vse_grad *g, const float *scale);   /* (n cells), NULL = all 1 */
+int vse_grad_forward(vse_grad *g, int n_steps, int check_every, float *traces, float *elapsed_ms);
+int vse_grad_misfit(vse_grad *g, const float *observed, double *misfit);
+int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms);
+/* FP64 gradient map VSE_GRAD_* (n cells, grid layout) of the last backward. */
+int vse_grad_read(vse_grad *g, int which, double *out);
+/* Field 0..4 = vx, vy, sxx, syy, sxy of the last forward's final state. */
+int vse_grad_read_state(vse_grad *g, int field, float *out);
+int vse_grad_segment(vse_grad *g);
+size_t vse_grad_device_bytes(vse_grad *g);
+const char *vse_grad_last_error(vse_grad *g);
+void vse_grad_destroy(vse_grad *g);
+
 #ifdef __cplusplus
 }
 #endif

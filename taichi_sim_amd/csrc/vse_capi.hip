@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -916,5 +917,330 @@ int vse_batch_groups(vse_batch *b, int *first, int max_groups) {
 int vse_batch_size(vse_batch *b) { return b ? (int)b->sims.size() : 0; }
 
 const char *vse_batch_last_error(vse_batch *b) { return b ? b->err.c_str() : "null handle"; }
+
+}  // extern "C"
+
+// ------------------------------------------------------- gradient (2D visco)
+// Checkpointed adjoint of one 2D viscoelastic simulation (WGSL variant).  The
+// forward pass keeps the full state every K steps (K = ceil(sqrt(n_rows)) by
+// default); the reverse sweep recomputes each K-step segment from its
+// checkpoint into K stored levels, then runs the adjoint steps of that
+// segment backwards.  Device memory: (ceil(n_rows/K) + K) levels of
+// (5 + 3L) fields plus the CPML bands, one adjoint level, 8 derivative
+// adjoints and 6 FP64 gradient maps.
+struct vse_grad {
+  vse_sim *s = nullptr;
+  int K = 0, n_ckpt = 0, n_fwd = 0;
+  bool last_seg_stored = false;   // the forward left the last segment's levels in seg[]
+  std::vector<DLevel> ckpt, seg;
+  DLevel bar{};
+  float *abar[8] = {};
+  double *g[VSE_GR_COUNT] = {};
+  float *ts = nullptr, *traces = nullptr, *gbar = nullptr, *obs = nullptr;
+  int *src_map = nullptr;
+  double *misfit = nullptr;
+  unsigned *ctrl = nullptr, *host_ctrl = nullptr;
+  std::vector<void *> allocs;
+  size_t bytes = 0;
+  std::string err;
+
+  int fail(int code, const std::string &m) { err = m; return code; }
+  int hip(hipError_t e, const char *what) {
+    if (e == hipSuccess) return 0;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return VSE_ERR_HIP;
+  }
+  template <typename T>
+  int alloc(T **p, size_t n) {
+    const size_t nb = std::max<size_t>(n, 1) * sizeof(T);
+    void *q = nullptr;
+    int rc = hip(hipMalloc(&q, nb), "hipMalloc");
+    if (rc) return rc;
+    allocs.push_back(q);
+    bytes += nb;
+    *p = (T *)q;
+    return hip(hipMemsetAsync(q, 0, nb, s->stream), "hipMemsetAsync");
+  }
+  long psi_elems(int i) const {   // local slot i of kPsiSlots2D: even = x axis, odd = y axis
+    return (i % 2 == 0) ? (long)s->ax[0].nb * s->d.ny : (long)s->d.nx * s->ax[1].nb;
+  }
+  long level_floats() const {
+    long total = (5 + 3L * s->d.n_sls) * s->ncell;
+    for (int i = 0; i < 8; i++) total += psi_elems(i);
+    return total;
+  }
+  int alloc_level(DLevel &v) {
+    const long nc = s->ncell;
+    const int L = s->d.n_sls;
+    float *base = nullptr;
+    int rc = alloc(&base, (size_t)level_floats());
+    if (rc) return rc;
+    float *p = base;
+    float **f[5] = {&v.vx, &v.vy, &v.sxx, &v.syy, &v.sxy};
+    for (int i = 0; i < 5; i++) { *f[i] = p; p += nc; }
+    for (int c = 0; c < 3; c++) { v.r[c] = p; p += (long)L * nc; }
+    for (int i = 0; i < 8; i++) { v.psi[i] = p; p += psi_elems(i); }
+    return 0;
+  }
+  int zero_level(const DLevel &v) {
+    return hip(hipMemsetAsync(v.vx, 0, (size_t)level_floats() * sizeof(float), s->stream), "zero level");
+  }
+  // level j of segment sg (state sg*K + j)
+  const DLevel &lev(int sg, int j) const { return j == 0 ? ckpt[sg] : seg[j - 1]; }
+};
+
+static void make_diff(vse_grad *g, DiffArgs &A) {
+  vse_sim *s = g->s;
+  const vse_desc &d = s->d;
+  memset(&A, 0, sizeof(A));
+  A.nx = d.nx; A.ny = d.ny; A.ord = d.ord; A.ncell = s->ncell;
+  A.dx = d.dx; A.dy = d.dy; A.dt = d.dt;
+  for (int c = 0; c < d.ord; c++) A.C[c] = d.coefs[c];
+  A.ax[0] = s->ax[0]; A.ax[1] = s->ax[1];
+  A.rho = s->rho; A.cp = s->cp; A.cs = s->cs; A.ts = g->ts;
+  A.L = d.n_sls;
+  A.zk = s->zk;
+  A.bar = g->bar;
+  for (int i = 0; i < 8; i++) A.abar[i] = g->abar[i];
+  for (int i = 0; i < VSE_GR_COUNT; i++) A.g[i] = g->g[i];
+  A.v2max = g->ctrl;
+  A.src_map = g->src_map; A.src = s->src; A.src_cols = s->src_cols; A.src_rows = d.n_rows;
+  A.n_rec = s->n_rec; A.rec_gate = s->rec_gate; A.rec_beg = s->rec_beg; A.rec_cell = s->rec_cell;
+  A.rec_delay = s->rec_delay; A.traces = g->traces; A.gbar = g->gbar;
+}
+
+// One out-of-place forward step n: level `in` -> level `out` (two launches).
+static int grad_step(vse_grad *g, DiffArgs &A, const DLevel &in, const DLevel &out, int n, bool record) {
+  A.in = in; A.out = out;
+  A.n = n; A.record = record ? 1 : 0;
+  hipError_t e = vse_launch_dstress(A, g->s->stream);
+  if (e == hipSuccess) e = vse_launch_dvelocity(A, g->s->stream);
+  return g->hip(e, "forward step");
+}
+
+static int grad_check(vse_grad *g, int n) {
+  int rc = g->hip(hipMemcpyAsync(g->host_ctrl, g->ctrl, sizeof(unsigned), hipMemcpyDeviceToHost, g->s->stream), "ctrl");
+  if (!rc) rc = g->hip(hipStreamSynchronize(g->s->stream), "sync");
+  if (!rc && g->host_ctrl[0] >= 0x7f800000u) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "Simulacao tornando-se instavel [gradient forward, before step %d]", n);
+    return g->fail(VSE_ERR_UNSTABLE, buf);
+  }
+  return rc;
+}
+
+// The per-cell source-column map from the simulation's current source list.
+static int grad_src_map(vse_grad *g) {
+  vse_sim *s = g->s;
+  return g->hip(vse_launch_src_map(g->src_map, s->ncell, s->src_cell, s->src_col, s->n_src_pts, s->stream), "source map");
+}
+
+extern "C" {
+
+void vse_grad_destroy(vse_grad *g) {
+  if (!g) return;
+  if (g->s && g->s->stream) (void)hipStreamSynchronize(g->s->stream);
+  for (void *p : g->allocs) (void)hipFree(p);
+  if (g->host_ctrl) (void)hipHostFree(g->host_ctrl);
+  delete g;
+}
+
+int vse_grad_create(vse_sim *s, int seg_len, vse_grad **out) {
+  if (!s || !out || seg_len < 0 || seg_len == 1) return VSE_ERR_ARG;
+  *out = nullptr;
+  if (s->d.ndim != 2 || s->d.variant != VSE_VARIANT_WGSL || !s->d.visco)
+    return s->fail(VSE_ERR_ARG, "gradient: 2D viscoelastic WGSL-variant simulations only");
+  int rc = finalize(s);
+  if (rc) return rc;
+  vse_grad *g = new vse_grad();
+  g->s = s;
+  const int N = s->d.n_rows;
+  g->K = seg_len ? seg_len : std::max(2, (int)std::ceil(std::sqrt((double)N)));
+  g->n_ckpt = (N + g->K - 1) / g->K;
+  g->ckpt.resize(g->n_ckpt);
+  g->seg.resize(g->K);
+  for (auto &v : g->ckpt)
+    if (!rc) rc = g->alloc_level(v);
+  for (auto &v : g->seg)
+    if (!rc) rc = g->alloc_level(v);
+  if (!rc) rc = g->alloc_level(g->bar);
+  for (int i = 0; i < 8 && !rc; i++) rc = g->alloc(&g->abar[i], s->ncell);
+  for (int i = 0; i < VSE_GR_COUNT && !rc; i++) rc = g->alloc(&g->g[i], s->ncell);
+  if (!rc) rc = g->alloc(&g->ts, s->ncell);
+  if (!rc) rc = g->alloc(&g->src_map, s->ncell);
+  const size_t nt = (size_t)N * std::max(s->n_rec, 1);
+  if (!rc) rc = g->alloc(&g->traces, nt);
+  if (!rc) rc = g->alloc(&g->gbar, nt);
+  if (!rc) rc = g->alloc(&g->obs, nt);
+  if (!rc) rc = g->alloc(&g->misfit, 1);
+  if (!rc) rc = g->alloc(&g->ctrl, 1);
+  if (!rc) rc = g->hip(hipHostMalloc((void **)&g->host_ctrl, sizeof(unsigned), hipHostMallocDefault), "hipHostMalloc");
+  if (!rc) {
+    std::vector<float> ones(s->ncell, 1.0f);
+    rc = g->hip(hipMemcpyAsync(g->ts, ones.data(), ones.size() * sizeof(float), hipMemcpyHostToDevice, s->stream), "tau scale");
+    if (!rc) rc = g->hip(hipStreamSynchronize(s->stream), "create");
+  }
+  if (rc) {
+    s->err = g->err;
+    vse_grad_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return VSE_OK;
+}
+
+int vse_grad_set_tau_scale(vse_grad *g, const float *scale) {
+  if (!g) return VSE_ERR_ARG;
+  vse_sim *s = g->s;
+  std::vector<float> ones;
+  if (!scale) {
+    ones.assign(s->ncell, 1.0f);
+    scale = ones.data();
+  }
+  for (long i = 0; i < s->ncell; i++)
+    if (!(scale[i] > 0.f) || !std::isfinite(scale[i])) return g->fail(VSE_ERR_ARG, "tau scale must be positive and finite");
+  int rc = g->hip(hipMemcpyAsync(g->ts, scale, s->ncell * sizeof(float), hipMemcpyHostToDevice, s->stream), "tau scale");
+  if (!rc) rc = g->hip(hipStreamSynchronize(s->stream), "tau scale");
+  return rc;
+}
+
+int vse_grad_forward(vse_grad *g, int n_steps, int check_every, float *traces, float *elapsed_ms) {
+  if (!g || n_steps < 1 || n_steps > g->s->d.n_rows) return VSE_ERR_ARG;
+  vse_sim *s = g->s;
+  int rc = finalize(s);
+  if (rc) return g->fail(rc, s->err);
+  DiffArgs A;
+  make_diff(g, A);
+  rc = g->zero_level(g->ckpt[0]);
+  const size_t nt = (size_t)s->d.n_rows * std::max(s->n_rec, 1);
+  if (!rc) rc = g->hip(hipMemsetAsync(g->traces, 0, nt * sizeof(float), s->stream), "zero traces");
+  if (!rc) rc = g->hip(hipMemsetAsync(g->ctrl, 0, sizeof(unsigned), s->stream), "zero ctrl");
+  if (!rc) rc = grad_src_map(g);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && elapsed_ms) {
+    rc = g->hip(hipEventCreate(&e0), "event");
+    if (!rc) rc = g->hip(hipEventCreate(&e1), "event");
+    if (!rc) rc = g->hip(hipEventRecord(e0, s->stream), "event");
+  }
+  // Steps before the last segment write checkpoints at multiples of K and
+  // otherwise alternate between two scratch levels; the last segment's steps
+  // write straight into its segment levels, which the reverse sweep then
+  // uses without recomputing them.
+  const int last = (n_steps - 1) / g->K;      // index of the last segment
+  const DLevel *cur = &g->ckpt[0];
+  for (int n = 0; n < n_steps && !rc; n++) {
+    const int m = n + 1;
+    const DLevel *nxt;
+    if (n >= last * g->K) nxt = &g->lev(last, m - last * g->K);
+    else if (m % g->K == 0) nxt = &g->ckpt[m / g->K];
+    else nxt = (cur == &g->seg[0]) ? &g->seg[1] : &g->seg[0];
+    rc = grad_step(g, A, *cur, *nxt, n, true);
+    cur = nxt;
+    if (!rc && check_every > 0 && m % check_every == 0 && m < n_steps) rc = grad_check(g, m);
+  }
+  if (!rc && elapsed_ms) {
+    rc = g->hip(hipEventRecord(e1, s->stream), "event");
+    if (!rc) rc = g->hip(hipEventSynchronize(e1), "event");
+    if (!rc) rc = g->hip(hipEventElapsedTime(elapsed_ms, e0, e1), "elapsed");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (!rc) rc = grad_check(g, n_steps);
+  if (!rc && traces)
+    rc = g->hip(hipMemcpyAsync(traces, g->traces, (size_t)n_steps * s->n_rec * sizeof(float), hipMemcpyDeviceToHost,
+                               s->stream), "read traces");
+  if (!rc) rc = g->hip(hipStreamSynchronize(s->stream), "forward");
+  g->n_fwd = rc ? 0 : n_steps;
+  g->last_seg_stored = !rc;
+  return rc;
+}
+
+int vse_grad_misfit(vse_grad *g, const float *observed, double *misfit) {
+  if (!g || !observed || !misfit) return VSE_ERR_ARG;
+  if (g->n_fwd < 1) return g->fail(VSE_ERR_STATE, "vse_grad_forward not run");
+  vse_sim *s = g->s;
+  const long n = (long)g->n_fwd * s->n_rec;
+  int rc = g->hip(hipMemcpyAsync(g->obs, observed, n * sizeof(float), hipMemcpyHostToDevice, s->stream), "observed");
+  if (!rc) rc = g->hip(hipMemsetAsync(g->misfit, 0, sizeof(double), s->stream), "zero misfit");
+  if (!rc && n > 0) rc = g->hip(vse_launch_misfit(g->traces, g->obs, g->gbar, n, g->misfit, s->stream), "misfit");
+  if (!rc) rc = g->hip(hipMemcpyAsync(misfit, g->misfit, sizeof(double), hipMemcpyDeviceToHost, s->stream), "misfit");
+  if (!rc) rc = g->hip(hipStreamSynchronize(s->stream), "misfit");
+  return rc;
+}
+
+int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms) {
+  if (!g) return VSE_ERR_ARG;
+  const int N = g->n_fwd;
+  if (N < 1) return g->fail(VSE_ERR_STATE, "vse_grad_forward not run");
+  vse_sim *s = g->s;
+  int rc = 0;
+  if (grad_traces)
+    rc = g->hip(hipMemcpyAsync(g->gbar, grad_traces, (size_t)N * s->n_rec * sizeof(float), hipMemcpyHostToDevice,
+                               s->stream), "adjoint source");
+  if (!rc) rc = g->zero_level(g->bar);
+  for (int i = 0; i < 8 && !rc; i++)
+    rc = g->hip(hipMemsetAsync(g->abar[i], 0, s->ncell * sizeof(float), s->stream), "zero abar");
+  for (int i = 0; i < VSE_GR_COUNT && !rc; i++)
+    rc = g->hip(hipMemsetAsync(g->g[i], 0, s->ncell * sizeof(double), s->stream), "zero gradient");
+  if (!rc) rc = grad_src_map(g);
+  DiffArgs A;
+  make_diff(g, A);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && elapsed_ms) {
+    rc = g->hip(hipEventCreate(&e0), "event");
+    if (!rc) rc = g->hip(hipEventCreate(&e1), "event");
+    if (!rc) rc = g->hip(hipEventRecord(e0, s->stream), "event");
+  }
+  const int nseg = (N + g->K - 1) / g->K;
+  for (int sg = nseg - 1; sg >= 0 && !rc; sg--) {
+    const int n0 = sg * g->K, n1 = std::min(n0 + g->K, N);
+    if (!(sg == nseg - 1 && g->last_seg_stored))
+      for (int j = 0; j < n1 - n0 && !rc; j++) rc = grad_step(g, A, g->lev(sg, j), g->lev(sg, j + 1), n0 + j, false);
+    for (int n = n1 - 1; n >= n0 && !rc; n--) {
+      A.in = g->lev(sg, n - n0);
+      A.out = g->lev(sg, n - n0 + 1);
+      A.n = n;
+      hipError_t e = vse_launch_adj_velocity(A, s->stream);
+      if (e == hipSuccess) e = vse_launch_adj_stress(A, s->stream);
+      rc = g->hip(e, "adjoint step");
+    }
+  }
+  g->last_seg_stored = false;    // the recomputed segments overwrote it
+  if (!rc && elapsed_ms) {
+    rc = g->hip(hipEventRecord(e1, s->stream), "event");
+    if (!rc) rc = g->hip(hipEventSynchronize(e1), "event");
+    if (!rc) rc = g->hip(hipEventElapsedTime(elapsed_ms, e0, e1), "elapsed");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (!rc) rc = g->hip(hipStreamSynchronize(s->stream), "backward");
+  return rc;
+}
+
+int vse_grad_read(vse_grad *g, int which, double *out) {
+  if (!g || which < 0 || which >= VSE_GR_COUNT || !out) return VSE_ERR_ARG;
+  vse_sim *s = g->s;
+  int rc = g->hip(hipMemcpyAsync(out, g->g[which], s->ncell * sizeof(double), hipMemcpyDeviceToHost, s->stream), "read gradient");
+  if (!rc) rc = g->hip(hipStreamSynchronize(s->stream), "read gradient");
+  return rc;
+}
+
+int vse_grad_read_state(vse_grad *g, int field, float *out) {
+  if (!g || field < 0 || field > 4 || !out) return VSE_ERR_ARG;
+  if (g->n_fwd < 1 || !g->last_seg_stored) return g->fail(VSE_ERR_STATE, "no forward state (run vse_grad_forward)");
+  const int m = g->n_fwd, last = (m - 1) / g->K;
+  const DLevel &lv = g->lev(last, m - last * g->K);    // where the forward ended
+  const float *src[5] = {lv.vx, lv.vy, lv.sxx, lv.syy, lv.sxy};
+  vse_sim *s = g->s;
+  int rc = g->hip(hipMemcpyAsync(out, src[field], s->ncell * sizeof(float), hipMemcpyDeviceToHost, s->stream), "read state");
+  if (!rc) rc = g->hip(hipStreamSynchronize(s->stream), "read state");
+  return rc;
+}
+
+int vse_grad_segment(vse_grad *g) { return g ? g->K : 0; }
+
+size_t vse_grad_device_bytes(vse_grad *g) { return g ? g->bytes : 0; }
+
+const char *vse_grad_last_error(vse_grad *g) { return g ? g->err.c_str() : "null handle"; }
 
 }  // extern "C"

@@ -28,7 +28,7 @@ __device__ __forceinline__ float cpml_stretch(const float *a, const float *b, co
   return D / k[j] + pn;
 }
 
-// Max over a 64-wide wavefront, then one atomicMax per block.  For |v|^2 >= 0
+// Max over a 64-wide wavefront, then at most one atomicMax per block.  For |v|^2 >= 0
 // the IEEE bit pattern orders like the value; NaN sorts above +inf, so the
 // running max also flags NaN (the reference's `vsn2 > 1e38` test does not).
 __device__ __forceinline__ void block_max_u32(unsigned v, unsigned *dst) {
@@ -45,6 +45,9 @@ __device__ __forceinline__ void block_max_u32(unsigned v, unsigned *dst) {
   if (tid == 0) {
     unsigned m = 0;
     for (int w = 0; w < (nthr + 63) / 64; w++) m = m > wmax[w] ? m : wmax[w];
-    if (m) atomicMax(dst, m);
+    // Read the running max first: once it is up to date most workgroups skip
+    // the atomic, instead of every workgroup of the launch serialising on one
+    // address.  A stale read only costs an atomic that changes nothing.
+    if (m && m > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(dst, m);
   }
 }

@@ -130,6 +130,54 @@ struct BatchArgs {
   int tile_beg[VSE_MAX_BATCH + 1];
 };
 
+// ---- differentiable 2D viscoelastic path (vse_adj2d.hip) -----------------
+// One time level of the 2D viscoelastic state.  psi[i] is 2D slot
+// kPsiSlots2D[i] in the compact band layout of the forward kernels.
+struct DLevel {
+  VSE_G float *vx, *vy, *sxx, *syy, *sxy;
+  VSE_G float *r[3];                 // xx, yy, xy: L planes of ncell each
+  VSE_G float *psi[8];
+};
+
+// Gradient accumulators (FP64), one per coefficient the kernels use.
+enum { VSE_GR_LAMBDA = 0, VSE_GR_MU = 1, VSE_GR_MU_XY = 2, VSE_GR_RHO_VX = 3, VSE_GR_RHO_VY = 4,
+       VSE_GR_TAU = 5, VSE_GR_COUNT = 6 };
+
+struct DiffArgs {
+  int nx, ny, ord;
+  long ncell;
+  float dx, dy, dt;
+  float C[VSE_MAX_ORD];
+  AxisProf ax[2];
+  const VSE_G float *rho, *cp, *cs;
+  const VSE_G float *ts;             // per-cell relaxation-time scale
+  int L;
+  DLevel in, out;                    // forward: n -> n+1; adjoint: levels n and n+1
+  DLevel bar;                        // adjoint state (in place)
+  VSE_G float *abar[8];              // adjoints of the raw derivatives a1..a8
+  VSE_G double *g[VSE_GR_COUNT];
+  unsigned *v2max;
+  ZenerK zk;                         // the constants at scale 1 (vse_create's values)
+  // sources and receivers, handled inside the velocity kernels
+  int n;                             // the step being advanced / reversed
+  const VSE_G int *src_map;          // per cell: source column or -1
+  const VSE_G float *src;            // (src_rows, src_cols)
+  int src_cols, src_rows;
+  int n_rec, rec_gate;
+  const VSE_G int *rec_beg, *rec_cell, *rec_delay;
+  VSE_G float *traces;               // (rows, n_rec), forward
+  const VSE_G float *gbar;           // (rows, n_rec), adjoint source
+  int record;                        // forward: record traces of step n
+};
+
+hipError_t vse_launch_dstress(const DiffArgs &A, hipStream_t st);
+hipError_t vse_launch_dvelocity(const DiffArgs &A, hipStream_t st);
+hipError_t vse_launch_adj_velocity(const DiffArgs &A, hipStream_t st);
+hipError_t vse_launch_src_map(int *map, long ncell, const int *cells, const int *cols, int n, hipStream_t st);
+hipError_t vse_launch_adj_stress(const DiffArgs &A, hipStream_t st);
+hipError_t vse_launch_misfit(const float *traces, const float *obs, float *resid, long n, double *acc,
+                             hipStream_t st);
+
 // launchers (vse_k2d.hip / vse_k3d.hip)
 hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st);
 hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st);

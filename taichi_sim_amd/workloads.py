@@ -14,6 +14,8 @@ dt = 0.5 dx / (cp_max sqrt(ndim)).  Random media are seeded.
   workload of a full focal-law scan.
 * ``fmc_2d``: full-matrix capture (law k fires element k only) on
   ``visco_2d_array`` specimens of several sizes.
+* ``adjoint_2d``: the inversion test problem -- a true model with seeded
+  Gaussian velocity anomalies, a homogeneous starting model, FMC shots.
 * ``visco_3d_block``: 3D Zener two-layer block (aluminium over steel) with
   random ellipsoidal density inclusions, a circular piston on the top face and
   a square grid of surface receivers.
@@ -159,6 +161,36 @@ def fmc_2d(sizes=(512, 1024, 1536, 2048), n_elem=16, steps=3000, seeds=(40, 41, 
         np.fill_diagonal(laws, 1.0)
         out.append((cfg, laws))
     return out
+
+
+def adjoint_2d(n=1024, steps=3000, n_shots=8, n_anom=8, seed=5, amp=0.05, depth=(4.0, 30.0)):
+    """(config, laws, true_maps): a steel plate of n^2 cells (incl. CPML), an
+    ``n_shots``-element array on the top face firing one element per shot
+    (law k fires element k), and the true model: the background with
+    ``n_anom`` seeded Gaussian anomalies of +-amp in cp and cs (sigma 2-5 mm)
+    centred ``depth`` mm below the array -- within reach of a reflection in
+    3000 steps (18 us of two-way time in steel is ~53 mm) -- as (rho, cp, cs)
+    maps over the full grid.  The configuration itself holds the homogeneous
+    background -- the starting model of an inversion."""
+    cfg, laws = fmc_2d(sizes=(n,), n_elem=n_shots, steps=steps, seeds=(seed,), n_voids=0)[0]
+    L, Lz = cfg["roi"]["w_len"], cfg["roi"]["h_len"]
+    nx, nz = L + 2 * PML + 2, Lz + 2 * PML + 2
+    rng = np.random.default_rng(seed)
+    X, Z = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    pert_p = np.zeros((nx, nz))
+    pert_s = np.zeros((nx, nz))
+    z_top = PML + 1                                    # grid row of the top ROI face (the array)
+    for _ in range(n_anom):
+        cx = rng.uniform(0.2, 0.8) * nx
+        cz = z_top + rng.uniform(depth[0], depth[1]) / DX
+        sig = rng.uniform(2.0, 5.0) / DX
+        g = np.exp(-((X - cx) ** 2 + (Z - cz) ** 2) / (2 * sig * sig))
+        pert_p += rng.choice([-1.0, 1.0]) * amp * g
+        pert_s += rng.choice([-1.0, 1.0]) * amp * g
+    cp, cs, rho = STEEL
+    true = (np.full((nx, nz), rho, np.float32), (cp * (1 + pert_p)).astype(np.float32),
+            (cs * (1 + pert_s)).astype(np.float32))
+    return cfg, laws, true
 
 
 def visco_3d_block(n=512, steps=2000, n_incl=24, seed=3, q=50.0, n_rec_side=32, rec_pitch=1.0):

@@ -59,6 +59,8 @@ def main():
     ap.add_argument("--workload", choices=("3d", "shots", "adjoint"), default="3d")
     ap.add_argument("--fd-dirs", type=int, default=2, help="finite-difference directions (--workload adjoint)")
     ap.add_argument("--adj-shots", type=int, default=8, help="shots (--workload adjoint)")
+    ap.add_argument("--sizes", default="512,1024,1536,2048",
+                    help="specimen grid sizes (--workload shots), comma separated")
     ap.add_argument("--shots", type=int, default=16, help="array elements = FMC shots per specimen (--workload shots)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 5 launches): rocprof counter calibration")
@@ -186,11 +188,12 @@ def bench_shots(args):
     from taichi_sim_amd.sim_support.simulator import Simulator
 
     shot_len = 3000                                   # steps of one shot
-    sizes = (512, 1024, 1536, 2048)
+    sizes = tuple(int(v) for v in args.sizes.split(","))
     steps = args.steps
     t_setup = time.perf_counter()
     sims, cells = [], 0
-    for cfg, laws in workloads.fmc_2d(sizes=sizes, n_elem=args.shots, steps=shot_len):
+    for cfg, laws in workloads.fmc_2d(sizes=sizes, n_elem=args.shots, steps=shot_len,
+                                      seeds=tuple(40 + i for i in range(len(sizes)))):
         sim = Simulator(cfg)
         sim._emission_laws = laws
         att = attenuation_from_config(cfg)

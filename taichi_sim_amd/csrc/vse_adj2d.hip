@@ -3,7 +3,10 @@
 // Forward: the WGSL 2D viscoelastic step (shader_webgpu_viscoelastic.wgsl
 // :829-1065, the same arithmetic as vse_k2d.hip) written out of place --
 // level n is read, level n+1 written -- so that the recompute sweep of the
-// checkpointed gradient stores every state of a segment without copies.  It
+// checkpointed gradient stores every state of a segment without copies.
+// Every thread reads its own cell's state before writing it, and neighbours
+// are read only from fields the pass does not write, so the kernels are also
+// correct in place (in == out).  It
 // adds a per-cell relaxation-time scale s: every tau of the Zener mechanisms
 // at a cell is multiplied by s, so 1/tau_sigma becomes (1/tau_sigma)*(1/s)
 // in phi and h (wgsl2d :881-894); at s = 1 the step is bit-identical to the

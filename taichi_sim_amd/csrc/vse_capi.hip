@@ -1122,10 +1122,12 @@ int vse_grad_forward(vse_grad *g, int n_steps, int check_every, float *traces, f
     if (!rc) rc = g->hip(hipEventCreate(&e1), "event");
     if (!rc) rc = g->hip(hipEventRecord(e0, s->stream), "event");
   }
-  // Steps before the last segment write checkpoints at multiples of K and
-  // otherwise alternate between two scratch levels; the last segment's steps
-  // write straight into its segment levels, which the reverse sweep then
-  // uses without recomputing them.
+  // Steps before the last segment run in place on one working level (the
+  // kernels read a cell's own values before writing them, so in == out is
+  // valid, and in place keeps half as many address streams open), except
+  // that a step reaching a multiple of K writes its checkpoint out of place.
+  // The last segment's steps write straight into its segment levels, which
+  // the reverse sweep then uses without recomputing them.
   const int last = (n_steps - 1) / g->K;      // index of the last segment
   const DLevel *cur = &g->ckpt[0];
   for (int n = 0; n < n_steps && !rc; n++) {
@@ -1133,7 +1135,7 @@ int vse_grad_forward(vse_grad *g, int n_steps, int check_every, float *traces, f
     const DLevel *nxt;
     if (n >= last * g->K) nxt = &g->lev(last, m - last * g->K);
     else if (m % g->K == 0) nxt = &g->ckpt[m / g->K];
-    else nxt = (cur == &g->seg[0]) ? &g->seg[1] : &g->seg[0];
+    else nxt = &g->seg[0];
     rc = grad_step(g, A, *cur, *nxt, n, true);
     cur = nxt;
     if (!rc && check_every > 0 && m % check_every == 0 && m < n_steps) rc = grad_check(g, m);

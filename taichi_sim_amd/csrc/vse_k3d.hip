@@ -82,7 +82,12 @@ __device__ __forceinline__ float dterm(const StepArgs &A, int c, float a, float 
   else return A.C[c] * (a - b) / (AX == 0 ? A.dx : (AX == 1 ? A.dy : A.dz));
 }
 
-template <bool VISCO, int ORD, int TY, bool NT, bool FAST>
+// LC > 0: the number of Zener mechanisms is the compile-time LC, and every
+// own-cell load of a section (memory variables, then stresses) is issued
+// before its arithmetic, so a wavefront has them all in flight at once
+// (the runtime-L loop waits on each mechanism's loads in turn).  The
+// arithmetic and its order are the same, so results are bit-identical.
+template <bool VISCO, int ORD, int TY, bool NT, bool FAST, int LC = 0>
 __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, int nzb, int nyb, int nxc, int remap) {
   constexpr int CP = NT ? 2 : 0;
   const Tile T = tile_of_block(nzb, nyb, nxc, remap);
@@ -155,7 +160,35 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
       const float mu = rho0 * (cs0 * cs0);
       const float lp2m = lambda + 2.0f * mu;
       const PB sxxP(A.sxx + rofs, pbytes), syyP(A.syy + rofs, pbytes), szzP(A.szz + rofs, pbytes);
-      if constexpr (VISCO) {
+      if constexpr (VISCO && LC > 0) {
+        float oxx[LC], oyy[LC], ozz[LC];
+#pragma unroll
+        for (int l = 0; l < LC; l++) {
+          const long lo = (long)l * A.ncell + rofs;
+          oxx[l] = PB(A.r[0] + lo, pbytes).ld<CP>(col);
+          oyy[l] = PB(A.r[1] + lo, pbytes).ld<CP>(col);
+          ozz[l] = PB(A.r[2] + lo, pbytes).ld<CP>(col);
+        }
+        const float sxx0 = sxxP.ld<CP>(col), syy0 = syyP.ld<CP>(col), szz0 = szzP.ld<CP>(col);
+        float Sxx = 0.f, Syy = 0.f, Szz = 0.f;
+        const float div = dvx + dvy + dvz;
+        const float div3 = FAST ? div * (1.0f / 3.0f) : div / 3.0f;
+#pragma unroll
+        for (int l = 0; l < LC; l++) {
+          const float nxx = ((oxx[l] + div * A.zk.phi_p[l] - oxx[l] * A.zk.h_p[l]) * A.zk.m_p[l]);
+          const float nyy = ((oyy[l] + (dvx - div3) * A.zk.phi_s[l] - oyy[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
+          const float nzz = ((ozz[l] + (dvy - div3) * A.zk.phi_s[l] - ozz[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
+          Sxx += oxx[l] + nxx; Syy += oyy[l] + nyy; Szz += ozz[l] + nzz;
+          const long lo = (long)l * A.ncell + rofs;
+          PB(A.r[0] + lo, pbytes).st<CP>(col, nxx);
+          PB(A.r[1] + lo, pbytes).st<CP>(col, nyy);
+          PB(A.r[2] + lo, pbytes).st<CP>(col, nzz);
+        }
+        const float l23 = lambda + 2.0f * mu / 3.0f;
+        sxxP.st<CP>(col, sxx0 + (lp2m * dvx + lambda * (dvy + dvz) + l23 * 0.5f * Sxx + mu * Syy) * dt);
+        syyP.st<CP>(col, syy0 + (lambda * (dvx + dvz) + lp2m * dvy + l23 * 0.5f * Sxx + mu * Szz) * dt);
+        szzP.st<CP>(col, szz0 + (lambda * (dvx + dvy) + lp2m * dvz + lp2m * 0.5f * Sxx - (mu / 3.0f) * (Syy + Szz)) * dt);
+      } else if constexpr (VISCO) {
         float Sxx = 0.f, Syy = 0.f, Szz = 0.f;
         const float div = dvx + dvy + dvz;
         const float div3 = FAST ? div * (1.0f / 3.0f) : div / 3.0f;
@@ -224,7 +257,21 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
       }
       const float mu = rho4 * (cs4 * cs4);
       const PB sigP(sig + rofs, pbytes);
-      if constexpr (VISCO) {
+      if constexpr (VISCO && LC > 0) {
+        float o[LC];
+#pragma unroll
+        for (int l = 0; l < LC; l++) o[l] = PB(rr + (long)l * A.ncell + rofs, pbytes).ld<CP>(col);
+        const float sig0 = sigP.ld<CP>(col);
+        float S = 0.f;
+        const float X = d1 + d2;
+#pragma unroll
+        for (int l = 0; l < LC; l++) {
+          const float n = (o[l] + X * A.zk.phi_s[l] - o[l] * A.zk.h_s[l]) * A.zk.m_s[l];
+          S += o[l] + n;
+          PB(rr + (long)l * A.ncell + rofs, pbytes).st<CP>(col, n);
+        }
+        sigP.st<CP>(col, sig0 + (mu * (d1 + d2) + mu * 0.5f * S) * dt);
+      } else if constexpr (VISCO) {
         float S = 0.f;
         const float X = d1 + d2;
         for (int l = 0; l < A.L; l++) {
@@ -292,15 +339,32 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
       const PB sxyP(A.sxy + rofs, pbytes), sxzP(A.sxz + rofs, pbytes), syyP(A.syy + rofs, pbytes);
       const PB syzP(A.syz + rofs, pbytes), szzP(A.szz + rofs, pbytes), rhoP(A.rho + rofs, pbytes);
       const PB vxP(A.vx + rofs, pbytes), vyP(A.vy + rofs, pbytes), vzP(A.vz + rofs, pbytes);
+      // Every load of the plane first, unconditionally: buffer loads are
+      // range-checked per plane (an offset outside it reads 0, and such
+      // values are used only outside the interiors, where they are not),
+      // so the wavefront has all of them in flight before any arithmetic.
       float vxn = vxP.ld<CP>(col), vyn = vyP.ld<CP>(col), vzn = vzP.ld<CP>(col);
+      float xy_p[ORD], xy_m[ORD], xz_p[ORD], xz_m[ORD];      // vx: sxy along y, sxz along z (D-)
+      float yy_p[ORD], yy_m[ORD], yz_p[ORD], yz_m[ORD];      // vy: syy along y (D+), syz along z (D-)
+      float zy_p[ORD], zy_m[ORD], zz_p[ORD], zz_m[ORD];      // vz: syz along y (D-), szz along z (D+)
+#pragma unroll
+      for (int c = 0; c < ORD; c++) {
+        xy_p[c] = sxyP.ld(col + c * sy); xy_m[c] = sxyP.ld(col - (c + 1) * sy);
+        xz_p[c] = sxzP.ld(col + c); xz_m[c] = sxzP.ld(col - c - 1);
+        yy_p[c] = syyP.ld(col + (c + 1) * sy); yy_m[c] = syyP.ld(col - c * sy);
+        yz_p[c] = syzP.ld(col + c); yz_m[c] = syzP.ld(col - c - 1);
+        zy_p[c] = syzP.ld(col + c * sy); zy_m[c] = syzP.ld(col - (c + 1) * sy);
+        zz_p[c] = szzP.ld(col + c + 1); zz_m[c] = szzP.ld(col - c);
+      }
+      const float rho_y1 = rhoP.ld(col + sy);
       // vx (wgsl3d :1647-1681)
       if (x >= last + 1 && x < nx - last && yi_vx && zi_vx) {
         float a = 0.f, b = 0.f, c_ = 0.f;
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
           a += dterm<FAST, 0>(A, c, wxx[ORD + c], wxx[ORD - 1 - c]);          // sxx[x+c] - sxx[x-c-1]
-          b += dterm<FAST, 1>(A, c, sxyP.ld(col + c * sy), sxyP.ld(col - (c + 1) * sy));
-          c_ += dterm<FAST, 2>(A, c, sxzP.ld(col + c), sxzP.ld(col - c - 1));
+          b += dterm<FAST, 1>(A, c, xy_p[c], xy_m[c]);
+          c_ += dterm<FAST, 2>(A, c, xz_p[c], xz_m[c]);
         }
         if (bx >= 0) a = stretch<CP, FAST>(A.ax[0].a, A.ax[0].b, A.ax[0].k, A.ax[0].ki, PB(A.psi[P_DSXX_DX] + (long)bx * sx, pbytes), col, jx, a);
         if (by >= 0) b = stretch<CP, FAST>(A.ax[1].a, A.ax[1].b, A.ax[1].k, A.ax[1].ki, PB(A.psi[P_DSXY_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
@@ -315,13 +379,13 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
           a += dterm<FAST, 0>(A, c, wxy[ORD + c], wxy[ORD - 1 - c]);          // sxy[x+c+1] - sxy[x-c]
-          b += dterm<FAST, 1>(A, c, syyP.ld(col + (c + 1) * sy), syyP.ld(col - c * sy));
-          c_ += dterm<FAST, 2>(A, c, syzP.ld(col + c), syzP.ld(col - c - 1));
+          b += dterm<FAST, 1>(A, c, yy_p[c], yy_m[c]);
+          c_ += dterm<FAST, 2>(A, c, yz_p[c], yz_m[c]);
         }
         if (bx >= 0) a = stretch<CP, FAST>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, A.ax[0].khi, PB(A.psi[P_DSXY_DX] + (long)bx * sx, pbytes), col, jx, a);
         if (by >= 0) b = stretch<CP, FAST>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, A.ax[1].khi, PB(A.psi[P_DSYY_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
         if (bz >= 0) c_ = stretch<CP, FAST>(A.ax[2].a, A.ax[2].b, A.ax[2].k, A.ax[2].ki, PB(A.psi[P_DSYZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
-        const float rho = 0.5f * (rhoP.ld(col + sy) + rho0);
+        const float rho = 0.5f * (rho_y1 + rho0);
         if (rho > 0.f) vyn = (a + b + c_) * dt / rho + vyn;
       }
       // vz, with the vy density (wgsl3d :1719-1758)
@@ -330,13 +394,13 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
           a += dterm<FAST, 0>(A, c, wxz[ORD + c], wxz[ORD - 1 - c]);          // sxz[x+c+1] - sxz[x-c]
-          b += dterm<FAST, 1>(A, c, syzP.ld(col + c * sy), syzP.ld(col - (c + 1) * sy));
-          c_ += dterm<FAST, 2>(A, c, szzP.ld(col + c + 1), szzP.ld(col - c));
+          b += dterm<FAST, 1>(A, c, zy_p[c], zy_m[c]);
+          c_ += dterm<FAST, 2>(A, c, zz_p[c], zz_m[c]);
         }
         if (bx >= 0) a = stretch<CP, FAST>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, A.ax[0].khi, PB(A.psi[P_DSXZ_DX] + (long)bx * sx, pbytes), col, jx, a);
         if (by >= 0) b = stretch<CP, FAST>(A.ax[1].a, A.ax[1].b, A.ax[1].k, A.ax[1].ki, PB(A.psi[P_DSYZ_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
         if (bz >= 0) c_ = stretch<CP, FAST>(A.ax[2].ah, A.ax[2].bh, A.ax[2].kh, A.ax[2].khi, PB(A.psi[P_DSZZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
-        const float rho = 0.5f * (rhoP.ld(col + sy) + rho0);
+        const float rho = 0.5f * (rho_y1 + rho0);
         if (rho > 0.f) vzn = (a + b + c_) * dt / rho + vzn;
       } else {
         vxn = 0.f; vyn = 0.f; vzn = 0.f;
@@ -366,7 +430,7 @@ __device__ __forceinline__ float cpml_stretch_m(const float *a, const float *b, 
   else return D / k[j] + pn;
 }
 
-template <bool VISCO, int ORD, bool FAST>
+template <bool VISCO, int ORD, bool FAST, int LC = 0>
 __global__ __launch_bounds__(BZ * BYY) void k_stress3d_plain(StepArgs A) {
   const int z = blockIdx.x * BZ + threadIdx.x;
   const int y = blockIdx.y * BYY + threadIdx.y;
@@ -402,7 +466,33 @@ __global__ __launch_bounds__(BZ * BYY) void k_stress3d_plain(StepArgs A) {
     const float lambda = rho * (cp * cp - 2.0f * cs * cs);
     const float mu = rho * (cs * cs);
     const float lp2m = lambda + 2.0f * mu;
-    if constexpr (VISCO) {
+    if constexpr (VISCO && LC > 0) {      // all own-cell loads first (see k_stress3d)
+      float oxx[LC], oyy[LC], ozz[LC];
+#pragma unroll
+      for (int l = 0; l < LC; l++) {
+        oxx[l] = A.r[0][l * A.ncell + id];
+        oyy[l] = A.r[1][l * A.ncell + id];
+        ozz[l] = A.r[2][l * A.ncell + id];
+      }
+      const float sxx0 = A.sxx[id], syy0 = A.syy[id], szz0 = A.szz[id];
+      float Sxx = 0.f, Syy = 0.f, Szz = 0.f;
+      const float div = dvx + dvy + dvz;
+      const float div3 = FAST ? div * (1.0f / 3.0f) : div / 3.0f;
+#pragma unroll
+      for (int l = 0; l < LC; l++) {
+        const float nxx = ((oxx[l] + div * A.zk.phi_p[l] - oxx[l] * A.zk.h_p[l]) * A.zk.m_p[l]);
+        const float nyy = ((oyy[l] + (dvx - div3) * A.zk.phi_s[l] - oyy[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
+        const float nzz = ((ozz[l] + (dvy - div3) * A.zk.phi_s[l] - ozz[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
+        Sxx += oxx[l] + nxx; Syy += oyy[l] + nyy; Szz += ozz[l] + nzz;
+        A.r[0][l * A.ncell + id] = nxx;
+        A.r[1][l * A.ncell + id] = nyy;
+        A.r[2][l * A.ncell + id] = nzz;
+      }
+      const float l23 = lambda + 2.0f * mu / 3.0f;
+      A.sxx[id] = sxx0 + (lp2m * dvx + lambda * (dvy + dvz) + l23 * 0.5f * Sxx + mu * Syy) * dt;
+      A.syy[id] = syy0 + (lambda * (dvx + dvz) + lp2m * dvy + l23 * 0.5f * Sxx + mu * Szz) * dt;
+      A.szz[id] = szz0 + (lambda * (dvx + dvy) + lp2m * dvz + lp2m * 0.5f * Sxx - (mu / 3.0f) * (Syy + Szz)) * dt;
+    } else if constexpr (VISCO) {
       float Sxx = 0.f, Syy = 0.f, Szz = 0.f;
       const float div = dvx + dvy + dvz;
       const float div3 = FAST ? div * (1.0f / 3.0f) : div / 3.0f;
@@ -472,7 +562,21 @@ __global__ __launch_bounds__(BZ * BYY) void k_stress3d_plain(StepArgs A) {
       sig = A.syz; rr = A.r[5];
     }
     const float mu = rho4 * (cs4 * cs4);
-    if constexpr (VISCO) {
+    if constexpr (VISCO && LC > 0) {
+      float o[LC];
+#pragma unroll
+      for (int l = 0; l < LC; l++) o[l] = rr[l * A.ncell + id];
+      const float sig0 = sig[id];
+      float S = 0.f;
+      const float X = d1 + d2;
+#pragma unroll
+      for (int l = 0; l < LC; l++) {
+        const float n = (o[l] + X * A.zk.phi_s[l] - o[l] * A.zk.h_s[l]) * A.zk.m_s[l];
+        S += o[l] + n;
+        rr[l * A.ncell + id] = n;
+      }
+      sig[id] = sig0 + (mu * (d1 + d2) + mu * 0.5f * S) * dt;
+    } else if constexpr (VISCO) {
       float S = 0.f;
       const float X = d1 + d2;
       for (int l = 0; l < A.L; l++) {
@@ -584,7 +688,14 @@ hipError_t launch_one(bool stress, int visco, const StepArgs &A, int xchunk, int
   const Launch L = launch_shape(A, TY, xchunk);
   const dim3 grid((unsigned)(L.nzb * L.nyb * L.nxc)), block(BZ, TY);
   if (stress) {
-    if (visco) hipLaunchKernelGGL((k_stress3d<true, ORD, TY, NT, FAST>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
+    bool done = false;
+    if constexpr (ORD == 2)
+      if (visco && A.L == 3) {
+        hipLaunchKernelGGL((k_stress3d<true, ORD, TY, NT, FAST, 3>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
+        done = true;
+      }
+    if (done) {
+    } else if (visco) hipLaunchKernelGGL((k_stress3d<true, ORD, TY, NT, FAST>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
     else hipLaunchKernelGGL((k_stress3d<false, ORD, TY, NT, FAST>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
   } else {
     hipLaunchKernelGGL((k_velocity3d<ORD, TY, NT, FAST>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
@@ -597,7 +708,14 @@ hipError_t launch_tuned(bool stress, int visco, const StepArgs &A, const KTune &
   if (t.xchunk == 0) {     // plain one-cell-per-thread kernels
     const dim3 block(BZ, BYY), grid((A.nz + BZ - 1) / BZ, (A.ny + BYY - 1) / BYY, A.nx);
     if (stress) {
-      if (visco) hipLaunchKernelGGL((k_stress3d_plain<true, ORD, FAST>), grid, block, 0, st, A);
+      bool done = false;
+      if constexpr (ORD == 2)
+        if (visco && A.L == 3) {
+          hipLaunchKernelGGL((k_stress3d_plain<true, ORD, FAST, 3>), grid, block, 0, st, A);
+          done = true;
+        }
+      if (done) {
+      } else if (visco) hipLaunchKernelGGL((k_stress3d_plain<true, ORD, FAST>), grid, block, 0, st, A);
       else hipLaunchKernelGGL((k_stress3d_plain<false, ORD, FAST>), grid, block, 0, st, A);
     } else {
       hipLaunchKernelGGL((k_velocity3d_plain<ORD, FAST>), grid, block, 0, st, A);

@@ -142,6 +142,31 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
     const bool xm = x >= last + 1 && x < nx - last;
     const PB vxP = P(A.vx, x), vyP = P(A.vy, x), vzP = P(A.vz, x), rhoP = P(A.rho, x), csP = P(A.cs, x);
     const long rofs = (long)x * sx;
+    // LC > 0: the plane's neighbour loads (v along y and z, rho/cs next to
+    // the shear points) are all issued here, before any arithmetic; buffer
+    // loads outside a plane read 0 and those values are used only outside
+    // the interiors, where they are not.
+    constexpr bool PRE = VISCO && LC > 0;
+    float nvy_p[ORD], nvy_m[ORD], nvz_p[ORD], nvz_m[ORD];        // normal: dvy (D-), dvz (D-)
+    float nxy_p[ORD], nxy_m[ORD], nxz_p[ORD], nxz_m[ORD];        // shear: dvx/dy, dvx/dz (D+)
+    float nzy_p[ORD], nzy_m[ORD], nyz_p[ORD], nyz_m[ORD];        // shear: dvz/dy, dvy/dz (D+)
+    float m_r0y = 0.f, m_r1y = 0.f, m_c0y = 0.f, m_c1y = 0.f, m_r0z = 0.f, m_r1z = 0.f, m_c0z = 0.f, m_c1z = 0.f;
+    float m_r0yz = 0.f, m_c0yz = 0.f, m_cp = 0.f;
+    if constexpr (PRE) {
+#pragma unroll
+      for (int c = 0; c < ORD; c++) {
+        nvy_p[c] = vyP.ld(col + c * sy); nvy_m[c] = vyP.ld(col - (c + 1) * sy);
+        nvz_p[c] = vzP.ld(col + c); nvz_m[c] = vzP.ld(col - c - 1);
+        nxy_p[c] = vxP.ld(col + (c + 1) * sy); nxy_m[c] = vxP.ld(col - c * sy);
+        nxz_p[c] = vxP.ld(col + c + 1); nxz_m[c] = vxP.ld(col - c);
+        nzy_p[c] = vzP.ld(col + (c + 1) * sy); nzy_m[c] = vzP.ld(col - c * sy);
+        nyz_p[c] = vyP.ld(col + c + 1); nyz_m[c] = vyP.ld(col - c);
+      }
+      m_r0y = rhoP.ld(col + sy); m_r1y = rho1P.ld(col + sy); m_c0y = csP.ld(col + sy); m_c1y = cs1P.ld(col + sy);
+      m_r0z = rhoP.ld(col + 1); m_r1z = rho1P.ld(col + 1); m_c0z = csP.ld(col + 1); m_c1z = cs1P.ld(col + 1);
+      m_r0yz = rhoP.ld(col + sy + 1); m_c0yz = csP.ld(col + sy + 1);
+      m_cp = PB(A.cp + rofs, pbytes).ld<CP>(col);
+    }
 
     // ---- normal stresses (wgsl3d :1356-1453): x D+, y D-, z D-
     if (xp && ym && zm) {
@@ -149,13 +174,18 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
 #pragma unroll
       for (int c = 0; c < ORD; c++) {
         dvx += dterm<FAST, 0>(A, c, wvx[ORD + c], wvx[ORD - 1 - c]);            // vx[x+c+1] - vx[x-c]
-        dvy += dterm<FAST, 1>(A, c, vyP.ld(col + c * sy), vyP.ld(col - (c + 1) * sy));
-        dvz += dterm<FAST, 2>(A, c, vzP.ld(col + c), vzP.ld(col - c - 1));
+        if constexpr (PRE) {
+          dvy += dterm<FAST, 1>(A, c, nvy_p[c], nvy_m[c]);
+          dvz += dterm<FAST, 2>(A, c, nvz_p[c], nvz_m[c]);
+        } else {
+          dvy += dterm<FAST, 1>(A, c, vyP.ld(col + c * sy), vyP.ld(col - (c + 1) * sy));
+          dvz += dterm<FAST, 2>(A, c, vzP.ld(col + c), vzP.ld(col - c - 1));
+        }
       }
       if (bx >= 0) dvx = stretch<CP, FAST>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, A.ax[0].khi, PB(A.psi[P_DVX_DX] + (long)bx * sx, pbytes), col, jx, dvx);
       if (by >= 0) dvy = stretch<CP, FAST>(A.ax[1].a, A.ax[1].b, A.ax[1].k, A.ax[1].ki, PB(A.psi[P_DVY_DY] + (long)x * nby * nz, pyb), py_col, jy, dvy);
       if (bz >= 0) dvz = stretch<CP, FAST>(A.ax[2].a, A.ax[2].b, A.ax[2].k, A.ax[2].ki, PB(A.psi[P_DVZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, dvz);
-      const float cp = PB(A.cp + rofs, pbytes).ld<CP>(col);
+      const float cp = PRE ? m_cp : PB(A.cp + rofs, pbytes).ld<CP>(col);
       const float lambda = rho0 * (cp * cp - 2.0f * cs0 * cs0);
       const float mu = rho0 * (cs0 * cs0);
       const float lp2m = lambda + 2.0f * mu;
@@ -225,34 +255,56 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
           d1 += dterm<FAST, 0>(A, c, wvy[ORD + c], wvy[ORD - 1 - c]);         // vy[x+c] - vy[x-c-1]
-          d2 += dterm<FAST, 1>(A, c, vxP.ld(col + (c + 1) * sy), vxP.ld(col - c * sy));
+          if constexpr (PRE) d2 += dterm<FAST, 1>(A, c, nxy_p[c], nxy_m[c]);
+          else d2 += dterm<FAST, 1>(A, c, vxP.ld(col + (c + 1) * sy), vxP.ld(col - c * sy));
         }
         if (bx >= 0) d1 = stretch<CP, FAST>(A.ax[0].a, A.ax[0].b, A.ax[0].k, A.ax[0].ki, PB(A.psi[P_DVY_DX] + (long)bx * sx, pbytes), col, jx, d1);
         if (by >= 0) d2 = stretch<CP, FAST>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, A.ax[1].khi, PB(A.psi[P_DVX_DY] + (long)x * nby * nz, pyb), py_col, jy, d2);
-        rho4 = 0.25f * (rho_n + rho0 + rhoP.ld(col + sy) + rho1P.ld(col + sy));
-        cs4 = 0.25f * (cs_n + cs0 + csP.ld(col + sy) + cs1P.ld(col + sy));
+        if constexpr (PRE) {
+          rho4 = 0.25f * (rho_n + rho0 + m_r0y + m_r1y);
+          cs4 = 0.25f * (cs_n + cs0 + m_c0y + m_c1y);
+        } else {
+          rho4 = 0.25f * (rho_n + rho0 + rhoP.ld(col + sy) + rho1P.ld(col + sy));
+          cs4 = 0.25f * (cs_n + cs0 + csP.ld(col + sy) + cs1P.ld(col + sy));
+        }
         sig = A.sxy; rr = A.r[3];
       } else if (sh == 1) {     // sigma_xz: dvz/dx (D-), dvx/dz (D+)
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
           d1 += dterm<FAST, 0>(A, c, wvz[ORD + c], wvz[ORD - 1 - c]);         // vz[x+c] - vz[x-c-1]
-          d2 += dterm<FAST, 2>(A, c, vxP.ld(col + c + 1), vxP.ld(col - c));
+          if constexpr (PRE) d2 += dterm<FAST, 2>(A, c, nxz_p[c], nxz_m[c]);
+          else d2 += dterm<FAST, 2>(A, c, vxP.ld(col + c + 1), vxP.ld(col - c));
         }
         if (bx >= 0) d1 = stretch<CP, FAST>(A.ax[0].a, A.ax[0].b, A.ax[0].k, A.ax[0].ki, PB(A.psi[P_DVZ_DX] + (long)bx * sx, pbytes), col, jx, d1);
         if (bz >= 0) d2 = stretch<CP, FAST>(A.ax[2].ah, A.ax[2].bh, A.ax[2].kh, A.ax[2].khi, PB(A.psi[P_DVX_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, d2);
-        rho4 = 0.25f * (rho_n + rho0 + rhoP.ld(col + 1) + rho1P.ld(col + 1));
-        cs4 = 0.25f * (cs_n + cs0 + csP.ld(col + 1) + cs1P.ld(col + 1));
+        if constexpr (PRE) {
+          rho4 = 0.25f * (rho_n + rho0 + m_r0z + m_r1z);
+          cs4 = 0.25f * (cs_n + cs0 + m_c0z + m_c1z);
+        } else {
+          rho4 = 0.25f * (rho_n + rho0 + rhoP.ld(col + 1) + rho1P.ld(col + 1));
+          cs4 = 0.25f * (cs_n + cs0 + csP.ld(col + 1) + cs1P.ld(col + 1));
+        }
         sig = A.sxz; rr = A.r[4];
       } else {                  // sigma_yz: dvz/dy (D+), dvy/dz (D+)
 #pragma unroll
         for (int c = 0; c < ORD; c++) {
-          d1 += dterm<FAST, 1>(A, c, vzP.ld(col + (c + 1) * sy), vzP.ld(col - c * sy));
-          d2 += dterm<FAST, 2>(A, c, vyP.ld(col + c + 1), vyP.ld(col - c));
+          if constexpr (PRE) {
+            d1 += dterm<FAST, 1>(A, c, nzy_p[c], nzy_m[c]);
+            d2 += dterm<FAST, 2>(A, c, nyz_p[c], nyz_m[c]);
+          } else {
+            d1 += dterm<FAST, 1>(A, c, vzP.ld(col + (c + 1) * sy), vzP.ld(col - c * sy));
+            d2 += dterm<FAST, 2>(A, c, vyP.ld(col + c + 1), vyP.ld(col - c));
+          }
         }
         if (by >= 0) d1 = stretch<CP, FAST>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, A.ax[1].khi, PB(A.psi[P_DVZ_DY] + (long)x * nby * nz, pyb), py_col, jy, d1);
         if (bz >= 0) d2 = stretch<CP, FAST>(A.ax[2].ah, A.ax[2].bh, A.ax[2].kh, A.ax[2].khi, PB(A.psi[P_DVY_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, d2);
-        rho4 = 0.25f * (rhoP.ld(col + sy) + rho0 + rhoP.ld(col + 1) + rhoP.ld(col + sy + 1));
-        cs4 = 0.25f * (csP.ld(col + sy) + cs0 + csP.ld(col + 1) + csP.ld(col + sy + 1));
+        if constexpr (PRE) {
+          rho4 = 0.25f * (m_r0y + rho0 + m_r0z + m_r0yz);
+          cs4 = 0.25f * (m_c0y + cs0 + m_c0z + m_c0yz);
+        } else {
+          rho4 = 0.25f * (rhoP.ld(col + sy) + rho0 + rhoP.ld(col + 1) + rhoP.ld(col + sy + 1));
+          cs4 = 0.25f * (csP.ld(col + sy) + cs0 + csP.ld(col + 1) + csP.ld(col + sy + 1));
+        }
         sig = A.syz; rr = A.r[5];
       }
       const float mu = rho4 * (cs4 * cs4);

@@ -39,6 +39,13 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the 512^3 viscoelastic stress kernel, by launch-shape
+# family (ty, non-temporal, x-march), measured with rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE (scaled by 2.0 and 1.0 from the copy-kernel calibration); the
+# table and commands are in profiles/visco3d_512cube_kernels.md.
+STRESS_TRAFFIC_512 = {(4, 0, True): 32.76e9, (1, 0, True): 33.65e9, (4, 1, True): 31.68e9,
+                      (1, 1, True): 31.63e9, (4, 0, False): 38.90e9}
+
 
 def bytes_per_cell(g, L):
     """Algorithmic bytes per cell and step: every state field read once and
@@ -136,6 +143,8 @@ def main():
 
     # per-kernel durations (HIP events around each launch), after the timed loop
     k_ms = g.profile(args.profile_steps)
+    ty, nt, xc = tuned["fast"]["stress"][:3]
+    traffic = STRESS_TRAFFIC_512.get((ty, nt if xc else 0, xc > 0)) if args.n == 512 else None
     b_step, b_stress, b_vel = bytes_per_cell(g, g.n_sls)
     achieved = b_stress * ncell / (k_ms[0] * 1e-3) / 1e9
     step_frac = gcell * 1e9 * b_step / (HBM_PEAK_GBS * 1e9)
@@ -181,7 +190,8 @@ def main():
         "bytes_per_cell_step": round(b_step, 2),
         "kernel_ms": {"stress": round(k_ms[0], 4), "velocity": round(k_ms[1], 4), "sparse": round(k_ms[2], 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None else round(traffic),
                      "kernel": "stress3d", "bytes_per_cell": round(b_stress, 2)},
         "cpu_baseline": cpu,
     }

@@ -188,8 +188,10 @@ void vse_destroy(vse_sim *s);
  * of MI355X).  Each group runs all its steps before the next group starts,
  * as hipGraphs of up to check_every steps in which every step is one stress,
  * one velocity and one sparse launch over the tiles of all the group's
- * members; a group of one member runs vse_run on it.  Members must share
- * variant, viscoelastic flag, FD order and device; they keep their own state,
+ * members; a group of one member runs vse_run on it.  Members must be
+ * WGSL-variant simulations (Taichi-variant batches are switched off and
+ * return VSE_ERR_ARG) sharing viscoelastic flag, FD order, number of
+ * mechanisms and device; they keep their own state,
  * so vse_read_* on a member reads its results, and every member's result is
  * identical to vse_run on it alone.  A batch does not own its members:
  * destroy it before them, and do not call a member while a batch call is in

@@ -133,7 +133,10 @@ __device__ __forceinline__ float dT(const DiffArgs &A, const VSE_G float *h, int
 }
 
 // ---------------------------------------------------------------- forward
-template <int ORD>
+// LC > 0: compile-time mechanism count; every memory-variable load of a point
+// is issued before its arithmetic (same operations and order as the runtime
+// loop, so bit-identical).
+template <int ORD, int LC = 0>
 __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
   const int nx = A.nx, ny = A.ny;
   if (x >= nx || y >= ny) return;
@@ -169,7 +172,21 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
     const float lp2m = lambda + 2.0f * mu, lpm = lambda + mu;
     const float rs = 1.0f / A.ts[id];
     float sum_xx = 0.f, sum_yy = 0.f;
-    for (int l = 0; l < L; l++) {
+    if constexpr (LC > 0) {
+      float oxx[LC], oyy[LC];
+#pragma unroll
+      for (int l = 0; l < LC; l++) { oxx[l] = I.r[0][l * nc + id]; oyy[l] = I.r[1][l * nc + id]; }
+#pragma unroll
+      for (int l = 0; l < LC; l++) {
+        const ZCell z = zener_cell(A, l, rs);
+        const float nxx = ((oxx[l] + (dvx + dvy) * z.phi_p - oxx[l] * z.h_p) * z.m_p);
+        const float nyy = ((oyy[l] + 0.5f * (dvx - dvy) * z.phi_s - oyy[l] * z.h_s) * z.m_s);
+        sum_xx += oxx[l] + nxx;
+        sum_yy += oyy[l] + nyy;
+        O.r[0][l * nc + id] = nxx;
+        O.r[1][l * nc + id] = nyy;
+      }
+    } else for (int l = 0; l < L; l++) {
       const ZCell z = zener_cell(A, l, rs);
       const float oxx = I.r[0][l * nc + id], oyy = I.r[1][l * nc + id];
       const float nxx = ((oxx + (dvx + dvy) * z.phi_p - oxx * z.h_p) * z.m_p);
@@ -211,7 +228,18 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
     const float mu = rho_h * (cs_h * cs_h);
     const float rs = 1.0f / A.ts[id];
     float sum_xy = 0.f;
-    for (int l = 0; l < L; l++) {
+    if constexpr (LC > 0) {
+      float o[LC];
+#pragma unroll
+      for (int l = 0; l < LC; l++) o[l] = I.r[2][l * nc + id];
+#pragma unroll
+      for (int l = 0; l < LC; l++) {
+        const ZCell z = zener_cell(A, l, rs);
+        const float n = ((o[l] + (dvydx + dvxdy) * z.phi_s - o[l] * z.h_s) * z.m_s);
+        sum_xy += o[l] + n;
+        O.r[2][l * nc + id] = n;
+      }
+    } else for (int l = 0; l < L; l++) {
       const ZCell z = zener_cell(A, l, rs);
       const float o = I.r[2][l * nc + id];
       const float n = ((o + (dvydx + dvxdy) * z.phi_s - o * z.h_s) * z.m_s);
@@ -298,9 +326,9 @@ __device__ __forceinline__ void record_traces(const DiffArgs &A) {
   }
 }
 
-template <int ORD>
+template <int ORD, int LC>
 __global__ __launch_bounds__(BX * BY) void k_dstress(DiffArgs A) {
-  dstress_cell<ORD>(A, blockIdx.y * BX + threadIdx.y, blockIdx.x * BY + threadIdx.x);
+  dstress_cell<ORD, LC>(A, blockIdx.y * BX + threadIdx.y, blockIdx.x * BY + threadIdx.x);
 }
 
 template <int ORD>
@@ -396,7 +424,7 @@ __global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
   B.vy[id] = inY ? vyb : 0.f;
 }
 
-template <int ORD>
+template <int ORD, int LC>
 __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
   const int x = blockIdx.y * BX + threadIdx.y, y = blockIdx.x * BY + threadIdx.x;
   const int nx = A.nx, ny = A.ny;
@@ -445,15 +473,24 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
     const float Sxxb = 0.5f * lpm * dt * (P + Q), Syyb = mu * dt * (P - Q);
     const float Xp = e1 + e2, Xs = 0.5f * (e1 - e2);
     float sum_xx = 0.f, sum_yy = 0.f;
-    for (int l = 0; l < L; l++) {
+    float pxx[LC > 0 ? LC : 1], pyy[LC > 0 ? LC : 1], bxx[LC > 0 ? LC : 1], byy[LC > 0 ? LC : 1];
+    if constexpr (LC > 0) {
+#pragma unroll
+      for (int l = 0; l < LC; l++) {
+        pxx[l] = I.r[0][l * nc + id]; pyy[l] = I.r[1][l * nc + id];
+        bxx[l] = B.r[0][l * nc + id]; byy[l] = B.r[1][l * nc + id];
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < (LC > 0 ? LC : L); l++) {
       const ZCell z = zener_cell(A, l, rs);
-      const float oxx = I.r[0][l * nc + id], oyy = I.r[1][l * nc + id];
+      const float oxx = LC > 0 ? pxx[l] : I.r[0][l * nc + id], oyy = LC > 0 ? pyy[l] : I.r[1][l * nc + id];
       const float nxx = ((oxx + Xp * z.phi_p - oxx * z.h_p) * z.m_p);
       const float nyy = ((oyy + 0.5f * (e1 - e2) * z.phi_s - oyy * z.h_s) * z.m_s);
       sum_xx += oxx + nxx;
       sum_yy += oyy + nyy;
-      const float Rxx = B.r[0][l * nc + id] + Sxxb;      // adjoint of r'_xx
-      const float Ryy = B.r[1][l * nc + id] + Syyb;
+      const float Rxx = (LC > 0 ? bxx[l] : B.r[0][l * nc + id]) + Sxxb;      // adjoint of r'_xx
+      const float Ryy = (LC > 0 ? byy[l] : B.r[1][l * nc + id]) + Syyb;
       B.r[0][l * nc + id] = Sxxb + Rxx * ((1.0f - z.h_p) * z.m_p);
       B.r[1][l * nc + id] = Syyb + Ryy * ((1.0f - z.h_s) * z.m_s);
       e1b += Rxx * z.phi_p * z.m_p + 0.5f * Ryy * z.phi_s * z.m_s;
@@ -486,12 +523,18 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
     float eb = mu * dt * R;
     const float Sb = 0.5f * mu * dt * R;
     float sum_xy = 0.f;
-    for (int l = 0; l < L; l++) {
+    float po[LC > 0 ? LC : 1], pb[LC > 0 ? LC : 1];
+    if constexpr (LC > 0) {
+#pragma unroll
+      for (int l = 0; l < LC; l++) { po[l] = I.r[2][l * nc + id]; pb[l] = B.r[2][l * nc + id]; }
+    }
+#pragma unroll
+    for (int l = 0; l < (LC > 0 ? LC : L); l++) {
       const ZCell z = zener_cell(A, l, rs);
-      const float o = I.r[2][l * nc + id];
+      const float o = LC > 0 ? po[l] : I.r[2][l * nc + id];
       const float n = ((o + X * z.phi_s - o * z.h_s) * z.m_s);
       sum_xy += o + n;
-      const float Rl = B.r[2][l * nc + id] + Sb;
+      const float Rl = (LC > 0 ? pb[l] : B.r[2][l * nc + id]) + Sb;
       B.r[2][l * nc + id] = Sb + Rl * ((1.0f - z.h_s) * z.m_s);
       eb += Rl * z.phi_s * z.m_s;
       gs += (double)Rl * ((double)z.m_s * rs) * ((double)o * z.h_s - (double)X * z.phi_s + (double)n * z.h_s);
@@ -547,7 +590,9 @@ dim3 grid_of(const DiffArgs &A) { return dim3((A.ny + BY - 1) / BY, (A.nx + BX -
 }  // namespace
 
 hipError_t vse_launch_dstress(const DiffArgs &A, hipStream_t st) {
-#define CALL(O) hipLaunchKernelGGL((k_dstress<O>), grid_of(A), dim3(BY, BX), 0, st, A)
+#define CALL(O)                                                                                 \
+  if (A.L == 3) hipLaunchKernelGGL((k_dstress<O, 3>), grid_of(A), dim3(BY, BX), 0, st, A);      \
+  else hipLaunchKernelGGL((k_dstress<O, 0>), grid_of(A), dim3(BY, BX), 0, st, A)
   VSE_ORD_SWITCH(A.ord, CALL)
 #undef CALL
   return hipGetLastError();
@@ -568,7 +613,9 @@ hipError_t vse_launch_adj_velocity(const DiffArgs &A, hipStream_t st) {
 }
 
 hipError_t vse_launch_adj_stress(const DiffArgs &A, hipStream_t st) {
-#define CALL(O) hipLaunchKernelGGL((k_adj_stress<O>), grid_of(A), dim3(BY, BX), 0, st, A)
+#define CALL(O)                                                                                 \
+  if (A.L == 3) hipLaunchKernelGGL((k_adj_stress<O, 3>), grid_of(A), dim3(BY, BX), 0, st, A);   \
+  else hipLaunchKernelGGL((k_adj_stress<O, 0>), grid_of(A), dim3(BY, BX), 0, st, A)
   VSE_ORD_SWITCH(A.ord, CALL)
 #undef CALL
   return hipGetLastError();

@@ -642,6 +642,7 @@ static int build_group(vse_batch *b, BatchGroup &g, int first, int n) {
   g.B.args = b->d_args + first;
   g.B.sparse = b->d_sparse + first;
   g.B.n = n;
+  g.B.L = b->sims[0]->d.n_sls;
   g.B.rows = b->rows;
   long tiles = 0;
   for (int i = 0; i < n; i++) {
@@ -804,10 +805,13 @@ int vse_batch_create(vse_sim *const *sims, int n, vse_batch **out) {
   *out = nullptr;
   vse_sim *s0 = sims[0];
   if (!s0) return VSE_ERR_ARG;
+  // Taichi-variant batches are switched off: a process ended in SIGABRT while
+  // tearing down the solo runs that followed one, and no cause has been shown.
+  if (s0->d.variant != VSE_VARIANT_WGSL) return s0->fail(VSE_ERR_ARG, "batches take WGSL-variant simulations only");
   for (int i = 0; i < n; i++) {
     vse_sim *s = sims[i];
     if (!s || s->d.ndim != 2 || s->d.variant != s0->d.variant || s->d.visco != s0->d.visco ||
-        s->d.ord != s0->d.ord || s->device != s0->device)
+        s->d.ord != s0->d.ord || s->d.n_sls != s0->d.n_sls || s->device != s0->device)
       return VSE_ERR_ARG;   // members: 2D, one device, one kernel instantiation
     for (int j = 0; j < i; j++)
       if (sims[j] == s) return VSE_ERR_ARG;

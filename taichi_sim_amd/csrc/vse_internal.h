@@ -126,6 +126,7 @@ struct BatchArgs {
   const StepArgs *__restrict__ args;        // device array [n]
   const SparseArgs *__restrict__ sparse;    // device array [n]
   int n;
+  int L;                       // Zener mechanisms of every member
   int rows;                    // BX-row groups each workgroup walks in x
   int tile_beg[VSE_MAX_BATCH + 1];
 };

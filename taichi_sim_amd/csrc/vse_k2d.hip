@@ -12,7 +12,11 @@ namespace {
 
 constexpr int BY = 64, BX = 4;
 
-template <int VAR, bool VISCO, int ORD>
+// LC > 0: the mechanism count is the compile-time LC and every memory-variable
+// load of a point is issued before its arithmetic (same operations, same
+// order: bit-identical to the runtime-L loop, which waits on each
+// mechanism's loads in turn).
+template <int VAR, bool VISCO, int ORD, int LC = 0>
 __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
   const int nx = A.nx, ny = A.ny;
   if (x >= nx || y >= ny) return;
@@ -46,7 +50,28 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
       A.syy[id] += (dl + 2.0f * mu * dvy) * dt;
     } else {
       const float lp2m = lambda + 2.0f * mu;
-      if constexpr (VISCO) {
+      if constexpr (VISCO && LC > 0) {
+        const float lpm = lambda + mu;
+        float oxx[LC], oyy[LC];
+#pragma unroll
+        for (int l = 0; l < LC; l++) {
+          oxx[l] = A.r[0][l * A.ncell + id];
+          oyy[l] = A.r[1][l * A.ncell + id];
+        }
+        const float sxx0 = A.sxx[id], syy0 = A.syy[id];
+        float sum_xx = 0.f, sum_yy = 0.f;
+#pragma unroll
+        for (int l = 0; l < LC; l++) {
+          const float nxx = ((oxx[l] + (dvx + dvy) * A.zk.phi_p[l] - oxx[l] * A.zk.h_p[l]) * A.zk.m_p[l]);
+          const float nyy = ((oyy[l] + 0.5f * (dvx - dvy) * A.zk.phi_s[l] - oyy[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
+          sum_xx += oxx[l] + nxx;
+          sum_yy += oyy[l] + nyy;
+          A.r[0][l * A.ncell + id] = nxx;
+          A.r[1][l * A.ncell + id] = nyy;
+        }
+        A.sxx[id] = sxx0 + (lp2m * dvx + lambda * dvy + 0.5f * lpm * sum_xx + mu * sum_yy) * dt;
+        A.syy[id] = syy0 + (lambda * dvx + lp2m * dvy + 0.5f * lpm * sum_xx - mu * sum_yy) * dt;
+      } else if constexpr (VISCO) {
         const float lpm = lambda + mu;
         float sum_xx = 0.f, sum_yy = 0.f;
         for (int l = 0; l < A.L; l++) {
@@ -85,7 +110,20 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
     const float csa = A.cs[id + 1], csb = A.cs[id];
     const float cs_h = (fminf(csa, csb) == 0.f) ? 0.f : 0.5f * (csa + csb);
     const float mu = rho_h * (cs_h * cs_h);
-    if constexpr (VISCO && VAR == VSE_VAR_WGSL) {
+    if constexpr (VISCO && VAR == VSE_VAR_WGSL && LC > 0) {
+      float o[LC];
+#pragma unroll
+      for (int l = 0; l < LC; l++) o[l] = A.r[3][l * A.ncell + id];
+      const float sxy0 = A.sxy[id];
+      float sum_xy = 0.f;
+#pragma unroll
+      for (int l = 0; l < LC; l++) {
+        const float n = ((o[l] + (dvydx + dvxdy) * A.zk.phi_s[l] - o[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
+        sum_xy += o[l] + n;
+        A.r[3][l * A.ncell + id] = n;
+      }
+      A.sxy[id] = sxy0 + (mu * (dvydx + dvxdy) + 0.5f * mu * sum_xy) * dt;
+    } else if constexpr (VISCO && VAR == VSE_VAR_WGSL) {
       float sum_xy = 0.f;
       for (int l = 0; l < A.L; l++) {
         float *pxy = A.r[3] + l * A.ncell + id;
@@ -170,9 +208,9 @@ __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, in
   return v2bits;
 }
 
-template <int VAR, bool VISCO, int ORD>
+template <int VAR, bool VISCO, int ORD, int LC = 0>
 __global__ __launch_bounds__(BX * BY) void k_stress2d(StepArgs A) {
-  stress2d_cell<VAR, VISCO, ORD>(A, blockIdx.y * BX + threadIdx.y, blockIdx.x * BY + threadIdx.x);
+  stress2d_cell<VAR, VISCO, ORD, LC>(A, blockIdx.y * BX + threadIdx.y, blockIdx.x * BY + threadIdx.x);
 }
 
 template <int VAR, bool VISCO, int ORD>
@@ -196,7 +234,7 @@ __device__ __forceinline__ int batch_instance(const BatchArgs &B, int b) {
   return lo;
 }
 
-template <int VAR, bool VISCO, int ORD>
+template <int VAR, bool VISCO, int ORD, int LC = 0>
 __global__ __launch_bounds__(BX * BY) void k_stress2d_batch(BatchArgs B) {
   const int i = batch_instance(B, blockIdx.x);
   const StepArgs &A = B.args[i];
@@ -204,7 +242,7 @@ __global__ __launch_bounds__(BX * BY) void k_stress2d_batch(BatchArgs B) {
   const int nyb = (A.ny + BY - 1) / BY;
   const int y = (t % nyb) * BY + threadIdx.x;
   const int x0 = (t / nyb) * BX * B.rows + threadIdx.y;
-  for (int k = 0; k < B.rows; k++) stress2d_cell<VAR, VISCO, ORD>(A, x0 + k * BX, y);
+  for (int k = 0; k < B.rows; k++) stress2d_cell<VAR, VISCO, ORD, LC>(A, x0 + k * BX, y);
 }
 
 template <int VAR, bool VISCO, int ORD>
@@ -227,9 +265,11 @@ template <int VAR, bool VISCO>
 hipError_t launch_ord(bool stress, int ord, const StepArgs &A, hipStream_t st) {
   dim3 block(BY, BX);
   dim3 grid((A.ny + BY - 1) / BY, (A.nx + BX - 1) / BX);
+  const bool l3 = VISCO && VAR == VSE_VAR_WGSL && A.L == 3;
 #define VSE_CASE(O)                                                                              \
   case O:                                                                                        \
-    if (stress) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O>), grid, block, 0, st, A);          \
+    if (stress && l3) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O, 3>), grid, block, 0, st, A); \
+    else if (stress) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O>), grid, block, 0, st, A);     \
     else hipLaunchKernelGGL((k_velocity2d<VAR, VISCO, O>), grid, block, 0, st, A);               \
     break;
   switch (ord) {
@@ -243,9 +283,11 @@ hipError_t launch_ord(bool stress, int ord, const StepArgs &A, hipStream_t st) {
 template <int VAR, bool VISCO>
 hipError_t launch_batch_ord(bool stress, int ord, const BatchArgs &B, int total_tiles, hipStream_t st) {
   dim3 block(BY, BX), grid(total_tiles);
+  const bool l3 = VISCO && VAR == VSE_VAR_WGSL && B.L == 3;
 #define VSE_CASE(O)                                                                              \
   case O:                                                                                        \
-    if (stress) hipLaunchKernelGGL((k_stress2d_batch<VAR, VISCO, O>), grid, block, 0, st, B);    \
+    if (stress && l3) hipLaunchKernelGGL((k_stress2d_batch<VAR, VISCO, O, 3>), grid, block, 0, st, B); \
+    else if (stress) hipLaunchKernelGGL((k_stress2d_batch<VAR, VISCO, O>), grid, block, 0, st, B); \
     else hipLaunchKernelGGL((k_velocity2d_batch<VAR, VISCO, O>), grid, block, 0, st, B);         \
     break;
   switch (ord) {

@@ -313,8 +313,8 @@ class VSEBatch:
         h = ctypes.c_void_p()
         rc = self._lib.vse_batch_create(arr, len(self.sims), ctypes.byref(h))
         if rc != 0 or not h.value:
-            raise _lib.VSEError(rc, "vse_batch_create failed: members must be open 2D simulations on one device "
-                                    "with the same variant, viscoelastic flag and FD order")
+            raise _lib.VSEError(rc, "vse_batch_create failed: members must be open 2D WGSL-variant simulations on "
+                                    "one device with the same viscoelastic flag, FD order and mechanism count")
         self._h = h
 
     def run(self, n_steps, check_every=100, timed=False):

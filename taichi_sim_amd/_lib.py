@@ -2,12 +2,43 @@
 
 The library is built in-tree (``taichi_sim_amd/libvse.so``).  There is no
 fallback: if it is missing or cannot be loaded, every entry point raises.
+
+HIP runtime: libvse.so needs libamdhip64.so.7 and libhsa-runtime64.so.1.  The
+dynamic linker resolves those names to whichever copy the process loaded
+first, and the torch wheel ships its own copies (an older ROCm release) under
+the same names.  Load this library before importing torch to run it on the
+ROCm runtime it was compiled with; ``runtime_path()`` names the copy in use
+and ``load()`` warns when it lies outside that ROCm installation.
 """
 import ctypes
 import os
+import warnings
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvse.so")
+ROCM_PATH = os.path.realpath(os.environ.get("ROCM_PATH", "/opt/rocm"))
+
+
+class _DlInfo(ctypes.Structure):
+    _fields_ = [("dli_fname", ctypes.c_char_p), ("dli_fbase", ctypes.c_void_p),
+                ("dli_sname", ctypes.c_char_p), ("dli_saddr", ctypes.c_void_p)]
+
+
+def runtime_path(lib=None):
+    """File of the HIP runtime libvse's own calls bind to (dladdr of hipMalloc
+    as resolved from the library's handle), or None if it cannot be told."""
+    lib = lib if lib is not None else load()
+    try:
+        addr = ctypes.cast(lib.hipMalloc, ctypes.c_void_p).value
+        info = _DlInfo()
+        dladdr = ctypes.CDLL(None).dladdr
+        dladdr.argtypes = [ctypes.c_void_p, ctypes.POINTER(_DlInfo)]
+        if not addr or dladdr(addr, ctypes.byref(info)) == 0 or not info.dli_fname:
+            return None
+        return os.path.realpath(info.dli_fname.decode())
+    except (AttributeError, OSError):
+        return None
+
 
 VSE_OK = 0
 VSE_ERR_ARG = -1
@@ -110,5 +141,10 @@ def load():
         f.argtypes = args
     if lib.vse_abi_version() != 1:
         raise ImportError("libvse ABI version mismatch")
+    rt = runtime_path(lib)
+    if rt is not None and not rt.startswith(ROCM_PATH + os.sep):
+        warnings.warn(f"libvse is bound to the HIP runtime {rt}, loaded earlier by another package, not the one "
+                      f"under {ROCM_PATH} it was built with; import taichi_sim_amd before torch to use the latter",
+                      RuntimeWarning, stacklevel=2)
     _lib = lib
     return lib

@@ -19,10 +19,16 @@ without a round trip through the host.
 import ctypes
 
 import numpy as np
-import torch
 
 from . import _lib
-from .engine import VSESim
+
+try:        # bind libvse to the ROCm runtime it was built with before torch loads its own copy
+    _lib.load()
+except ImportError:
+    pass    # not built: VSEGrad raises when used
+import torch  # noqa: E402
+
+from .engine import VSESim  # noqa: E402
 from .sim_support import StabilityError
 
 GRAD_NAMES = ("lambda", "mu", "mu_xy", "rho_vx", "rho_vy", "tau")

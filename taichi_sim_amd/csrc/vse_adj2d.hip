@@ -31,8 +31,11 @@
 // is two launches in either direction.
 // The forward values the adjoint needs (raw derivatives, CPML-stretched
 // derivatives, memory variables) are recomputed from the stored levels n
-// and n+1 with the forward's own FP32 arithmetic.  Gradients accumulate in
-// FP64.
+// and n+1 with the forward's own FP32 arithmetic.  Each step's gradient
+// terms (formed in FP64) are added to FP32 per-cell partials; k_grad_flush
+// adds the partials into the FP64 gradients after every step n divisible by
+// VSE_GRAD_FLUSH, so the time accumulation is FP64 at that granularity while
+// every step moves 4-byte partials instead of 8-byte totals.
 #include "vse_device.h"
 
 namespace {
@@ -82,32 +85,21 @@ __device__ __forceinline__ int owned_band(const AxisProf &P, int c, int off) {
   return band_index(P, j);
 }
 
-// CPML of the out-of-place forward: psi' = b psi + a D into pout, D/k + psi'.
-__device__ __forceinline__ float cpml_fwd(const VSE_G float *a, const VSE_G float *b, const VSE_G float *k,
-                                          const VSE_G float *pin, VSE_G float *pout, long pidx, int j, bool band,
-                                          float D) {
-  if (!band) return D;
-  const float pn = b[j] * pin[pidx] + a[j] * D;
-  pout[pidx] = pn;
+// CPML stage of a forward pass at profile index j: psi' = b psi + a D from
+// psi of level n (q); returns D/k + psi' and leaves psi' in pn.
+__device__ __forceinline__ float cpml_step(const VSE_G float *a, const VSE_G float *b, const VSE_G float *k, int j,
+                                           float q, float D, float &pn) {
+  pn = b[j] * q + a[j] * D;
   return D / k[j] + pn;
 }
 
-// The same stretched derivative, recomputed without writing.
-__device__ __forceinline__ float cpml_val(const VSE_G float *a, const VSE_G float *b, const VSE_G float *k,
-                                          const VSE_G float *pin, long pidx, int j, bool band, float D) {
-  if (!band) return D;
-  const float pn = b[j] * pin[pidx] + a[j] * D;
-  return D / k[j] + pn;
-}
-
-// Adjoint of the CPML stage: given the adjoint eb of the stretched derivative
-// and psib = adjoint of psi', returns the adjoint of the raw derivative and
-// leaves the adjoint of psi in psib.
-__device__ __forceinline__ float cpml_adj(const VSE_G float *a, const VSE_G float *b, const VSE_G float *k,
-                                          VSE_G float *psib, long pidx, int j, bool band, float eb) {
-  if (!band) return eb;
-  const float pt = psib[pidx] + eb;
-  psib[pidx] = b[j] * pt;
+// Adjoint of that stage: given the adjoint eb of the stretched derivative and
+// qb = adjoint of psi', returns the adjoint of the raw derivative and leaves
+// the adjoint of psi in qn.
+__device__ __forceinline__ float cpml_back(const VSE_G float *a, const VSE_G float *b, const VSE_G float *k, int j,
+                                           float qb, float eb, float &qn) {
+  const float pt = qb + eb;
+  qn = b[j] * pt;
   return eb / k[j] + a[j] * pt;
 }
 
@@ -115,32 +107,73 @@ __device__ __forceinline__ float ld(const VSE_G float *f, int x, int y, int nx, 
   return (x >= 0 && x < nx && y >= 0 && y < ny) ? f[(long)x * ny + y] : 0.f;
 }
 
-// Transposed staggered differences.  Forward D+ f(i) = sum C (f(i+c+1) - f(i-c)) / d
-// has transpose  g(p) = sum C (h(p-c-1) - h(p+c)) / d;  D- f(i) = sum C (f(i+c) - f(i-c-1)) / d
-// has transpose  g(p) = sum C (h(p-c) - h(p+c+1)) / d.
-template <int ORD, bool PLUS, int AX>
-__device__ __forceinline__ float dT(const DiffArgs &A, const VSE_G float *h, int x, int y) {
-  const float d = AX == 0 ? A.dx : A.dy;
-  float acc = 0.f;
+// Operands of one staggered difference, loaded before any arithmetic so that
+// a cell's loads are all in flight together; the sums below then run the
+// same operations in the same order as summing while loading.
+template <int ORD>
+struct Taps {
+  float a[ORD], b[ORD];
+};
+
+// Along stride s: D+ has a[c] = f(i+c+1), b[c] = f(i-c); D- has a[c] = f(i+c), b[c] = f(i-c-1).
+template <int ORD, bool PLUS>
+__device__ __forceinline__ Taps<ORD> taps(const VSE_G float *f, long id, long s) {
+  Taps<ORD> t;
 #pragma unroll
   for (int c = 0; c < ORD; c++) {
-    const int o1 = PLUS ? -c - 1 : -c, o2 = PLUS ? c : c + 1;
-    const float h1 = AX == 0 ? ld(h, x + o1, y, A.nx, A.ny) : ld(h, x, y + o1, A.nx, A.ny);
-    const float h2 = AX == 0 ? ld(h, x + o2, y, A.nx, A.ny) : ld(h, x, y + o2, A.nx, A.ny);
-    acc += A.C[c] * (h1 - h2) / d;
+    t.a[c] = PLUS ? f[id + (long)(c + 1) * s] : f[id + (long)c * s];
+    t.b[c] = PLUS ? f[id - (long)c * s] : f[id - (long)(c + 1) * s];
   }
+  return t;
+}
+
+// sum_c C_c (a_c - b_c) / d, the reference's derivative (wgsl2d :846-851)
+template <int ORD>
+__device__ __forceinline__ float diff(const DiffArgs &A, const Taps<ORD> &t, float d) {
+  float acc = 0.f;
+#pragma unroll
+  for (int c = 0; c < ORD; c++) acc += fd_term<VSE_VAR_WGSL>(A.C[c], t.a[c], t.b[c], d, 0.f);
   return acc;
 }
 
+// Transposed staggered differences.  Forward D+ f(i) = sum C (f(i+c+1) - f(i-c)) / d
+// has transpose  g(p) = sum C (h(p-c-1) - h(p+c)) / d;  D- f(i) = sum C (f(i+c) - f(i-c-1)) / d
+// has transpose  g(p) = sum C (h(p-c) - h(p+c+1)) / d.  Operands outside the grid are 0.
+template <int ORD, bool PLUS, int AX>
+__device__ __forceinline__ Taps<ORD> tapsT(const DiffArgs &A, const VSE_G float *h, int x, int y) {
+  Taps<ORD> t;
+#pragma unroll
+  for (int c = 0; c < ORD; c++) {
+    const int o1 = PLUS ? -c - 1 : -c, o2 = PLUS ? c : c + 1;
+    t.a[c] = AX == 0 ? ld(h, x + o1, y, A.nx, A.ny) : ld(h, x, y + o1, A.nx, A.ny);
+    t.b[c] = AX == 0 ? ld(h, x + o2, y, A.nx, A.ny) : ld(h, x, y + o2, A.nx, A.ny);
+  }
+  return t;
+}
+
+template <int ORD, int AX>
+__device__ __forceinline__ float sumT(const DiffArgs &A, const Taps<ORD> &t) {
+  const float d = AX == 0 ? A.dx : A.dy;
+  float acc = 0.f;
+#pragma unroll
+  for (int c = 0; c < ORD; c++) acc += A.C[c] * (t.a[c] - t.b[c]) / d;
+  return acc;
+}
+
+// Each kernel below issues every load of its cell before its first store.
+// None of them reads, at another cell, anything it writes, and it reads its
+// own cell's values before overwriting them, so the order of loads and stores
+// changes no result; issuing the loads together keeps more of them in flight.
+// LC > 0 is the mechanism count fixed at compile time, which lets the memory
+// variables join the early loads; LC = 0 loops over A.L mechanisms.
+
 // ---------------------------------------------------------------- forward
-// LC > 0: compile-time mechanism count; every memory-variable load of a point
-// is issued before its arithmetic (same operations and order as the runtime
-// loop, so bit-identical).
 template <int ORD, int LC = 0>
 __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
   const int nx = A.nx, ny = A.ny;
   if (x >= nx || y >= ny) return;
   constexpr int last = ORD - 1, off = ORD - 1;
+  constexpr int LR = LC > 0 ? LC : 1;
   const long id = (long)x * ny + y;
   const long nc = A.ncell;
   const float dt = A.dt;
@@ -148,51 +181,80 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
   const int L = A.L;
   const int ex = owned_band(A.ax[0], x, off), ey = owned_band(A.ax[1], y, off);
   const long px = (long)ex * ny + y, py = (long)x * A.ax[1].nb + ey;
+  const int jx = x - off, jy = y - off;
+  const bool inN = x >= last && x < nx - last - 1 && y >= last + 1 && y < ny - last;   // wgsl2d :842-912
+  const bool inS = x >= last + 1 && x < nx - last && y >= last && y < ny - last - 1;   // wgsl2d :927-975
 
-  // ---- normal stresses (wgsl2d :842-912)
-  const bool inN = x >= last && x < nx - last - 1 && y >= last + 1 && y < ny - last;
-  float sxx = I.sxx[id], syy = I.syy[id];
+  // ---- loads
+  float sxx = I.sxx[id], syy = I.syy[id], sxy = I.sxy[id];
+  Taps<ORD> tvx, tvy, tyx, txy;
+  float qxx = 0.f, qyy = 0.f, qyx = 0.f, qxy = 0.f;
+  float rho0 = 0.f, rhoN = 0.f, rhoS = 0.f, cp0 = 0.f, cpN = 0.f, cs0 = 0.f, csN = 0.f, csS = 0.f, ts = 1.f;
+  float oxx[LR], oyy[LR], oxy[LR];
+  if (inN || inS) {
+    rho0 = A.rho[id];
+    cs0 = A.cs[id];
+    ts = A.ts[id];
+  }
   if (inN) {
-    float dvx = 0.f, dvy = 0.f;
+    tvx = taps<ORD, true>(I.vx, id, ny);
+    tvy = taps<ORD, false>(I.vy, id, 1);
+    if (ex >= 0) qxx = I.psi[SL_VXX][px];
+    if (ey >= 0) qyy = I.psi[SL_VYY][py];
+    rhoN = A.rho[id + ny];
+    cpN = A.cp[id + ny];
+    cp0 = A.cp[id];
+    csN = A.cs[id + ny];
+    if constexpr (LC > 0) {
 #pragma unroll
-    for (int c = 0; c < ORD; c++) {
-      dvx += fd_term<VSE_VAR_WGSL>(A.C[c], I.vx[id + (long)(c + 1) * ny], I.vx[id - (long)c * ny], A.dx, 0.f);
-      dvy += fd_term<VSE_VAR_WGSL>(A.C[c], I.vy[id + c], I.vy[id - c - 1], A.dy, 0.f);
+      for (int l = 0; l < LC; l++) {
+        oxx[l] = I.r[0][l * nc + id];
+        oyy[l] = I.r[1][l * nc + id];
+      }
     }
-    const int jx = x - off, jy = y - off;
-    dvx = cpml_fwd(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, I.psi[SL_VXX], O.psi[SL_VXX], px, jx, ex >= 0, dvx);
-    dvy = cpml_fwd(A.ax[1].a, A.ax[1].b, A.ax[1].k, I.psi[SL_VYY], O.psi[SL_VYY], py, jy, ey >= 0, dvy);
-    const float rho_h = 0.5f * (A.rho[id + ny] + A.rho[id]);
-    const float cp_h = 0.5f * (A.cp[id + ny] + A.cp[id]);
-    const float csa = A.cs[id + ny], csb = A.cs[id];
-    const float cs_l = 0.5f * (csa + csb);
-    const float cs_m = (fminf(csa, csb) == 0.f) ? 0.f : cs_l;
+  }
+  if (inS) {
+    tyx = taps<ORD, false>(I.vy, id, ny);
+    txy = taps<ORD, true>(I.vx, id, 1);
+    if (ex >= 0) qyx = I.psi[SL_VYX][px];
+    if (ey >= 0) qxy = I.psi[SL_VXY][py];
+    rhoS = A.rho[id + 1];
+    csS = A.cs[id + 1];
+    if constexpr (LC > 0) {
+#pragma unroll
+      for (int l = 0; l < LC; l++) oxy[l] = I.r[2][l * nc + id];
+    }
+  }
+  const float rs = 1.0f / ts;
+
+  // ---- normal stresses
+  if (inN) {
+    float pn;
+    float dvx = diff<ORD>(A, tvx, A.dx), dvy = diff<ORD>(A, tvy, A.dy);
+    if (ex >= 0) {
+      dvx = cpml_step(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, jx, qxx, dvx, pn);
+      O.psi[SL_VXX][px] = pn;
+    }
+    if (ey >= 0) {
+      dvy = cpml_step(A.ax[1].a, A.ax[1].b, A.ax[1].k, jy, qyy, dvy, pn);
+      O.psi[SL_VYY][py] = pn;
+    }
+    const float rho_h = 0.5f * (rhoN + rho0);
+    const float cp_h = 0.5f * (cpN + cp0);
+    const float cs_l = 0.5f * (csN + cs0);
+    const float cs_m = (fminf(csN, cs0) == 0.f) ? 0.f : cs_l;
     const float lambda = rho_h * (cp_h * cp_h - 2.0f * cs_l * cs_l);
     const float mu = rho_h * (cs_m * cs_m);
     const float lp2m = lambda + 2.0f * mu, lpm = lambda + mu;
-    const float rs = 1.0f / A.ts[id];
     float sum_xx = 0.f, sum_yy = 0.f;
-    if constexpr (LC > 0) {
-      float oxx[LC], oyy[LC];
 #pragma unroll
-      for (int l = 0; l < LC; l++) { oxx[l] = I.r[0][l * nc + id]; oyy[l] = I.r[1][l * nc + id]; }
-#pragma unroll
-      for (int l = 0; l < LC; l++) {
-        const ZCell z = zener_cell(A, l, rs);
-        const float nxx = ((oxx[l] + (dvx + dvy) * z.phi_p - oxx[l] * z.h_p) * z.m_p);
-        const float nyy = ((oyy[l] + 0.5f * (dvx - dvy) * z.phi_s - oyy[l] * z.h_s) * z.m_s);
-        sum_xx += oxx[l] + nxx;
-        sum_yy += oyy[l] + nyy;
-        O.r[0][l * nc + id] = nxx;
-        O.r[1][l * nc + id] = nyy;
-      }
-    } else for (int l = 0; l < L; l++) {
+    for (int l = 0; l < (LC > 0 ? LC : L); l++) {
       const ZCell z = zener_cell(A, l, rs);
-      const float oxx = I.r[0][l * nc + id], oyy = I.r[1][l * nc + id];
-      const float nxx = ((oxx + (dvx + dvy) * z.phi_p - oxx * z.h_p) * z.m_p);
-      const float nyy = ((oyy + 0.5f * (dvx - dvy) * z.phi_s - oyy * z.h_s) * z.m_s);
-      sum_xx += oxx + nxx;
-      sum_yy += oyy + nyy;
+      const float ox = LC > 0 ? oxx[l] : I.r[0][l * nc + id], oy = LC > 0 ? oyy[l] : I.r[1][l * nc + id];
+      const float nxx = ((ox + (dvx + dvy) * z.phi_p - ox * z.h_p) * z.m_p);
+      const float nyy = ((oy + 0.5f * (dvx - dvy) * z.phi_s - oy * z.h_s) * z.m_s);
+      sum_xx += ox + nxx;
+      sum_yy += oy + nyy;
       O.r[0][l * nc + id] = nxx;
       O.r[1][l * nc + id] = nyy;
     }
@@ -206,42 +268,27 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
     if (ex >= 0) O.psi[SL_VXX][px] = I.psi[SL_VXX][px];
     if (ey >= 0) O.psi[SL_VYY][py] = I.psi[SL_VYY][py];
   }
-  O.sxx[id] = sxx;
-  O.syy[id] = syy;
 
-  // ---- shear stress (wgsl2d :927-975)
-  const bool inS = x >= last + 1 && x < nx - last && y >= last && y < ny - last - 1;
-  float sxy = I.sxy[id];
+  // ---- shear stress
   if (inS) {
-    float dvydx = 0.f, dvxdy = 0.f;
-#pragma unroll
-    for (int c = 0; c < ORD; c++) {
-      dvydx += fd_term<VSE_VAR_WGSL>(A.C[c], I.vy[id + (long)c * ny], I.vy[id - (long)(c + 1) * ny], A.dx, 0.f);
-      dvxdy += fd_term<VSE_VAR_WGSL>(A.C[c], I.vx[id + c + 1], I.vx[id - c], A.dy, 0.f);
+    float pn;
+    float dvydx = diff<ORD>(A, tyx, A.dx), dvxdy = diff<ORD>(A, txy, A.dy);
+    if (ex >= 0) {
+      dvydx = cpml_step(A.ax[0].a, A.ax[0].b, A.ax[0].k, jx, qyx, dvydx, pn);
+      O.psi[SL_VYX][px] = pn;
     }
-    const int jx = x - off, jy = y - off;
-    dvydx = cpml_fwd(A.ax[0].a, A.ax[0].b, A.ax[0].k, I.psi[SL_VYX], O.psi[SL_VYX], px, jx, ex >= 0, dvydx);
-    dvxdy = cpml_fwd(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, I.psi[SL_VXY], O.psi[SL_VXY], py, jy, ey >= 0, dvxdy);
-    const float rho_h = 0.5f * (A.rho[id + 1] + A.rho[id]);
-    const float csa = A.cs[id + 1], csb = A.cs[id];
-    const float cs_h = (fminf(csa, csb) == 0.f) ? 0.f : 0.5f * (csa + csb);
+    if (ey >= 0) {
+      dvxdy = cpml_step(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, jy, qxy, dvxdy, pn);
+      O.psi[SL_VXY][py] = pn;
+    }
+    const float rho_h = 0.5f * (rhoS + rho0);
+    const float cs_h = (fminf(csS, cs0) == 0.f) ? 0.f : 0.5f * (csS + cs0);
     const float mu = rho_h * (cs_h * cs_h);
-    const float rs = 1.0f / A.ts[id];
     float sum_xy = 0.f;
-    if constexpr (LC > 0) {
-      float o[LC];
 #pragma unroll
-      for (int l = 0; l < LC; l++) o[l] = I.r[2][l * nc + id];
-#pragma unroll
-      for (int l = 0; l < LC; l++) {
-        const ZCell z = zener_cell(A, l, rs);
-        const float n = ((o[l] + (dvydx + dvxdy) * z.phi_s - o[l] * z.h_s) * z.m_s);
-        sum_xy += o[l] + n;
-        O.r[2][l * nc + id] = n;
-      }
-    } else for (int l = 0; l < L; l++) {
+    for (int l = 0; l < (LC > 0 ? LC : L); l++) {
       const ZCell z = zener_cell(A, l, rs);
-      const float o = I.r[2][l * nc + id];
+      const float o = LC > 0 ? oxy[l] : I.r[2][l * nc + id];
       const float n = ((o + (dvydx + dvxdy) * z.phi_s - o * z.h_s) * z.m_s);
       sum_xy += o + n;
       O.r[2][l * nc + id] = n;
@@ -252,6 +299,8 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
     if (ex >= 0) O.psi[SL_VYX][px] = I.psi[SL_VYX][px];
     if (ey >= 0) O.psi[SL_VXY][py] = I.psi[SL_VXY][py];
   }
+  O.sxx[id] = sxx;
+  O.syy[id] = syy;
   O.sxy[id] = sxy;
 }
 
@@ -266,49 +315,77 @@ __device__ __forceinline__ unsigned dvelocity_cell(const DiffArgs &A, int x, int
   const DLevel &I = A.in, &O = A.out;
   const int ex = owned_band(A.ax[0], x, off), ey = owned_band(A.ax[1], y, off);
   const long px = (long)ex * ny + y, py = (long)x * A.ax[1].nb + ey;
-  const bool inX = x >= last + 1 && x < nx - last && y >= last + 1 && y < ny - last;
-  const bool inY = x >= last && x < nx - last - 1 && y >= last && y < ny - last - 1;
+  const bool inX = x >= last + 1 && x < nx - last && y >= last + 1 && y < ny - last;   // wgsl2d :998-1025
+  const bool inY = x >= last && x < nx - last - 1 && y >= last && y < ny - last - 1;   // wgsl2d :1027-1057
+
+  // ---- loads
   float vxn = I.vx[id], vyn = I.vy[id];
-  // vx (wgsl2d :998-1025)
+  Taps<ORD> t1, t2, t3, t4;
+  float q5 = 0.f, q6 = 0.f, q7 = 0.f, q8 = 0.f;
+  float rhoX = 0.f, r00 = 0.f, r10 = 0.f, r11 = 0.f, r01 = 0.f;
   if (inX) {
-    float d1 = 0.f, d2 = 0.f;
-#pragma unroll
-    for (int c = 0; c < ORD; c++) {
-      d1 += fd_term<VSE_VAR_WGSL>(A.C[c], O.sxx[id + (long)c * ny], O.sxx[id - (long)(c + 1) * ny], A.dx, 0.f);
-      d2 += fd_term<VSE_VAR_WGSL>(A.C[c], O.sxy[id + c], O.sxy[id - c - 1], A.dy, 0.f);
+    t1 = taps<ORD, false>(O.sxx, id, ny);
+    t2 = taps<ORD, false>(O.sxy, id, 1);
+    if (ex >= 0) q5 = I.psi[SL_SXXX][px];
+    if (ey >= 0) q6 = I.psi[SL_SXYY][py];
+    rhoX = A.rho[id];
+  }
+  if (inY) {
+    t3 = taps<ORD, true>(O.sxy, id, ny);
+    t4 = taps<ORD, true>(O.syy, id, 1);
+    if (ex >= 0) q7 = I.psi[SL_SXYX][px];
+    if (ey >= 0) q8 = I.psi[SL_SYYY][py];
+    r00 = A.rho[id];
+    r10 = A.rho[id + ny];
+    r11 = A.rho[id + ny + 1];
+    r01 = A.rho[id + 1];
+  }
+  // source after the update, anywhere (wgsl2d :1059-1065)
+  const int col = A.src_map[id];
+  const bool src_on = col >= 0 && A.n < A.src_rows;
+  float rho_src = 0.f, sval = 0.f;
+  if (src_on) {
+    rho_src = rho_vy_edge(A, x, y, id);
+    sval = A.src[(long)A.n * A.src_cols + col];
+  }
+
+  // ---- update
+  if (inX) {
+    float pn;
+    float dx1 = diff<ORD>(A, t1, A.dx), dx2 = diff<ORD>(A, t2, A.dy);
+    if (ex >= 0) {
+      dx1 = cpml_step(A.ax[0].a, A.ax[0].b, A.ax[0].k, x - off, q5, dx1, pn);
+      O.psi[SL_SXXX][px] = pn;
     }
-    d1 = cpml_fwd(A.ax[0].a, A.ax[0].b, A.ax[0].k, I.psi[SL_SXXX], O.psi[SL_SXXX], px, x - off, ex >= 0, d1);
-    d2 = cpml_fwd(A.ax[1].a, A.ax[1].b, A.ax[1].k, I.psi[SL_SXYY], O.psi[SL_SXYY], py, y - off, ey >= 0, d2);
-    const float rho = A.rho[id];
-    if (rho > 0.f) vxn = (d1 + d2) * dt / rho + vxn;
+    if (ey >= 0) {
+      dx2 = cpml_step(A.ax[1].a, A.ax[1].b, A.ax[1].k, y - off, q6, dx2, pn);
+      O.psi[SL_SXYY][py] = pn;
+    }
+    if (rhoX > 0.f) vxn = (dx1 + dx2) * dt / rhoX + vxn;
   } else {
     if (ex >= 0) O.psi[SL_SXXX][px] = I.psi[SL_SXXX][px];
     if (ey >= 0) O.psi[SL_SXYY][py] = I.psi[SL_SXYY][py];
   }
-  // vy (wgsl2d :1027-1057)
   if (inY) {
-    float d1 = 0.f, d2 = 0.f;
-#pragma unroll
-    for (int c = 0; c < ORD; c++) {
-      d1 += fd_term<VSE_VAR_WGSL>(A.C[c], O.sxy[id + (long)(c + 1) * ny], O.sxy[id - (long)c * ny], A.dx, 0.f);
-      d2 += fd_term<VSE_VAR_WGSL>(A.C[c], O.syy[id + c + 1], O.syy[id - c], A.dy, 0.f);
+    float pn;
+    float dy1 = diff<ORD>(A, t3, A.dx), dy2 = diff<ORD>(A, t4, A.dy);
+    const float rhoY = 0.25f * (r00 + r10 + r11 + r01);
+    if (ex >= 0) {
+      dy1 = cpml_step(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, x - off, q7, dy1, pn);
+      O.psi[SL_SXYX][px] = pn;
     }
-    d1 = cpml_fwd(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, I.psi[SL_SXYX], O.psi[SL_SXYX], px, x - off, ex >= 0, d1);
-    d2 = cpml_fwd(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, I.psi[SL_SYYY], O.psi[SL_SYYY], py, y - off, ey >= 0, d2);
-    const float rho = 0.25f * (A.rho[id] + A.rho[id + ny] + A.rho[id + ny + 1] + A.rho[id + 1]);
-    if (rho > 0.f) vyn = (d1 + d2) * dt / rho + vyn;
+    if (ey >= 0) {
+      dy2 = cpml_step(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, y - off, q8, dy2, pn);
+      O.psi[SL_SYYY][py] = pn;
+    }
+    if (rhoY > 0.f) vyn = (dy1 + dy2) * dt / rhoY + vyn;
   } else {
     if (ex >= 0) O.psi[SL_SXYX][px] = I.psi[SL_SXYX][px];
     if (ey >= 0) O.psi[SL_SYYY][py] = I.psi[SL_SYYY][py];
     vxn = 0.f;
     vyn = 0.f;
   }
-  // source after the update, anywhere (wgsl2d :1059-1065)
-  const int col = A.src_map[id];
-  if (col >= 0 && A.n < A.src_rows) {
-    const float rho = rho_vy_edge(A, x, y, id);
-    if (rho > 0.f) vyn = vyn + A.src[(long)A.n * A.src_cols + col] * dt / rho;
-  }
+  if (src_on && rho_src > 0.f) vyn = vyn + sval * dt / rho_src;
   O.vx[id] = vxn;
   O.vy[id] = vyn;
   return __float_as_uint(vxn * vxn + vyn * vyn);
@@ -362,58 +439,100 @@ __global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
   const long px = (long)ex * ny + y, py = (long)x * A.ax[1].nb + ey;
   const bool inX = x >= last + 1 && x < nx - last && y >= last + 1 && y < ny - last;
   const bool inY = x >= last && x < nx - last - 1 && y >= last && y < ny - last - 1;
+  const bool inXY = inX && inY;
 
+  // ---- loads
   // adjoint of v_{n+1}: carried part + transposed stencils of the stress
   // pass of step n+1 (a1 = D+x vx, a4 = D+y vx, a2 = D-y vy, a3 = D-x vy)
-  const float vxb = B.vx[id] + dT<ORD, true, 0>(A, A.abar[0], x, y) + dT<ORD, true, 1>(A, A.abar[3], x, y);
-  const float vyb = B.vy[id] + dT<ORD, false, 1>(A, A.abar[1], x, y) + dT<ORD, false, 0>(A, A.abar[2], x, y);
-
-  float a5b = 0.f, a6b = 0.f, a7b = 0.f, a8b = 0.f;
+  const float bvx = B.vx[id], bvy = B.vy[id];
+  const Taps<ORD> u0 = tapsT<ORD, true, 0>(A, A.abar[0], x, y), u3 = tapsT<ORD, true, 1>(A, A.abar[3], x, y);
+  const Taps<ORD> u1 = tapsT<ORD, false, 1>(A, A.abar[1], x, y), u2 = tapsT<ORD, false, 0>(A, A.abar[2], x, y);
+  Taps<ORD> t1, t2, t3, t4;
+  float p5 = 0.f, p6 = 0.f, p7 = 0.f, p8 = 0.f;       // psi' of level n+1
+  float b5 = 0.f, b6 = 0.f, b7 = 0.f, b8 = 0.f;       // their adjoints
+  float rhoX = 0.f, r00 = 0.f, r10 = 0.f, r11 = 0.f, r01 = 0.f, gx = 0.f, gy = 0.f;
   if (inX) {
-    const float rho = A.rho[id];
-    float eb = 0.f;
-    if (inY && rho > 0.f) {
-      eb = vxb * dt / rho;
-      float d1 = 0.f, d2 = 0.f;
-#pragma unroll
-      for (int c = 0; c < ORD; c++) {
-        d1 += fd_term<VSE_VAR_WGSL>(A.C[c], O.sxx[id + (long)c * ny], O.sxx[id - (long)(c + 1) * ny], A.dx, 0.f);
-        d2 += fd_term<VSE_VAR_WGSL>(A.C[c], O.sxy[id + c], O.sxy[id - c - 1], A.dy, 0.f);
-      }
-      // e = D/k + psi', psi' as stored in level n+1
-      const float e5 = ex >= 0 ? d1 / A.ax[0].k[x - off] + O.psi[SL_SXXX][px] : d1;
-      const float e6 = ey >= 0 ? d2 / A.ax[1].k[y - off] + O.psi[SL_SXYY][py] : d2;
-      A.g[VSE_GR_RHO_VX][id] += -((double)(e5 + e6) * dt / ((double)rho * rho)) * vxb;
-    }
-    a5b = cpml_adj(A.ax[0].a, A.ax[0].b, A.ax[0].k, B.psi[SL_SXXX], px, x - off, ex >= 0, eb);
-    a6b = cpml_adj(A.ax[1].a, A.ax[1].b, A.ax[1].k, B.psi[SL_SXYY], py, y - off, ey >= 0, eb);
+    rhoX = A.rho[id];
+    if (ex >= 0) b5 = B.psi[SL_SXXX][px];
+    if (ey >= 0) b6 = B.psi[SL_SXYY][py];
+  }
+  if (inXY) {
+    t1 = taps<ORD, false>(O.sxx, id, ny);
+    t2 = taps<ORD, false>(O.sxy, id, 1);
+    if (ex >= 0) p5 = O.psi[SL_SXXX][px];
+    if (ey >= 0) p6 = O.psi[SL_SXYY][py];
+    gx = A.gf[VSE_GR_RHO_VX][id];
   }
   if (inY) {
-    const float rho = 0.25f * (A.rho[id] + A.rho[id + ny] + A.rho[id + ny + 1] + A.rho[id + 1]);
-    float eb = 0.f;
-    if (rho > 0.f) {
-      eb = vyb * dt / rho;
-      float d1 = 0.f, d2 = 0.f;
-#pragma unroll
-      for (int c = 0; c < ORD; c++) {
-        d1 += fd_term<VSE_VAR_WGSL>(A.C[c], O.sxy[id + (long)(c + 1) * ny], O.sxy[id - (long)c * ny], A.dx, 0.f);
-        d2 += fd_term<VSE_VAR_WGSL>(A.C[c], O.syy[id + c + 1], O.syy[id - c], A.dy, 0.f);
-      }
-      const float e7 = ex >= 0 ? d1 / A.ax[0].kh[x - off] + O.psi[SL_SXYX][px] : d1;
-      const float e8 = ey >= 0 ? d2 / A.ax[1].kh[y - off] + O.psi[SL_SYYY][py] : d2;
-      A.g[VSE_GR_RHO_VY][id] += -((double)(e7 + e8) * dt / ((double)rho * rho)) * vyb;
+    r00 = A.rho[id];
+    r10 = A.rho[id + ny];
+    r11 = A.rho[id + ny + 1];
+    r01 = A.rho[id + 1];
+    t3 = taps<ORD, true>(O.sxy, id, ny);
+    t4 = taps<ORD, true>(O.syy, id, 1);
+    gy = A.gf[VSE_GR_RHO_VY][id];
+    if (ex >= 0) {
+      p7 = O.psi[SL_SXYX][px];
+      b7 = B.psi[SL_SXYX][px];
     }
-    a7b = cpml_adj(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, B.psi[SL_SXYX], px, x - off, ex >= 0, eb);
-    a8b = cpml_adj(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, B.psi[SL_SYYY], py, y - off, ey >= 0, eb);
+    if (ey >= 0) {
+      p8 = O.psi[SL_SYYY][py];
+      b8 = B.psi[SL_SYYY][py];
+    }
+  }
+  const int col = A.src_map[id];
+  const bool src_on = col >= 0 && A.n < A.src_rows;
+  float rho_src = 0.f, sval = 0.f;
+  if (src_on) {
+    rho_src = rho_vy_edge(A, x, y, id);
+    sval = A.src[(long)A.n * A.src_cols + col];
+  }
+  if (!inY && src_on) gy = A.gf[VSE_GR_RHO_VY][id];
+
+  // ---- adjoint of the velocity pass
+  const float vxb = bvx + sumT<ORD, 0>(A, u0) + sumT<ORD, 1>(A, u3);
+  const float vyb = bvy + sumT<ORD, 1>(A, u1) + sumT<ORD, 0>(A, u2);
+  float a5b = 0.f, a6b = 0.f, a7b = 0.f, a8b = 0.f;
+  if (inX) {
+    float eb = 0.f;
+    if (inY && rhoX > 0.f) {
+      eb = vxb * dt / rhoX;
+      const float dx1 = diff<ORD>(A, t1, A.dx), dx2 = diff<ORD>(A, t2, A.dy);
+      // e = D/k + psi', psi' as stored in level n+1
+      const float e5 = ex >= 0 ? dx1 / A.ax[0].k[x - off] + p5 : dx1;
+      const float e6 = ey >= 0 ? dx2 / A.ax[1].k[y - off] + p6 : dx2;
+      gx = gx + (float)(-((double)(e5 + e6) * dt / ((double)rhoX * rhoX)) * vxb);
+    }
+    a5b = ex >= 0 ? cpml_back(A.ax[0].a, A.ax[0].b, A.ax[0].k, x - off, b5, eb, b5) : eb;
+    a6b = ey >= 0 ? cpml_back(A.ax[1].a, A.ax[1].b, A.ax[1].k, y - off, b6, eb, b6) : eb;
+  }
+  double ty = 0.0;
+  if (inY) {
+    float eb = 0.f;
+    const float rhoY = 0.25f * (r00 + r10 + r11 + r01);
+    if (rhoY > 0.f) {
+      eb = vyb * dt / rhoY;
+      const float dy1 = diff<ORD>(A, t3, A.dx), dy2 = diff<ORD>(A, t4, A.dy);
+      const float e7 = ex >= 0 ? dy1 / A.ax[0].kh[x - off] + p7 : dy1;
+      const float e8 = ey >= 0 ? dy2 / A.ax[1].kh[y - off] + p8 : dy2;
+      ty += -((double)(e7 + e8) * dt / ((double)rhoY * rhoY)) * vyb;
+    }
+    a7b = ex >= 0 ? cpml_back(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, x - off, b7, eb, b7) : eb;
+    a8b = ey >= 0 ? cpml_back(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, y - off, b8, eb, b8) : eb;
   }
   // the source term dt*src/rho_vy of step n
-  const int col = A.src_map[id];
-  if (col >= 0 && A.n < A.src_rows) {
-    const float rho = rho_vy_edge(A, x, y, id);
-    if (rho > 0.f) {
-      const float val = A.src[(long)A.n * A.src_cols + col];
-      A.g[VSE_GR_RHO_VY][id] += -((double)val * dt / ((double)rho * rho)) * vyb;
-    }
+  if (src_on && rho_src > 0.f) ty += -((double)sval * dt / ((double)rho_src * rho_src)) * vyb;
+
+  // ---- stores
+  if (inXY) A.gf[VSE_GR_RHO_VX][id] = gx;
+  if (inY || src_on) A.gf[VSE_GR_RHO_VY][id] = gy + (float)ty;
+  if (inX) {
+    if (ex >= 0) B.psi[SL_SXXX][px] = b5;
+    if (ey >= 0) B.psi[SL_SXYY][py] = b6;
+  }
+  if (inY) {
+    if (ex >= 0) B.psi[SL_SXYX][px] = b7;
+    if (ey >= 0) B.psi[SL_SYYY][py] = b8;
   }
   A.abar[4][id] = a5b;
   A.abar[5][id] = a6b;
@@ -430,6 +549,7 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
   const int nx = A.nx, ny = A.ny;
   if (x >= nx || y >= ny) return;
   constexpr int last = ORD - 1, off = ORD - 1;
+  constexpr int LR = LC > 0 ? LC : 1;
   const long id = (long)x * ny + y;
   const long nc = A.ncell;
   const float dt = A.dt;
@@ -437,34 +557,93 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
   const DLevel &I = A.in, &B = A.bar;
   const int ex = owned_band(A.ax[0], x, off), ey = owned_band(A.ax[1], y, off);
   const long px = (long)ex * ny + y, py = (long)x * A.ax[1].nb + ey;
+  const bool inN = x >= last && x < nx - last - 1 && y >= last + 1 && y < ny - last;
+  const bool inS = x >= last + 1 && x < nx - last && y >= last && y < ny - last - 1;
 
+  // ---- loads
   // adjoint of sigma_{n+1}: carried part (incl. receivers) + transposed
   // stencils of the velocity pass (a5 = D-x sxx, a6 = D-y sxy, a7 = D+x sxy, a8 = D+y syy)
-  const float P = B.sxx[id] + dT<ORD, false, 0>(A, A.abar[4], x, y);
-  const float Q = B.syy[id] + dT<ORD, true, 1>(A, A.abar[7], x, y);
-  const float R = B.sxy[id] + dT<ORD, false, 1>(A, A.abar[5], x, y) + dT<ORD, true, 0>(A, A.abar[6], x, y);
-  B.sxx[id] = P;
-  B.syy[id] = Q;
-  B.sxy[id] = R;
+  const float bsxx = B.sxx[id], bsyy = B.syy[id], bsxy = B.sxy[id];
+  const Taps<ORD> u4 = tapsT<ORD, false, 0>(A, A.abar[4], x, y), u7 = tapsT<ORD, true, 1>(A, A.abar[7], x, y);
+  const Taps<ORD> u5 = tapsT<ORD, false, 1>(A, A.abar[5], x, y), u6 = tapsT<ORD, true, 0>(A, A.abar[6], x, y);
+  Taps<ORD> t1, t2, t3, t4;
+  float q1 = 0.f, q2 = 0.f, q3 = 0.f, q4 = 0.f;       // psi of level n (v slots)
+  float b1 = 0.f, b2 = 0.f, b3 = 0.f, b4 = 0.f;       // adjoints of psi'
+  float rho0 = 0.f, rhoN = 0.f, rhoS = 0.f, cp0 = 0.f, cpN = 0.f, cs0 = 0.f, csN = 0.f, csS = 0.f, ts = 1.f;
+  float gl = 0.f, gm = 0.f, gmxy = 0.f, gt = 0.f;
+  float pxx[LR], pyy[LR], bxx[LR], byy[LR], po[LR], pb[LR];
+  if (inN || inS) {
+    rho0 = A.rho[id];
+    cs0 = A.cs[id];
+    ts = A.ts[id];
+    gt = A.gf[VSE_GR_TAU][id];
+  }
+  if (inN) {
+    t1 = taps<ORD, true>(I.vx, id, ny);
+    t2 = taps<ORD, false>(I.vy, id, 1);
+    if (ex >= 0) {
+      q1 = I.psi[SL_VXX][px];
+      b1 = B.psi[SL_VXX][px];
+    }
+    if (ey >= 0) {
+      q2 = I.psi[SL_VYY][py];
+      b2 = B.psi[SL_VYY][py];
+    }
+    rhoN = A.rho[id + ny];
+    cpN = A.cp[id + ny];
+    cp0 = A.cp[id];
+    csN = A.cs[id + ny];
+    gl = A.gf[VSE_GR_LAMBDA][id];
+    gm = A.gf[VSE_GR_MU][id];
+    if constexpr (LC > 0) {
+#pragma unroll
+      for (int l = 0; l < LC; l++) {
+        pxx[l] = I.r[0][l * nc + id];
+        pyy[l] = I.r[1][l * nc + id];
+        bxx[l] = B.r[0][l * nc + id];
+        byy[l] = B.r[1][l * nc + id];
+      }
+    }
+  }
+  if (inS) {
+    t3 = taps<ORD, false>(I.vy, id, ny);
+    t4 = taps<ORD, true>(I.vx, id, 1);
+    if (ex >= 0) {
+      q3 = I.psi[SL_VYX][px];
+      b3 = B.psi[SL_VYX][px];
+    }
+    if (ey >= 0) {
+      q4 = I.psi[SL_VXY][py];
+      b4 = B.psi[SL_VXY][py];
+    }
+    rhoS = A.rho[id + 1];
+    csS = A.cs[id + 1];
+    gmxy = A.gf[VSE_GR_MU_XY][id];
+    if constexpr (LC > 0) {
+#pragma unroll
+      for (int l = 0; l < LC; l++) {
+        po[l] = I.r[2][l * nc + id];
+        pb[l] = B.r[2][l * nc + id];
+      }
+    }
+  }
+  const float rs = 1.0f / ts;
 
-  const float rs = 1.0f / A.ts[id];
+  // ---- adjoint of the stress pass
+  const float P = bsxx + sumT<ORD, 0>(A, u4);
+  const float Q = bsyy + sumT<ORD, 1>(A, u7);
+  const float R = bsxy + sumT<ORD, 1>(A, u5) + sumT<ORD, 0>(A, u6);
   double gs = 0.0;
   float a1b = 0.f, a2b = 0.f, a3b = 0.f, a4b = 0.f;
-  const bool inN = x >= last && x < nx - last - 1 && y >= last + 1 && y < ny - last;
   if (inN) {
-    float d1 = 0.f, d2 = 0.f;
-#pragma unroll
-    for (int c = 0; c < ORD; c++) {
-      d1 += fd_term<VSE_VAR_WGSL>(A.C[c], I.vx[id + (long)(c + 1) * ny], I.vx[id - (long)c * ny], A.dx, 0.f);
-      d2 += fd_term<VSE_VAR_WGSL>(A.C[c], I.vy[id + c], I.vy[id - c - 1], A.dy, 0.f);
-    }
-    const float e1 = cpml_val(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, I.psi[SL_VXX], px, x - off, ex >= 0, d1);
-    const float e2 = cpml_val(A.ax[1].a, A.ax[1].b, A.ax[1].k, I.psi[SL_VYY], py, y - off, ey >= 0, d2);
-    const float rho_h = 0.5f * (A.rho[id + ny] + A.rho[id]);
-    const float cp_h = 0.5f * (A.cp[id + ny] + A.cp[id]);
-    const float csa = A.cs[id + ny], csb = A.cs[id];
-    const float cs_l = 0.5f * (csa + csb);
-    const float cs_m = (fminf(csa, csb) == 0.f) ? 0.f : cs_l;
+    float pn;
+    const float d1 = diff<ORD>(A, t1, A.dx), d2 = diff<ORD>(A, t2, A.dy);
+    const float e1 = ex >= 0 ? cpml_step(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, x - off, q1, d1, pn) : d1;
+    const float e2 = ey >= 0 ? cpml_step(A.ax[1].a, A.ax[1].b, A.ax[1].k, y - off, q2, d2, pn) : d2;
+    const float rho_h = 0.5f * (rhoN + rho0);
+    const float cp_h = 0.5f * (cpN + cp0);
+    const float cs_l = 0.5f * (csN + cs0);
+    const float cs_m = (fminf(csN, cs0) == 0.f) ? 0.f : cs_l;
     const float lambda = rho_h * (cp_h * cp_h - 2.0f * cs_l * cs_l);
     const float mu = rho_h * (cs_m * cs_m);
     const float lp2m = lambda + 2.0f * mu, lpm = lambda + mu;
@@ -473,14 +652,6 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
     const float Sxxb = 0.5f * lpm * dt * (P + Q), Syyb = mu * dt * (P - Q);
     const float Xp = e1 + e2, Xs = 0.5f * (e1 - e2);
     float sum_xx = 0.f, sum_yy = 0.f;
-    float pxx[LC > 0 ? LC : 1], pyy[LC > 0 ? LC : 1], bxx[LC > 0 ? LC : 1], byy[LC > 0 ? LC : 1];
-    if constexpr (LC > 0) {
-#pragma unroll
-      for (int l = 0; l < LC; l++) {
-        pxx[l] = I.r[0][l * nc + id]; pyy[l] = I.r[1][l * nc + id];
-        bxx[l] = B.r[0][l * nc + id]; byy[l] = B.r[1][l * nc + id];
-      }
-    }
 #pragma unroll
     for (int l = 0; l < (LC > 0 ? LC : L); l++) {
       const ZCell z = zener_cell(A, l, rs);
@@ -499,35 +670,24 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
       gs += (double)Rxx * ((double)z.m_p * rs) * ((double)oxx * z.h_p - (double)Xp * z.phi_p + (double)nxx * z.h_p);
       gs += (double)Ryy * ((double)z.m_s * rs) * ((double)oyy * z.h_s - (double)Xs * z.phi_s + (double)nyy * z.h_s);
     }
-    A.g[VSE_GR_LAMBDA][id] += ((double)P + Q) * ((double)e1 + e2 + 0.5 * sum_xx) * dt;
-    A.g[VSE_GR_MU][id] += ((double)P * (2.0 * e1 + 0.5 * sum_xx + sum_yy) +
-                           (double)Q * (2.0 * e2 + 0.5 * sum_xx - sum_yy)) * dt;
-    a1b = cpml_adj(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, B.psi[SL_VXX], px, x - off, ex >= 0, e1b);
-    a2b = cpml_adj(A.ax[1].a, A.ax[1].b, A.ax[1].k, B.psi[SL_VYY], py, y - off, ey >= 0, e2b);
+    gl = gl + (float)(((double)P + Q) * ((double)e1 + e2 + 0.5 * sum_xx) * dt);
+    gm = gm + (float)(((double)P * (2.0 * e1 + 0.5 * sum_xx + sum_yy) +
+                       (double)Q * (2.0 * e2 + 0.5 * sum_xx - sum_yy)) * dt);
+    a1b = ex >= 0 ? cpml_back(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, x - off, b1, e1b, b1) : e1b;
+    a2b = ey >= 0 ? cpml_back(A.ax[1].a, A.ax[1].b, A.ax[1].k, y - off, b2, e2b, b2) : e2b;
   }
-  const bool inS = x >= last + 1 && x < nx - last && y >= last && y < ny - last - 1;
   if (inS) {
-    float d3 = 0.f, d4 = 0.f;
-#pragma unroll
-    for (int c = 0; c < ORD; c++) {
-      d3 += fd_term<VSE_VAR_WGSL>(A.C[c], I.vy[id + (long)c * ny], I.vy[id - (long)(c + 1) * ny], A.dx, 0.f);
-      d4 += fd_term<VSE_VAR_WGSL>(A.C[c], I.vx[id + c + 1], I.vx[id - c], A.dy, 0.f);
-    }
-    const float e3 = cpml_val(A.ax[0].a, A.ax[0].b, A.ax[0].k, I.psi[SL_VYX], px, x - off, ex >= 0, d3);
-    const float e4 = cpml_val(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, I.psi[SL_VXY], py, y - off, ey >= 0, d4);
-    const float rho_h = 0.5f * (A.rho[id + 1] + A.rho[id]);
-    const float csa = A.cs[id + 1], csb = A.cs[id];
-    const float cs_h = (fminf(csa, csb) == 0.f) ? 0.f : 0.5f * (csa + csb);
+    float pn;
+    const float d3 = diff<ORD>(A, t3, A.dx), d4 = diff<ORD>(A, t4, A.dy);
+    const float e3 = ex >= 0 ? cpml_step(A.ax[0].a, A.ax[0].b, A.ax[0].k, x - off, q3, d3, pn) : d3;
+    const float e4 = ey >= 0 ? cpml_step(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, y - off, q4, d4, pn) : d4;
+    const float rho_h = 0.5f * (rhoS + rho0);
+    const float cs_h = (fminf(csS, cs0) == 0.f) ? 0.f : 0.5f * (csS + cs0);
     const float mu = rho_h * (cs_h * cs_h);
     const float X = e3 + e4;
     float eb = mu * dt * R;
     const float Sb = 0.5f * mu * dt * R;
     float sum_xy = 0.f;
-    float po[LC > 0 ? LC : 1], pb[LC > 0 ? LC : 1];
-    if constexpr (LC > 0) {
-#pragma unroll
-      for (int l = 0; l < LC; l++) { po[l] = I.r[2][l * nc + id]; pb[l] = B.r[2][l * nc + id]; }
-    }
 #pragma unroll
     for (int l = 0; l < (LC > 0 ? LC : L); l++) {
       const ZCell z = zener_cell(A, l, rs);
@@ -539,15 +699,46 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
       eb += Rl * z.phi_s * z.m_s;
       gs += (double)Rl * ((double)z.m_s * rs) * ((double)o * z.h_s - (double)X * z.phi_s + (double)n * z.h_s);
     }
-    A.g[VSE_GR_MU_XY][id] += (double)R * ((double)X + 0.5 * sum_xy) * dt;
-    a3b = cpml_adj(A.ax[0].a, A.ax[0].b, A.ax[0].k, B.psi[SL_VYX], px, x - off, ex >= 0, eb);
-    a4b = cpml_adj(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, B.psi[SL_VXY], py, y - off, ey >= 0, eb);
+    gmxy = gmxy + (float)((double)R * ((double)X + 0.5 * sum_xy) * dt);
+    a3b = ex >= 0 ? cpml_back(A.ax[0].a, A.ax[0].b, A.ax[0].k, x - off, b3, eb, b3) : eb;
+    a4b = ey >= 0 ? cpml_back(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, y - off, b4, eb, b4) : eb;
   }
-  if (inN || inS) A.g[VSE_GR_TAU][id] += gs;
+
+  // ---- stores
+  B.sxx[id] = P;
+  B.syy[id] = Q;
+  B.sxy[id] = R;
+  if (inN) {
+    if (ex >= 0) B.psi[SL_VXX][px] = b1;
+    if (ey >= 0) B.psi[SL_VYY][py] = b2;
+    A.gf[VSE_GR_LAMBDA][id] = gl;
+    A.gf[VSE_GR_MU][id] = gm;
+  }
+  if (inS) {
+    if (ex >= 0) B.psi[SL_VYX][px] = b3;
+    if (ey >= 0) B.psi[SL_VXY][py] = b4;
+    A.gf[VSE_GR_MU_XY][id] = gmxy;
+  }
+  if (inN || inS) A.gf[VSE_GR_TAU][id] = gt + (float)gs;
   A.abar[0][id] = a1b;
   A.abar[1][id] = a2b;
   A.abar[2][id] = a3b;
   A.abar[3][id] = a4b;
+}
+
+// FP64 time accumulation: g += partial, partial = 0 (a partial that is 0
+// leaves both as they are).
+__global__ __launch_bounds__(256) void k_grad_flush(DiffArgs A) {
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i >= A.ncell) return;
+#pragma unroll
+  for (int k = 0; k < VSE_GR_COUNT; k++) {
+    const float p = A.gf[k][i];
+    if (p != 0.f) {
+      A.g[k][i] += (double)p;
+      A.gf[k][i] = 0.f;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_src_map(int *map, long ncell, const int *cells, const int *cols, int n) {
@@ -618,6 +809,11 @@ hipError_t vse_launch_adj_stress(const DiffArgs &A, hipStream_t st) {
   else hipLaunchKernelGGL((k_adj_stress<O, 0>), grid_of(A), dim3(BY, BX), 0, st, A)
   VSE_ORD_SWITCH(A.ord, CALL)
 #undef CALL
+  return hipGetLastError();
+}
+
+hipError_t vse_launch_grad_flush(const DiffArgs &A, hipStream_t st) {
+  hipLaunchKernelGGL(k_grad_flush, dim3((unsigned)((A.ncell + 255) / 256)), dim3(256), 0, st, A);
   return hipGetLastError();
 }
 

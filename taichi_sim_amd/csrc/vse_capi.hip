@@ -940,6 +940,7 @@ struct vse_grad {
   DLevel bar{};
   float *abar[8] = {};
   double *g[VSE_GR_COUNT] = {};
+  float *gf[VSE_GR_COUNT] = {};   // FP32 per-step partials of g
   float *ts = nullptr, *traces = nullptr, *gbar = nullptr, *obs = nullptr;
   int *src_map = nullptr;
   double *misfit = nullptr;
@@ -1006,7 +1007,10 @@ static void make_diff(vse_grad *g, DiffArgs &A) {
   A.zk = s->zk;
   A.bar = g->bar;
   for (int i = 0; i < 8; i++) A.abar[i] = g->abar[i];
-  for (int i = 0; i < VSE_GR_COUNT; i++) A.g[i] = g->g[i];
+  for (int i = 0; i < VSE_GR_COUNT; i++) {
+    A.g[i] = g->g[i];
+    A.gf[i] = g->gf[i];
+  }
   A.v2max = g->ctrl;
   A.src_map = g->src_map; A.src = s->src; A.src_cols = s->src_cols; A.src_rows = d.n_rows;
   A.n_rec = s->n_rec; A.rec_gate = s->rec_gate; A.rec_beg = s->rec_beg; A.rec_cell = s->rec_cell;
@@ -1070,6 +1074,7 @@ int vse_grad_create(vse_sim *s, int seg_len, vse_grad **out) {
   if (!rc) rc = g->alloc_level(g->bar);
   for (int i = 0; i < 8 && !rc; i++) rc = g->alloc(&g->abar[i], s->ncell);
   for (int i = 0; i < VSE_GR_COUNT && !rc; i++) rc = g->alloc(&g->g[i], s->ncell);
+  for (int i = 0; i < VSE_GR_COUNT && !rc; i++) rc = g->alloc(&g->gf[i], s->ncell);
   if (!rc) rc = g->alloc(&g->ts, s->ncell);
   if (!rc) rc = g->alloc(&g->src_map, s->ncell);
   const size_t nt = (size_t)N * std::max(s->n_rec, 1);
@@ -1186,8 +1191,10 @@ int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms) 
   if (!rc) rc = g->zero_level(g->bar);
   for (int i = 0; i < 8 && !rc; i++)
     rc = g->hip(hipMemsetAsync(g->abar[i], 0, s->ncell * sizeof(float), s->stream), "zero abar");
-  for (int i = 0; i < VSE_GR_COUNT && !rc; i++)
+  for (int i = 0; i < VSE_GR_COUNT && !rc; i++) {
     rc = g->hip(hipMemsetAsync(g->g[i], 0, s->ncell * sizeof(double), s->stream), "zero gradient");
+    if (!rc) rc = g->hip(hipMemsetAsync(g->gf[i], 0, s->ncell * sizeof(float), s->stream), "zero partials");
+  }
   if (!rc) rc = grad_src_map(g);
   DiffArgs A;
   make_diff(g, A);
@@ -1208,6 +1215,7 @@ int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms) 
       A.n = n;
       hipError_t e = vse_launch_adj_velocity(A, s->stream);
       if (e == hipSuccess) e = vse_launch_adj_stress(A, s->stream);
+      if (e == hipSuccess && n % VSE_GRAD_FLUSH == 0) e = vse_launch_grad_flush(A, s->stream);
       rc = g->hip(e, "adjoint step");
     }
   }

@@ -143,6 +143,10 @@ struct DLevel {
 // Gradient accumulators (FP64), one per coefficient the kernels use.
 enum { VSE_GR_LAMBDA = 0, VSE_GR_MU = 1, VSE_GR_MU_XY = 2, VSE_GR_RHO_VX = 3, VSE_GR_RHO_VY = 4,
        VSE_GR_TAU = 5, VSE_GR_COUNT = 6 };
+// The reverse sweep adds the FP32 partials into the FP64 gradients after every
+// step n with n % VSE_GRAD_FLUSH == 0, so the grouping depends on n only (not
+// on the checkpoint spacing).
+constexpr int VSE_GRAD_FLUSH = 32;
 
 struct DiffArgs {
   int nx, ny, ord;
@@ -156,7 +160,8 @@ struct DiffArgs {
   DLevel in, out;                    // forward: n -> n+1; adjoint: levels n and n+1
   DLevel bar;                        // adjoint state (in place)
   VSE_G float *abar[8];              // adjoints of the raw derivatives a1..a8
-  VSE_G double *g[VSE_GR_COUNT];
+  VSE_G double *g[VSE_GR_COUNT];     // gradients (FP64 time accumulation)
+  VSE_G float *gf[VSE_GR_COUNT];     // FP32 per-step partials, added into g every VSE_GRAD_FLUSH steps
   unsigned *v2max;
   ZenerK zk;                         // the constants at scale 1 (vse_create's values)
   // sources and receivers, handled inside the velocity kernels
@@ -176,6 +181,7 @@ hipError_t vse_launch_dvelocity(const DiffArgs &A, hipStream_t st);
 hipError_t vse_launch_adj_velocity(const DiffArgs &A, hipStream_t st);
 hipError_t vse_launch_src_map(int *map, long ncell, const int *cells, const int *cols, int n, hipStream_t st);
 hipError_t vse_launch_adj_stress(const DiffArgs &A, hipStream_t st);
+hipError_t vse_launch_grad_flush(const DiffArgs &A, hipStream_t st);
 hipError_t vse_launch_misfit(const float *traces, const float *obs, float *resid, long n, double *acc,
                              hipStream_t st);
 

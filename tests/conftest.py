@@ -11,6 +11,15 @@ if ROOT not in sys.path:
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
+# libvse, and with it the ROCm runtime it was compiled with, before any test
+# module imports torch (whose wheel carries another HIP runtime under the
+# same library names)
+try:
+    from taichi_sim_amd import _lib as _vse_lib
+    _vse_lib.load()
+except ImportError:
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU and the built libvse.so")

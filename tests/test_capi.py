@@ -25,6 +25,13 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
+def test_library_runs_on_the_rocm_runtime_it_was_built_with():
+    # tests/conftest.py loads libvse before any test module imports torch,
+    # whose wheel carries another libamdhip64 under the same library name
+    rt = _lib.runtime_path()
+    assert rt is not None and rt.startswith(_lib.ROCM_PATH + os.sep), rt
+
+
 def test_abi_version_and_arg_validation_without_gpu():
     lib = _lib.load()
     assert lib.vse_abi_version() == 1

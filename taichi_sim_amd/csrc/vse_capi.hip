@@ -125,7 +125,10 @@ int vse_create(const vse_desc *desc, vse_sim **out) {
 
   vse_sim *s = new vse_sim();
   s->d = d;
-  if (d.ndim == 2) s->d.nz = 1;
+  if (d.ndim == 2) {
+    s->d.nz = 1;
+    s->tune[0] = s->tune[1] = KTune{4, 0, 0, 1};   // XCD-contiguous tile order
+  }
   s->ncell = (long)s->d.nx * s->d.ny * s->d.nz;
   int rc = s->hip(hipGetDevice(&s->device), "hipGetDevice");
   if (!rc) rc = s->hip(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -381,9 +384,9 @@ static hipError_t enqueue_step(vse_sim *s, const StepArgs &A, const SparseArgs &
   const vse_desc &d = s->d;
   hipError_t e;
   if (d.ndim == 2) {
-    e = vse_launch_stress2d(d.variant, d.visco, d.ord, A, s->stream);
+    e = vse_launch_stress2d(d.variant, d.visco, d.ord, A, s->tune[0], s->stream);
     if (e != hipSuccess) return e;
-    e = vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->stream);
+    e = vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->tune[1], s->stream);
   } else {
     e = vse_launch_stress3d(d.visco, d.ord, s->math, A, s->tune[0], s->stream);
     if (e != hipSuccess) return e;
@@ -483,10 +486,10 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
   double acc[3] = {0, 0, 0};
   for (int it = 0; it < n_steps && !rc; it++) {
     hipError_t e = hipEventRecord(ev[0], s->stream);
-    if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_stress2d(d.variant, d.visco, d.ord, A, s->stream)
+    if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_stress2d(d.variant, d.visco, d.ord, A, s->tune[0], s->stream)
                                          : vse_launch_stress3d(d.visco, d.ord, s->math, A, s->tune[0], s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[1], s->stream);
-    if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->stream)
+    if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->tune[1], s->stream)
                                          : vse_launch_velocity3d(d.visco, d.ord, s->math, A, s->tune[1], s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
     if (e == hipSuccess) e = vse_launch_sparse(S, s->stream);

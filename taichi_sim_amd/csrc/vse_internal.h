@@ -186,8 +186,10 @@ hipError_t vse_launch_misfit(const float *traces, const float *obs, float *resid
                              hipStream_t st);
 
 // launchers (vse_k2d.hip / vse_k3d.hip)
-hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st);
-hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st);
+// 2D: t.remap = 1 hands each XCD a contiguous band of tiles (other fields of
+// t are not used in 2D)
+hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st);
+hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st);
 hipError_t vse_launch_stress3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
 hipError_t vse_launch_velocity3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
 hipError_t vse_launch_sparse(const SparseArgs &S, hipStream_t st);

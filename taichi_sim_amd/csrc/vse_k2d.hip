@@ -139,6 +139,10 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
   }
 }
 
+// All loads of a cell are issued before its arithmetic: raw stencil operands
+// go into registers and the sums run afterwards, in the same order, so the
+// results are the same; a division right after a load would otherwise make
+// the wavefront wait on it before issuing the loads that follow.
 template <int VAR, bool VISCO, int ORD>
 __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, int y) {
   const int nx = A.nx, ny = A.ny;
@@ -148,53 +152,102 @@ __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, in
   const float dt = A.dt;
   unsigned v2bits = 0;
   if (valid) {
-    float vxn = A.vx[id], vyn = A.vy[id];
     const bool in_vx = x >= last + 1 && x < nx - last && y >= last + 1 && y < ny - last;
     const bool in_vy = x >= last && x < nx - last - 1 && y >= last && y < ny - last - 1;
-    // vx (wgsl2d :998-1025)
+    const int jx = x - off, jy = y - off;
+    const int bx = band_index(A.ax[0], jx), by = band_index(A.ax[1], jy);
+    const long pxi = (long)bx * ny + y, pyi = (long)x * A.ax[1].nb + by;
+    // ---- loads
+    float vxn = A.vx[id], vyn = A.vy[id];
+    float xx_p[ORD], xx_m[ORD], xy_p[ORD], xy_m[ORD];     // vx: sxx along x, sxy along y (D-)
+    float yx_p[ORD], yx_m[ORD], yy_p[ORD], yy_m[ORD];     // vy: sxy along x, syy along y (D+)
+    float q1 = 0.f, q2 = 0.f, q3 = 0.f, q4 = 0.f;
+    float r00 = 0.f, r10 = 0.f, r11 = 0.f, r01 = 0.f;
+    int is = -1, step = 0;
+    if (in_vx) {
+#pragma unroll
+      for (int c = 0; c < ORD; c++) {
+        xx_p[c] = A.sxx[id + (long)c * ny];
+        xx_m[c] = A.sxx[id - (long)(c + 1) * ny];
+        xy_p[c] = A.sxy[id + c];
+        xy_m[c] = A.sxy[id - c - 1];
+      }
+      if (bx >= 0) q1 = A.psi[P_DSXX_DX][pxi];
+      if (by >= 0) q2 = A.psi[P_DSXY_DY][pyi];
+      r00 = A.rho[id];
+    }
+    if (in_vy) {
+#pragma unroll
+      for (int c = 0; c < ORD; c++) {
+        yx_p[c] = A.sxy[id + (long)(c + 1) * ny];
+        yx_m[c] = A.sxy[id - (long)c * ny];
+        yy_p[c] = A.syy[id + c + 1];
+        yy_m[c] = A.syy[id - c];
+      }
+      if (bx >= 0) q3 = A.psi[P_DSXY_DX][pxi];
+      if (by >= 0) q4 = A.psi[P_DSYY_DY][pyi];
+      // in the vy range x+1 and y+1 are inside the grid
+      r00 = A.rho[id];
+      r10 = A.rho[id + ny];
+      r11 = A.rho[id + ny + 1];
+      r01 = A.rho[id + 1];
+      if constexpr (VAR == VSE_VAR_TAICHI) {
+        is = A.src_map[id];
+        step = *A.step;
+      }
+    }
+    // ---- vx (wgsl2d :998-1025)
     if (in_vx) {
       float d1 = 0.f, d2 = 0.f;
 #pragma unroll
       for (int c = 0; c < ORD; c++) {
-        d1 += fd_term<VAR>(A.C[c], A.sxx[id + (long)c * ny], A.sxx[id - (long)(c + 1) * ny], A.dx, A.one_dx);
-        d2 += fd_term<VAR>(A.C[c], A.sxy[id + c], A.sxy[id - c - 1], A.dy, A.one_dy);
+        d1 += fd_term<VAR>(A.C[c], xx_p[c], xx_m[c], A.dx, A.one_dx);
+        d2 += fd_term<VAR>(A.C[c], xy_p[c], xy_m[c], A.dy, A.one_dy);
       }
-      const int jx = x - off, jy = y - off;
-      const int bx = band_index(A.ax[0], jx), by = band_index(A.ax[1], jy);
-      d1 = cpml_stretch(A.ax[0].a, A.ax[0].b, A.ax[0].k, A.psi[P_DSXX_DX], (long)bx * ny + y, jx, bx >= 0, d1);
-      d2 = cpml_stretch(A.ax[1].a, A.ax[1].b, A.ax[1].k, A.psi[P_DSXY_DY], (long)x * A.ax[1].nb + by, jy, by >= 0, d2);
-      const float rho = A.rho[id];
-      if (rho > 0.f) {
-        if constexpr (VAR == VSE_VAR_TAICHI) vxn += (d1 + d2) * dt / rho;
-        else vxn = (d1 + d2) * dt / rho + vxn;
+      if (bx >= 0) {
+        const float pn = A.ax[0].b[jx] * q1 + A.ax[0].a[jx] * d1;
+        A.psi[P_DSXX_DX][pxi] = pn;
+        d1 = d1 / A.ax[0].k[jx] + pn;
+      }
+      if (by >= 0) {
+        const float pn = A.ax[1].b[jy] * q2 + A.ax[1].a[jy] * d2;
+        A.psi[P_DSXY_DY][pyi] = pn;
+        d2 = d2 / A.ax[1].k[jy] + pn;
+      }
+      if (r00 > 0.f) {
+        if constexpr (VAR == VSE_VAR_TAICHI) vxn += (d1 + d2) * dt / r00;
+        else vxn = (d1 + d2) * dt / r00 + vxn;
       }
     } else if constexpr (VAR == VSE_VAR_TAICHI) {
       vxn = 0.f;     // per-component Dirichlet (sim_taichi_elastic.py:266-268)
     }
-    // vy (wgsl2d :1027-1057; sim_taichi_elastic.py:270-304)
+    // ---- vy (wgsl2d :1027-1057; sim_taichi_elastic.py:270-304)
     if (in_vy) {
       float d1 = 0.f, d2 = 0.f;
 #pragma unroll
       for (int c = 0; c < ORD; c++) {
-        d1 += fd_term<VAR>(A.C[c], A.sxy[id + (long)(c + 1) * ny], A.sxy[id - (long)c * ny], A.dx, A.one_dx);
-        d2 += fd_term<VAR>(A.C[c], A.syy[id + c + 1], A.syy[id - c], A.dy, A.one_dy);
+        d1 += fd_term<VAR>(A.C[c], yx_p[c], yx_m[c], A.dx, A.one_dx);
+        d2 += fd_term<VAR>(A.C[c], yy_p[c], yy_m[c], A.dy, A.one_dy);
       }
-      const int jx = x - off, jy = y - off;
-      const int bx = band_index(A.ax[0], jx), by = band_index(A.ax[1], jy);
-      d1 = cpml_stretch(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, A.psi[P_DSXY_DX], (long)bx * ny + y, jx, bx >= 0, d1);
-      d2 = cpml_stretch(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, A.psi[P_DSYY_DY], (long)x * A.ax[1].nb + by, jy, by >= 0, d2);
-      // in the vy range x+1 and y+1 are inside the grid
+      if (bx >= 0) {
+        const float pn = A.ax[0].bh[jx] * q3 + A.ax[0].ah[jx] * d1;
+        A.psi[P_DSXY_DX][pxi] = pn;
+        d1 = d1 / A.ax[0].kh[jx] + pn;
+      }
+      if (by >= 0) {
+        const float pn = A.ax[1].bh[jy] * q4 + A.ax[1].ah[jy] * d2;
+        A.psi[P_DSYY_DY][pyi] = pn;
+        d2 = d2 / A.ax[1].kh[jy] + pn;
+      }
       if constexpr (VAR == VSE_VAR_TAICHI) {
-        const float rho = 0.25f * (A.rho[id + 1] + A.rho[id] + A.rho[id + ny + 1] + A.rho[id + ny]);
+        const float rho = 0.25f * (r01 + r00 + r11 + r10);
         if (rho > 0.f) {
-          const int is = A.src_map[id];
           float vs = 0.f;
-          const int s = *A.step;
-          if (is != -1 && s < A.src_rows) vs = A.src[(long)s * A.src_cols + is] * dt / rho;
+          if (is != -1 && step < A.src_rows) vs = A.src[(long)step * A.src_cols + is] * dt / rho;
           vyn += vs + (d1 + d2) * dt / rho;
         }
       } else {
-        const float rho = 0.25f * (A.rho[id] + A.rho[id + ny] + A.rho[id + ny + 1] + A.rho[id + 1]);
+        const float rho = 0.25f * (r00 + r10 + r11 + r01);
         if (rho > 0.f) vyn = (d1 + d2) * dt / rho + vyn;
       }
     } else {
@@ -208,14 +261,33 @@ __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, in
   return v2bits;
 }
 
+// Tile of a 1D launch over nty (y) x ntx (x) tiles of BY x BX cells, y-tile
+// fastest.  Blocks land on the 8 XCDs round-robin, each with its own L2; with
+// remap = 1 XCD k gets the contiguous tile range [k/8, (k+1)/8) of the grid, so
+// the +-ORD halo rows and columns a tile reads mostly belong to tiles of the
+// same XCD (remap needs a tile count divisible by 8; otherwise it is ignored).
+__device__ __forceinline__ int xcd_tile(int b, int total, int remap) {
+  return (remap && (total & 7) == 0) ? (b & 7) * (total >> 3) + (b >> 3) : b;
+}
+
+struct Tile2 {
+  int ty, tx;
+};
+__device__ __forceinline__ Tile2 tile2d(int nty, int remap) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x, remap);
+  return Tile2{t % nty, t / nty};
+}
+
 template <int VAR, bool VISCO, int ORD, int LC = 0>
-__global__ __launch_bounds__(BX * BY) void k_stress2d(StepArgs A) {
-  stress2d_cell<VAR, VISCO, ORD, LC>(A, blockIdx.y * BX + threadIdx.y, blockIdx.x * BY + threadIdx.x);
+__global__ __launch_bounds__(BX * BY) void k_stress2d(StepArgs A, int nty, int remap) {
+  const Tile2 t = tile2d(nty, remap);
+  stress2d_cell<VAR, VISCO, ORD, LC>(A, t.tx * BX + threadIdx.y, t.ty * BY + threadIdx.x);
 }
 
 template <int VAR, bool VISCO, int ORD>
-__global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A) {
-  const unsigned v2 = velocity2d_cell<VAR, VISCO, ORD>(A, blockIdx.y * BX + threadIdx.y, blockIdx.x * BY + threadIdx.x);
+__global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A, int nty, int remap) {
+  const Tile2 t = tile2d(nty, remap);
+  const unsigned v2 = velocity2d_cell<VAR, VISCO, ORD>(A, t.tx * BX + threadIdx.y, t.ty * BY + threadIdx.x);
   block_max_u32(v2, A.v2max);
 }
 
@@ -236,9 +308,10 @@ __device__ __forceinline__ int batch_instance(const BatchArgs &B, int b) {
 
 template <int VAR, bool VISCO, int ORD, int LC = 0>
 __global__ __launch_bounds__(BX * BY) void k_stress2d_batch(BatchArgs B) {
-  const int i = batch_instance(B, blockIdx.x);
+  const int b = xcd_tile(blockIdx.x, gridDim.x, 1);     // XCD-contiguous tile order, as tile2d
+  const int i = batch_instance(B, b);
   const StepArgs &A = B.args[i];
-  const int t = blockIdx.x - B.tile_beg[i];
+  const int t = b - B.tile_beg[i];
   const int nyb = (A.ny + BY - 1) / BY;
   const int y = (t % nyb) * BY + threadIdx.x;
   const int x0 = (t / nyb) * BX * B.rows + threadIdx.y;
@@ -247,9 +320,10 @@ __global__ __launch_bounds__(BX * BY) void k_stress2d_batch(BatchArgs B) {
 
 template <int VAR, bool VISCO, int ORD>
 __global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B) {
-  const int i = batch_instance(B, blockIdx.x);
+  const int b = xcd_tile(blockIdx.x, gridDim.x, 1);
+  const int i = batch_instance(B, b);
   const StepArgs &A = B.args[i];
-  const int t = blockIdx.x - B.tile_beg[i];
+  const int t = b - B.tile_beg[i];
   const int nyb = (A.ny + BY - 1) / BY;
   const int y = (t % nyb) * BY + threadIdx.x;
   const int x0 = (t / nyb) * BX * B.rows + threadIdx.y;
@@ -262,15 +336,17 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B) {
 }
 
 template <int VAR, bool VISCO>
-hipError_t launch_ord(bool stress, int ord, const StepArgs &A, hipStream_t st) {
-  dim3 block(BY, BX);
-  dim3 grid((A.ny + BY - 1) / BY, (A.nx + BX - 1) / BX);
+hipError_t launch_ord(bool stress, int ord, const StepArgs &A, const KTune &t, hipStream_t st) {
+  const dim3 block(BY, BX);
+  const int nty = (A.ny + BY - 1) / BY, ntx = (A.nx + BX - 1) / BX;
+  const dim3 grid((unsigned)(nty * ntx));
+  const int rm = t.remap;
   const bool l3 = VISCO && VAR == VSE_VAR_WGSL && A.L == 3;
-#define VSE_CASE(O)                                                                              \
-  case O:                                                                                        \
-    if (stress && l3) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O, 3>), grid, block, 0, st, A); \
-    else if (stress) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O>), grid, block, 0, st, A);     \
-    else hipLaunchKernelGGL((k_velocity2d<VAR, VISCO, O>), grid, block, 0, st, A);               \
+#define VSE_CASE(O)                                                                                        \
+  case O:                                                                                                  \
+    if (stress && l3) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O, 3>), grid, block, 0, st, A, nty, rm); \
+    else if (stress) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O>), grid, block, 0, st, A, nty, rm);     \
+    else hipLaunchKernelGGL((k_velocity2d<VAR, VISCO, O>), grid, block, 0, st, A, nty, rm);               \
     break;
   switch (ord) {
     VSE_CASE(2) VSE_CASE(3) VSE_CASE(4) VSE_CASE(5) VSE_CASE(6)
@@ -298,10 +374,10 @@ hipError_t launch_batch_ord(bool stress, int ord, const BatchArgs &B, int total_
   return hipGetLastError();
 }
 
-hipError_t dispatch(bool stress, int variant, int visco, int ord, const StepArgs &A, hipStream_t st) {
-  if (variant == VSE_VAR_TAICHI) return launch_ord<VSE_VAR_TAICHI, false>(stress, ord, A, st);
-  if (visco) return launch_ord<VSE_VAR_WGSL, true>(stress, ord, A, st);
-  return launch_ord<VSE_VAR_WGSL, false>(stress, ord, A, st);
+hipError_t dispatch(bool stress, int variant, int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st) {
+  if (variant == VSE_VAR_TAICHI) return launch_ord<VSE_VAR_TAICHI, false>(stress, ord, A, t, st);
+  if (visco) return launch_ord<VSE_VAR_WGSL, true>(stress, ord, A, t, st);
+  return launch_ord<VSE_VAR_WGSL, false>(stress, ord, A, t, st);
 }
 
 }  // namespace
@@ -317,9 +393,9 @@ long vse_tiles2d(int nx, int ny, int rows) {
   return (long)((ny + BY - 1) / BY) * (((nx + BX - 1) / BX + rows - 1) / rows);
 }
 
-hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st) {
-  return dispatch(true, variant, visco, ord, A, st);
+hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st) {
+  return dispatch(true, variant, visco, ord, A, t, st);
 }
-hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, hipStream_t st) {
-  return dispatch(false, variant, visco, ord, A, st);
+hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st) {
+  return dispatch(false, variant, visco, ord, A, t, st);
 }

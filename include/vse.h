@@ -86,8 +86,10 @@ int vse_set_material(vse_sim *s, const float *rho, const float *cp, const float 
 int vse_set_cpml(vse_sim *s, int axis, const float *a, const float *b, const float *k,
                  const float *a_h, const float *b_h, const float *k_h);
 
-/* Source points: n_pts linear cell indices with their source column, and the
- * (n_rows, n_cols) row-major source matrix (value of column c at step s).
+/* Source points: n_pts distinct linear cell indices with their source column
+ * (a cell listed twice is VSE_ERR_ARG: the reference's per-cell map holds one
+ * column per cell), and the (n_rows, n_cols) row-major source matrix (value
+ * of column c at step s).
  * WGSL variants add src[s][c]*dt/rho after the velocity update; the Taichi
  * variant adds it inside the vy update.  Replaces b_force and b_idx_src
  * (sim_webgpu_viscoelastic.py:89-92; sim_taichi_elastic.py:120-130). */
@@ -115,12 +117,15 @@ int vse_run(vse_sim *s, int n_steps, int check_every, float *elapsed_ms);
 
 /* Run n_steps steps launching each kernel between its own HIP events on the
  * library's stream; ms[0..2] receive the average duration of the stress,
- * velocity and sparse (source/receiver/counter) kernels.  For roofline
+ * velocity and sparse (source/receiver/counter) kernels.  In 2D the sparse
+ * work runs inside the velocity kernel, so ms[2] is only the event gap.  For roofline
  * measurement; no reference counterpart (the reference only times the whole
  * loop, sim_webgpu_viscoelastic.py:470,537). */
 int vse_profile(vse_sim *s, int n_steps, float *ms);
 
-/* Launch shape of a 3D kernel (kernel 0 = stress, 1 = velocity): workgroup
+/* Launch shape of a kernel (kernel 0 = stress, 1 = velocity).  2D: only remap
+ * applies (1, the default: each XCD takes a contiguous band of 64x4 tiles;
+ * 0: round-robin order).  3D: workgroup
  * tile of 64 (z) x ty (y) cells, ty in {1,2,4,8}; nt = 1 streams the fields
  * touched only at the own cell with non-temporal loads/stores; each
  * workgroup marches through xchunk x-planes (0 = the plain kernel, one cell
@@ -197,7 +202,7 @@ void vse_destroy(vse_sim *s);
  * destroy it before them, and do not call a member while a batch call is in
  * flight on another thread.  No reference counterpart. */
 #define VSE_BATCH_MAX 256
-#define VSE_BATCH_GROUP_BYTES_DEFAULT ((size_t)192 << 20)
+#define VSE_BATCH_GROUP_BYTES_DEFAULT ((size_t)224 << 20)
 typedef struct vse_batch vse_batch;
 int vse_batch_create(vse_sim *const *sims, int n, vse_batch **out);
 /* Advance every member n_steps; VSE_ERR_UNSTABLE names the first unstable
@@ -209,7 +214,7 @@ int vse_batch_run(vse_batch *b, int n_steps, int check_every, float *elapsed_ms)
  * each launch bracketed by HIP events.  Advances the members. */
 int vse_batch_profile(vse_batch *b, int n_steps, float *ms);
 /* Each workgroup of a batched launch covers 64 (y) x 4*rows (x) cells of one
- * member, walking its `rows` groups of 4 x-rows in turn (default 4, 1..64).
+ * member, walking its `rows` groups of 4 x-rows in turn (default 1, 1..64).
  * Results are identical for every value; only speed changes. */
 int vse_batch_set_rows(vse_batch *b, int rows);
 /* Cache budget of a group in bytes of member device memory; 0 puts every

@@ -391,6 +391,18 @@ __device__ __forceinline__ unsigned dvelocity_cell(const DiffArgs &A, int x, int
   return __float_as_uint(vxn * vxn + vyn * vyn);
 }
 
+// Cell of this thread in a 1D launch over the (ny/BY) x (nx/BX) tiles.  Blocks
+// land on the 8 XCDs round-robin, each with its own L2; XCD k takes the
+// contiguous eighth k of the tiles (when the count divides by 8), so the
+// halo rows and columns of a tile mostly belong to tiles of the same XCD.
+__device__ __forceinline__ void tile_cell(const DiffArgs &A, int &x, int &y) {
+  const int nty = (A.ny + BY - 1) / BY;
+  const int b = blockIdx.x, total = gridDim.x;
+  const int t = (total & 7) == 0 ? (b & 7) * (total >> 3) + (b >> 3) : b;
+  x = (t / nty) * BX + threadIdx.y;
+  y = (t % nty) * BY + threadIdx.x;
+}
+
 // Receivers of step n: sigma_yy of level n+1 summed per receiver when
 // n + gate >= delay (wgsl2d :1117-1134); done by workgroup (0, 0).
 __device__ __forceinline__ void record_traces(const DiffArgs &A) {
@@ -405,24 +417,29 @@ __device__ __forceinline__ void record_traces(const DiffArgs &A) {
 
 template <int ORD, int LC>
 __global__ __launch_bounds__(BX * BY) void k_dstress(DiffArgs A) {
-  dstress_cell<ORD, LC>(A, blockIdx.y * BX + threadIdx.y, blockIdx.x * BY + threadIdx.x);
+  int x, y;
+  tile_cell(A, x, y);
+  dstress_cell<ORD, LC>(A, x, y);
 }
 
 template <int ORD>
 __global__ __launch_bounds__(BX * BY) void k_dvelocity(DiffArgs A) {
-  const unsigned v2 = dvelocity_cell<ORD>(A, blockIdx.y * BX + threadIdx.y, blockIdx.x * BY + threadIdx.x);
-  if (A.record && blockIdx.x == 0 && blockIdx.y == 0) record_traces(A);
+  int x, y;
+  tile_cell(A, x, y);
+  const unsigned v2 = dvelocity_cell<ORD>(A, x, y);
+  if (A.record && blockIdx.x == 0) record_traces(A);
   block_max_u32(v2, A.v2max);
 }
 
 // ---------------------------------------------------------------- adjoint
 template <int ORD>
 __global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
-  const int x = blockIdx.y * BX + threadIdx.y, y = blockIdx.x * BY + threadIdx.x;
+  int x, y;
+  tile_cell(A, x, y);
   const int nx = A.nx, ny = A.ny;
   // residual of step n into sigma_yy_bar at the receiver points (consumed by
   // adj_stress, which runs next; this kernel does not touch sigma_bar)
-  if (blockIdx.x == 0 && blockIdx.y == 0) {
+  if (blockIdx.x == 0) {
     const int t = threadIdx.x + threadIdx.y * blockDim.x;
     for (int r = t; r < A.n_rec; r += blockDim.x * blockDim.y) {
       if (A.n + A.rec_gate < A.rec_delay[r]) continue;
@@ -545,7 +562,8 @@ __global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
 
 template <int ORD, int LC>
 __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
-  const int x = blockIdx.y * BX + threadIdx.y, y = blockIdx.x * BY + threadIdx.x;
+  int x, y;
+  tile_cell(A, x, y);
   const int nx = A.nx, ny = A.ny;
   if (x >= nx || y >= ny) return;
   constexpr int last = ORD - 1, off = ORD - 1;
@@ -776,7 +794,7 @@ __global__ __launch_bounds__(256) void k_misfit(const float *tr, const float *ob
     default: return hipErrorInvalidValue;                                                              \
   }
 
-dim3 grid_of(const DiffArgs &A) { return dim3((A.ny + BY - 1) / BY, (A.nx + BX - 1) / BX); }
+dim3 grid_of(const DiffArgs &A) { return dim3((unsigned)(((A.ny + BY - 1) / BY) * ((A.nx + BX - 1) / BX))); }
 
 }  // namespace
 

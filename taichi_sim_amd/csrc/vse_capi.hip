@@ -48,6 +48,7 @@ struct vse_sim {
   // sources
   int n_src_pts = 0, src_cols = 1;
   int *src_cell = nullptr, *src_col = nullptr, *src_map = nullptr;
+  int *src_rowflag = nullptr;     // 2D: per x-row, 1 if it holds a source cell
   float *src = nullptr;
   // receivers
   int n_rec = 0, rec_gate = 0;
@@ -245,8 +246,16 @@ int vse_set_sources(vse_sim *s, int n_pts, const int *cells, const int *cols, in
   for (int i = 0; i < n_pts; i++)
     if (cells[i] < 0 || cells[i] >= s->ncell || cols[i] < 0 || cols[i] >= n_cols)
       return s->fail(VSE_ERR_ARG, "source point out of range");
+  {
+    // one column per cell, as the reference's per-cell source map
+    std::vector<int> sorted(cells, cells + n_pts);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+      return s->fail(VSE_ERR_ARG, "source cell listed twice");
+  }
   s->release(s->src_cell); s->release(s->src_col); s->release(s->src); s->release(s->src_map);
-  s->src_cell = s->src_col = s->src_map = nullptr;
+  s->release(s->src_rowflag);
+  s->src_cell = s->src_col = s->src_map = s->src_rowflag = nullptr;
   s->src = nullptr;
   int rc = s->alloc(&s->src_cell, n_pts);
   if (!rc) rc = s->alloc(&s->src_col, n_pts);
@@ -254,12 +263,18 @@ int vse_set_sources(vse_sim *s, int n_pts, const int *cells, const int *cols, in
   if (!rc && n_pts) rc = s->hip(hipMemcpyAsync(s->src_cell, cells, n_pts * sizeof(int), hipMemcpyHostToDevice, s->stream), "src cells");
   if (!rc && n_pts) rc = s->hip(hipMemcpyAsync(s->src_col, cols, n_pts * sizeof(int), hipMemcpyHostToDevice, s->stream), "src cols");
   if (!rc) rc = s->hip(hipMemcpyAsync(s->src, src, (size_t)s->d.n_rows * n_cols * sizeof(float), hipMemcpyHostToDevice, s->stream), "src");
-  std::vector<int> map;
-  if (!rc && s->d.variant == VSE_VARIANT_TAICHI) {
+  std::vector<int> map, rows;
+  if (!rc && s->d.ndim == 2) {     // 2D velocity kernels inject through a per-cell map
     map.assign(s->ncell, -1);
-    for (int i = 0; i < n_pts; i++) map[cells[i]] = cols[i];
+    rows.assign(s->d.nx, 0);
+    for (int i = 0; i < n_pts; i++) {
+      map[cells[i]] = cols[i];
+      rows[cells[i] / s->d.ny] = 1;
+    }
     rc = s->alloc(&s->src_map, s->ncell);
     if (!rc) rc = s->hip(hipMemcpyAsync(s->src_map, map.data(), s->ncell * sizeof(int), hipMemcpyHostToDevice, s->stream), "src map");
+    if (!rc) rc = s->alloc(&s->src_rowflag, s->d.nx);
+    if (!rc) rc = s->hip(hipMemcpyAsync(s->src_rowflag, rows.data(), s->d.nx * sizeof(int), hipMemcpyHostToDevice, s->stream), "src rows");
   }
   if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "set_sources");
   if (rc) return rc;
@@ -366,6 +381,9 @@ static void make_args(vse_sim *s, StepArgs &A, SparseArgs &S) {
   A.v2max = s->ctrl;
   A.step = (const int *)(s->ctrl + 1);
   A.src_map = s->src_map; A.src = s->src; A.src_cols = s->src_cols; A.src_rows = d.n_rows;
+  A.src_rowflag = s->src_rowflag;
+  A.n_rec = s->n_rec; A.rec_gate = s->rec_gate; A.trace_rows = d.n_rows;
+  A.rec_beg = s->rec_beg; A.rec_cell = s->rec_cell; A.rec_delay = s->rec_delay; A.traces = s->traces;
 
   memset(&S, 0, sizeof(S));
   S.ndim = d.ndim; S.nx = d.nx; S.ny = d.ny; S.nz = d.nz; S.dt = d.dt;
@@ -380,13 +398,17 @@ static void make_args(vse_sim *s, StepArgs &A, SparseArgs &S) {
   S.rec_gate = s->rec_gate; S.traces = s->traces; S.trace_rows = d.n_rows;
 }
 
-static hipError_t enqueue_step(vse_sim *s, const StepArgs &A, const SparseArgs &S) {
+// Step i of a run of k launched together.  2D: the velocity kernel advances
+// step *counter + i and does its sources and receivers, and advance_steps()
+// moves the counter after the k steps.  3D: the sparse kernel does that work
+// and increments the counter itself.
+static hipError_t enqueue_step(vse_sim *s, const StepArgs &A, const SparseArgs &S, int i) {
   const vse_desc &d = s->d;
   hipError_t e;
   if (d.ndim == 2) {
     e = vse_launch_stress2d(d.variant, d.visco, d.ord, A, s->tune[0], s->stream);
     if (e != hipSuccess) return e;
-    e = vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->tune[1], s->stream);
+    return vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->tune[1], i, s->stream);
   } else {
     e = vse_launch_stress3d(d.visco, d.ord, s->math, A, s->tune[0], s->stream);
     if (e != hipSuccess) return e;
@@ -396,6 +418,10 @@ static hipError_t enqueue_step(vse_sim *s, const StepArgs &A, const SparseArgs &
   return vse_launch_sparse(S, s->stream);
 }
 
+static hipError_t advance_steps(vse_sim *s, int k) {
+  return s->d.ndim == 2 ? vse_launch_advance((int *)(s->ctrl + 1), k, s->stream) : hipSuccess;
+}
+
 static int get_graph(vse_sim *s, int steps, const StepArgs &A, const SparseArgs &S) {
   if (s->gexec && s->gexec_steps == steps) return 0;
   s->drop_graph();
@@ -403,7 +429,8 @@ static int get_graph(vse_sim *s, int steps, const StepArgs &A, const SparseArgs 
   int rc = s->hip(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   if (rc) return rc;
   hipError_t e = hipSuccess;
-  for (int i = 0; i < steps && e == hipSuccess; i++) e = enqueue_step(s, A, S);
+  for (int i = 0; i < steps && e == hipSuccess; i++) e = enqueue_step(s, A, S, i);
+  if (e == hipSuccess) e = advance_steps(s, steps);
   hipError_t e2 = hipStreamEndCapture(s->stream, &g);
   if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return s->hip(e, "capture launch"); }
   rc = s->hip(e2, "end capture");
@@ -453,7 +480,8 @@ int vse_run(vse_sim *s, int n_steps, int check_every, float *elapsed_ms) {
       done += chunk;
       since_check += chunk;
     } else {
-      for (int i = 0; i < left && !rc; i++) rc = s->hip(enqueue_step(s, A, S), "step launch");
+      for (int i = 0; i < left && !rc; i++) rc = s->hip(enqueue_step(s, A, S, i), "step launch");
+      if (!rc) rc = s->hip(advance_steps(s, left), "step launch");
       done += left;
       since_check += left;
     }
@@ -489,10 +517,10 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
     if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_stress2d(d.variant, d.visco, d.ord, A, s->tune[0], s->stream)
                                          : vse_launch_stress3d(d.visco, d.ord, s->math, A, s->tune[0], s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[1], s->stream);
-    if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->tune[1], s->stream)
+    if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->tune[1], 0, s->stream)
                                          : vse_launch_velocity3d(d.visco, d.ord, s->math, A, s->tune[1], s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
-    if (e == hipSuccess) e = vse_launch_sparse(S, s->stream);
+    if (e == hipSuccess) e = d.ndim == 2 ? advance_steps(s, 1) : vse_launch_sparse(S, s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[3], s->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
     rc = s->hip(e, "profile step");
@@ -617,7 +645,7 @@ struct vse_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   int variant = 0, visco = 0, ord = 0;
-  int rows = 4;
+  int rows = 1;
   size_t group_bytes = 0;
   std::vector<BatchGroup> groups;
   BatchGroup all;                                   // every member in one launch (vse_batch_profile)
@@ -684,10 +712,12 @@ static int regroup(vse_batch *b) {
   return rc;
 }
 
-static hipError_t enqueue_group_step(vse_batch *b, const BatchGroup &g) {
-  hipError_t e = vse_launch_batch2d(true, b->variant, b->visco, b->ord, g.B, g.total_tiles, b->stream);
-  if (e == hipSuccess) e = vse_launch_batch2d(false, b->variant, b->visco, b->ord, g.B, g.total_tiles, b->stream);
-  if (e == hipSuccess) e = vse_launch_sparse_batch(g.B, b->stream);
+// Step i of k launched together for every member of g; the velocity pass
+// does the members' sources and receivers, and vse_launch_advance_batch
+// moves their counters after the k steps.
+static hipError_t enqueue_group_step(vse_batch *b, const BatchGroup &g, int i) {
+  hipError_t e = vse_launch_batch2d(true, b->variant, b->visco, b->ord, g.B, g.total_tiles, 0, b->stream);
+  if (e == hipSuccess) e = vse_launch_batch2d(false, b->variant, b->visco, b->ord, g.B, g.total_tiles, i, b->stream);
   return e;
 }
 
@@ -744,7 +774,8 @@ static int run_group(vse_batch *b, BatchGroup &g, int n_steps, int check_every, 
     rc = b->hip(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     if (rc) return rc;
     hipError_t e = hipSuccess;
-    for (int i = 0; i < chunk && e == hipSuccess; i++) e = enqueue_group_step(b, g);
+    for (int i = 0; i < chunk && e == hipSuccess; i++) e = enqueue_group_step(b, g, i);
+    if (e == hipSuccess) e = vse_launch_advance_batch(g.B, chunk, b->stream);
     hipError_t e2 = hipStreamEndCapture(b->stream, &gr);
     if (e != hipSuccess) { if (gr) (void)hipGraphDestroy(gr); return b->hip(e, "capture launch"); }
     rc = b->hip(e2, "end capture");
@@ -767,7 +798,8 @@ static int run_group(vse_batch *b, BatchGroup &g, int n_steps, int check_every, 
       done += chunk;
       since_check += chunk;
     } else {
-      for (int i = 0; i < left && !rc; i++) rc = b->hip(enqueue_group_step(b, g), "step launch");
+      for (int i = 0; i < left && !rc; i++) rc = b->hip(enqueue_group_step(b, g, i), "step launch");
+      if (!rc) rc = b->hip(vse_launch_advance_batch(g.B, left, b->stream), "step launch");
       done += left;
       since_check += left;
     }
@@ -879,11 +911,11 @@ int vse_batch_profile(vse_batch *b, int n_steps, float *ms) {
   double acc[3] = {0, 0, 0};
   for (int it = 0; it < n_steps && !rc; it++) {
     hipError_t e = hipEventRecord(ev[0], b->stream);
-    if (e == hipSuccess) e = vse_launch_batch2d(true, b->variant, b->visco, b->ord, g.B, g.total_tiles, b->stream);
+    if (e == hipSuccess) e = vse_launch_batch2d(true, b->variant, b->visco, b->ord, g.B, g.total_tiles, 0, b->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[1], b->stream);
-    if (e == hipSuccess) e = vse_launch_batch2d(false, b->variant, b->visco, b->ord, g.B, g.total_tiles, b->stream);
+    if (e == hipSuccess) e = vse_launch_batch2d(false, b->variant, b->visco, b->ord, g.B, g.total_tiles, 0, b->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[2], b->stream);
-    if (e == hipSuccess) e = vse_launch_sparse_batch(g.B, b->stream);
+    if (e == hipSuccess) e = vse_launch_advance_batch(g.B, 1, b->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[3], b->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
     rc = b->hip(e, "profile step");

@@ -78,6 +78,13 @@ struct StepArgs {
   const VSE_G int *src_map;          // full grid, -1 or column
   const VSE_G float *src;            // (n_rows, src_cols)
   int src_cols, src_rows;
+  // 2D single-simulation launches: the velocity kernel also does the step's
+  // sparse work -- the WGSL-variant source through src_map (rows flagged in
+  // src_rowflag) and the receivers in workgroup 0
+  const VSE_G int *src_rowflag;      // per x-row: 1 if the row holds a source cell
+  int n_rec, rec_gate, trace_rows;
+  const VSE_G int *rec_beg, *rec_cell, *rec_delay;
+  VSE_G float *traces;               // (trace_rows, n_rec)
 };
 
 // Sparse per-step work: source injection (WGSL variants), receiver gather,
@@ -189,12 +196,21 @@ hipError_t vse_launch_misfit(const float *traces, const float *obs, float *resid
 // 2D: t.remap = 1 hands each XCD a contiguous band of tiles (other fields of
 // t are not used in 2D)
 hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st);
-hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st);
+// 2D velocity (single simulation): advances step *A.step + i and does that
+// step's sources and receivers; vse_launch_advance(step, k) moves the counter
+// after k such launches
+hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, int i,
+                                 hipStream_t st);
+hipError_t vse_launch_advance(int *step, int n, hipStream_t st);
 hipError_t vse_launch_stress3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
 hipError_t vse_launch_velocity3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
 hipError_t vse_launch_sparse(const SparseArgs &S, hipStream_t st);
+// the velocity pass (stress = false) advances every member's step *step + i
+// and does its sources and receivers; vse_launch_advance_batch moves the
+// members' counters after k such steps
 hipError_t vse_launch_batch2d(bool stress, int variant, int visco, int ord, const BatchArgs &B, int total_tiles,
-                              hipStream_t st);
+                              int i, hipStream_t st);
+hipError_t vse_launch_advance_batch(const BatchArgs &B, int n, hipStream_t st);
 hipError_t vse_launch_sparse_batch(const BatchArgs &B, hipStream_t st);
 hipError_t vse_launch_batch_ctrl(const BatchArgs &B, unsigned *out, hipStream_t st);
 long vse_tiles2d(int nx, int ny, int rows);

@@ -143,8 +143,9 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
 // go into registers and the sums run afterwards, in the same order, so the
 // results are the same; a division right after a load would otherwise make
 // the wavefront wait on it before issuing the loads that follow.
+// s: the step being advanced (the Taichi-variant source row).
 template <int VAR, bool VISCO, int ORD>
-__device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, int y) {
+__device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, int y, int s) {
   const int nx = A.nx, ny = A.ny;
   const bool valid = x < nx && y < ny;
   constexpr int last = ORD - 1, off = ORD - 1;
@@ -163,7 +164,7 @@ __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, in
     float yx_p[ORD], yx_m[ORD], yy_p[ORD], yy_m[ORD];     // vy: sxy along x, syy along y (D+)
     float q1 = 0.f, q2 = 0.f, q3 = 0.f, q4 = 0.f;
     float r00 = 0.f, r10 = 0.f, r11 = 0.f, r01 = 0.f;
-    int is = -1, step = 0;
+    int is = -1;
     if (in_vx) {
 #pragma unroll
       for (int c = 0; c < ORD; c++) {
@@ -191,10 +192,7 @@ __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, in
       r10 = A.rho[id + ny];
       r11 = A.rho[id + ny + 1];
       r01 = A.rho[id + 1];
-      if constexpr (VAR == VSE_VAR_TAICHI) {
-        is = A.src_map[id];
-        step = *A.step;
-      }
+      if constexpr (VAR == VSE_VAR_TAICHI) is = A.src_map[id];
     }
     // ---- vx (wgsl2d :998-1025)
     if (in_vx) {
@@ -243,7 +241,7 @@ __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, in
         const float rho = 0.25f * (r01 + r00 + r11 + r10);
         if (rho > 0.f) {
           float vs = 0.f;
-          if (is != -1 && step < A.src_rows) vs = A.src[(long)step * A.src_cols + is] * dt / rho;
+          if (is != -1 && s < A.src_rows) vs = A.src[(long)s * A.src_cols + is] * dt / rho;
           vyn += vs + (d1 + d2) * dt / rho;
         }
       } else {
@@ -284,11 +282,74 @@ __global__ __launch_bounds__(BX * BY) void k_stress2d(StepArgs A, int nty, int r
   stress2d_cell<VAR, VISCO, ORD, LC>(A, t.tx * BX + threadIdx.y, t.ty * BY + threadIdx.x);
 }
 
+// The step's sparse work, done inside the velocity pass (vse_sparse.hip does
+// it for 3D, in the same way and order):
+// * WGSL-variant source: vy += src[s, col] * dt / rho at each source cell,
+//   after the update, anywhere on the grid, rho the vy-grid density with
+//   out-of-grid neighbours 0 (wgsl2d :1059-1065); |v|^2 of the cell is
+//   counted before and after;
+// * receivers of step s, by one workgroup: traces[s, r] += sigma_yy over the
+//   receiver's points in table order when s + gate >= delay (wgsl2d
+//   :1117-1134; sim_taichi_elastic.py:187-190).  sigma_yy is final after the
+//   stress pass and this pass does not write it.
+// The step is s = *A.step + i: a graph of k steps passes i = 0..k-1 and ends
+// with an advance of the counter by k, so it moves once per graph launch and
+// no workgroup waits for another (wgsl2d :1136-1141 advances it per step).
+template <int VAR>
+__device__ __forceinline__ void source_cell(const StepArgs &A, int x, int y, int s, unsigned &v2) {
+  if constexpr (VAR == VSE_VAR_WGSL) {
+    if (x < A.nx && y < A.ny && s < A.src_rows && A.src_rowflag[x]) {
+      const long id = (long)x * A.ny + y;
+      const int col = A.src_map[id];
+      if (col >= 0) {
+        const bool xe = x + 1 < A.nx, ye = y + 1 < A.ny;
+        const float rho = 0.25f * (A.rho[id] + (xe ? A.rho[id + A.ny] : 0.f) +
+                                   ((xe && ye) ? A.rho[id + A.ny + 1] : 0.f) + (ye ? A.rho[id + 1] : 0.f));
+        if (rho > 0.f) {
+          const float vy = A.vy[id] + A.src[(long)s * A.src_cols + col] * A.dt / rho;
+          A.vy[id] = vy;
+          const float vx = A.vx[id];
+          const unsigned u = __float_as_uint(vx * vx + vy * vy);
+          v2 = v2 > u ? v2 : u;
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void record_receivers(const StepArgs &A, int s) {
+  if (s >= A.trace_rows) return;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  for (int r = tid; r < A.n_rec; r += blockDim.x * blockDim.y) {
+    if (s + A.rec_gate >= A.rec_delay[r]) {
+      VSE_G float *tr = A.traces + (long)s * A.n_rec + r;
+      float acc = *tr;
+      for (int p = A.rec_beg[r]; p < A.rec_beg[r + 1]; p++) acc = acc + A.syy[A.rec_cell[p]];
+      *tr = acc;
+    }
+  }
+}
+
+// Velocity pass of one simulation, with its step's sparse work.
 template <int VAR, bool VISCO, int ORD>
-__global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A, int nty, int remap) {
+__global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A, int nty, int remap, int i) {
   const Tile2 t = tile2d(nty, remap);
-  const unsigned v2 = velocity2d_cell<VAR, VISCO, ORD>(A, t.tx * BX + threadIdx.y, t.ty * BY + threadIdx.x);
+  const int x = t.tx * BX + threadIdx.y, y = t.ty * BY + threadIdx.x;
+  const int s = *A.step + i;
+  unsigned v2 = velocity2d_cell<VAR, VISCO, ORD>(A, x, y, s);
+  source_cell<VAR>(A, x, y, s, v2);
+  if (blockIdx.x == 0) record_receivers(A, s);
   block_max_u32(v2, A.v2max);
+}
+
+__global__ __launch_bounds__(64) void k_advance(int *step, int n) {
+  if (threadIdx.x == 0) *step += n;
+}
+
+// every member's counter of a batch
+__global__ __launch_bounds__(256) void k_advance_batch(BatchArgs B, int n) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < B.n) *B.sparse[m].step += n;
 }
 
 // ---- batched: one launch over the tiles of many simulations --------------
@@ -318,25 +379,30 @@ __global__ __launch_bounds__(BX * BY) void k_stress2d_batch(BatchArgs B) {
   for (int k = 0; k < B.rows; k++) stress2d_cell<VAR, VISCO, ORD, LC>(A, x0 + k * BX, y);
 }
 
+// One launch over the tiles of every member; each member's step is
+// *step + i and its first tile records its receivers.
 template <int VAR, bool VISCO, int ORD>
-__global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B) {
+__global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B, int i) {
   const int b = xcd_tile(blockIdx.x, gridDim.x, 1);
-  const int i = batch_instance(B, b);
-  const StepArgs &A = B.args[i];
-  const int t = b - B.tile_beg[i];
+  const int m = batch_instance(B, b);
+  const StepArgs &A = B.args[m];
+  const int t = b - B.tile_beg[m];
   const int nyb = (A.ny + BY - 1) / BY;
   const int y = (t % nyb) * BY + threadIdx.x;
   const int x0 = (t / nyb) * BX * B.rows + threadIdx.y;
   unsigned v2 = 0;
+  const int s = *A.step + i;
   for (int k = 0; k < B.rows; k++) {
-    const unsigned u = velocity2d_cell<VAR, VISCO, ORD>(A, x0 + k * BX, y);
+    unsigned u = velocity2d_cell<VAR, VISCO, ORD>(A, x0 + k * BX, y, s);
+    source_cell<VAR>(A, x0 + k * BX, y, s, u);
     v2 = v2 > u ? v2 : u;
   }
+  if (t == 0) record_receivers(A, s);
   block_max_u32(v2, A.v2max);
 }
 
 template <int VAR, bool VISCO>
-hipError_t launch_ord(bool stress, int ord, const StepArgs &A, const KTune &t, hipStream_t st) {
+hipError_t launch_ord(bool stress, int ord, const StepArgs &A, const KTune &t, int i, hipStream_t st) {
   const dim3 block(BY, BX);
   const int nty = (A.ny + BY - 1) / BY, ntx = (A.nx + BX - 1) / BX;
   const dim3 grid((unsigned)(nty * ntx));
@@ -346,7 +412,7 @@ hipError_t launch_ord(bool stress, int ord, const StepArgs &A, const KTune &t, h
   case O:                                                                                                  \
     if (stress && l3) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O, 3>), grid, block, 0, st, A, nty, rm); \
     else if (stress) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O>), grid, block, 0, st, A, nty, rm);     \
-    else hipLaunchKernelGGL((k_velocity2d<VAR, VISCO, O>), grid, block, 0, st, A, nty, rm);               \
+    else hipLaunchKernelGGL((k_velocity2d<VAR, VISCO, O>), grid, block, 0, st, A, nty, rm, i);            \
     break;
   switch (ord) {
     VSE_CASE(2) VSE_CASE(3) VSE_CASE(4) VSE_CASE(5) VSE_CASE(6)
@@ -357,14 +423,14 @@ hipError_t launch_ord(bool stress, int ord, const StepArgs &A, const KTune &t, h
 }
 
 template <int VAR, bool VISCO>
-hipError_t launch_batch_ord(bool stress, int ord, const BatchArgs &B, int total_tiles, hipStream_t st) {
+hipError_t launch_batch_ord(bool stress, int ord, const BatchArgs &B, int total_tiles, int i, hipStream_t st) {
   dim3 block(BY, BX), grid(total_tiles);
   const bool l3 = VISCO && VAR == VSE_VAR_WGSL && B.L == 3;
 #define VSE_CASE(O)                                                                              \
   case O:                                                                                        \
     if (stress && l3) hipLaunchKernelGGL((k_stress2d_batch<VAR, VISCO, O, 3>), grid, block, 0, st, B); \
     else if (stress) hipLaunchKernelGGL((k_stress2d_batch<VAR, VISCO, O>), grid, block, 0, st, B); \
-    else hipLaunchKernelGGL((k_velocity2d_batch<VAR, VISCO, O>), grid, block, 0, st, B);         \
+    else hipLaunchKernelGGL((k_velocity2d_batch<VAR, VISCO, O>), grid, block, 0, st, B, i);      \
     break;
   switch (ord) {
     VSE_CASE(2) VSE_CASE(3) VSE_CASE(4) VSE_CASE(5) VSE_CASE(6)
@@ -374,19 +440,25 @@ hipError_t launch_batch_ord(bool stress, int ord, const BatchArgs &B, int total_
   return hipGetLastError();
 }
 
-hipError_t dispatch(bool stress, int variant, int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st) {
-  if (variant == VSE_VAR_TAICHI) return launch_ord<VSE_VAR_TAICHI, false>(stress, ord, A, t, st);
-  if (visco) return launch_ord<VSE_VAR_WGSL, true>(stress, ord, A, t, st);
-  return launch_ord<VSE_VAR_WGSL, false>(stress, ord, A, t, st);
+hipError_t dispatch(bool stress, int variant, int visco, int ord, const StepArgs &A, const KTune &t, int i,
+                    hipStream_t st) {
+  if (variant == VSE_VAR_TAICHI) return launch_ord<VSE_VAR_TAICHI, false>(stress, ord, A, t, i, st);
+  if (visco) return launch_ord<VSE_VAR_WGSL, true>(stress, ord, A, t, i, st);
+  return launch_ord<VSE_VAR_WGSL, false>(stress, ord, A, t, i, st);
 }
 
 }  // namespace
 
 hipError_t vse_launch_batch2d(bool stress, int variant, int visco, int ord, const BatchArgs &B, int total_tiles,
-                              hipStream_t st) {
-  if (variant == VSE_VAR_TAICHI) return launch_batch_ord<VSE_VAR_TAICHI, false>(stress, ord, B, total_tiles, st);
-  if (visco) return launch_batch_ord<VSE_VAR_WGSL, true>(stress, ord, B, total_tiles, st);
-  return launch_batch_ord<VSE_VAR_WGSL, false>(stress, ord, B, total_tiles, st);
+                              int i, hipStream_t st) {
+  if (variant == VSE_VAR_TAICHI) return launch_batch_ord<VSE_VAR_TAICHI, false>(stress, ord, B, total_tiles, i, st);
+  if (visco) return launch_batch_ord<VSE_VAR_WGSL, true>(stress, ord, B, total_tiles, i, st);
+  return launch_batch_ord<VSE_VAR_WGSL, false>(stress, ord, B, total_tiles, i, st);
+}
+
+hipError_t vse_launch_advance_batch(const BatchArgs &B, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance_batch, dim3((unsigned)((B.n + 255) / 256)), dim3(256), 0, st, B, n);
+  return hipGetLastError();
 }
 
 long vse_tiles2d(int nx, int ny, int rows) {
@@ -394,8 +466,13 @@ long vse_tiles2d(int nx, int ny, int rows) {
 }
 
 hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st) {
-  return dispatch(true, variant, visco, ord, A, t, st);
+  return dispatch(true, variant, visco, ord, A, t, 0, st);
 }
-hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st) {
-  return dispatch(false, variant, visco, ord, A, t, st);
+hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, int i,
+                                 hipStream_t st) {
+  return dispatch(false, variant, visco, ord, A, t, i, st);
+}
+hipError_t vse_launch_advance(int *step, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, step, n);
+  return hipGetLastError();
 }

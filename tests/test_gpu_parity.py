@@ -98,6 +98,17 @@ def test_2d_partial_emitters_wgsl_stride():
     _compare(Simulator(cfg), "wgsl", 120, att=attenuation_from_config(cfg))
 
 
+def test_source_cell_listed_twice_is_rejected():
+    # the reference's source map holds one column per cell
+    from taichi_sim_amd._lib import VSEError
+    sim = Simulator(workloads.elastic_2d_point(n=64, steps=20, n_rec=4))
+    g = VSESim.from_simulator(sim, variant="wgsl")
+    cells = np.array([100, 200, 100], dtype=np.int32)
+    with pytest.raises(VSEError, match="listed twice"):
+        g.set_sources(cells, np.zeros(3, np.int32), np.ones((g.n_rows, 1), np.float32))
+    g.close()
+
+
 @pytest.mark.parametrize("visco", [True, False])
 def test_3d_block(visco):
     cfg = workloads.visco_3d_block(n=56, steps=80, n_incl=4, n_rec_side=4)

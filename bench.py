@@ -14,6 +14,12 @@ as one batch (one launch per pass per step), against the same 64 simulations
 run one after another as the reference runs its emission laws.
 value = 3000-step shots per second.
 
+``--workload 2d`` measures the 2D path on its headline configuration: 2D
+viscoelastic (Zener L=3) 2048^2 incl. 20-cell CPML, steel plate with 16
+seeded voids, 64-element array with a seeded focusing law, 64 receivers.
+value = Gcell-updates/s; roofline of the 2D stress pass; the CPU oracle on a
+bounded sample beside it.
+
 ``--workload adjoint`` measures the gradient: 2D viscoelastic 1024^2, L=3,
 3000 steps, 8 FMC shots, observed traces from a true model with 8 seeded
 Gaussian velocity anomalies (+-5%, 4-30 mm under the array), gradient at the homogeneous starting
@@ -58,6 +64,81 @@ def bytes_per_cell(g, L):
     return step, stress, velocity
 
 
+def bytes_per_cell_2d(g, L):
+    """2D: every state field read and written once, materials read once, psi
+    (4 slots per axis, split between the passes) only inside its band."""
+    n = g.shape
+    band = sum(32.0 * g.band_cells()[a] / n[a] for a in range(2))      # 4 psi x (r+w) x 4 B per axis
+    step = 4.0 * (2 * (2 + 3 + 3 * L) + 3) + band
+    stress = 4.0 * (2 + 3 + 3 * L + 3 + 3 + 3 * L) + band / 2          # v, sigma, r, rho/cp/cs in; sigma, r out
+    velocity = 4.0 * (3 + 2 + 1 + 2) + band / 2                         # sigma, v, rho in; v out
+    return step, stress, velocity
+
+
+def bench_2d(args):
+    """2D viscoelastic time loop at 2048^2 (see module doc)."""
+    from taichi_sim_amd import workloads
+    from taichi_sim_amd.backend import attenuation_from_config
+    from taichi_sim_amd.engine import VSESim
+    from taichi_sim_amd.sim_support.simulator import Simulator
+
+    n = args.n2d
+    cfg = workloads.visco_2d_array(n=n, steps=args.warmup + args.steps)
+    sim = Simulator(cfg)
+    att = attenuation_from_config(cfg)
+    g = VSESim.from_simulator(sim, variant="wgsl", att=att)
+    ncell = int(np.prod(g.shape))
+    print(f"# setup done: grid {g.shape}, {g.device_bytes() / 2**30:.2f} GiB on {g.device_name()}",
+          file=sys.stderr, flush=True)
+    if args.warmup:
+        g.run(args.warmup, check_every=args.warmup)
+    g.reset()
+    g.run(min(args.steps, 256), check_every=256)           # capture the graph outside the timed region
+    g.reset()
+    t0 = time.perf_counter()
+    dev_ms = g.run(args.steps, check_every=500, timed=True)
+    wall = time.perf_counter() - t0
+    gcell = ncell * args.steps / wall / 1e9
+    k_ms = g.profile(args.profile_steps)
+    b_step, b_stress, b_vel = bytes_per_cell_2d(g, g.n_sls)
+    achieved = b_stress * ncell / (k_ms[0] * 1e-3) / 1e9
+    g.close()
+    print(f"# timed {args.steps} steps: wall {wall:.3f} s, device {dev_ms / 1e3:.3f} s", file=sys.stderr, flush=True)
+
+    cpu = None
+    if args.cpu_steps > 0:
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        steps_cpu = 10 * args.cpu_steps
+        o = oracle.Oracle(sim, variant="wgsl", att=att, n_threads=threads)
+        t = time.perf_counter()
+        o.run(steps_cpu)
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": round(ncell * steps_cpu / cpu_s / 1e9, 6), "unit": "Gcell-upd/s", "cores": threads,
+               "kind": "port", "sample": f"{steps_cpu} steps of the same {n}^2 grid, FP32 C oracle "
+                                         f"(reference update order, OpenMP), {cpu_s:.1f} s"}
+        del o
+
+    line = {
+        "metric": "Gcell-upd/s (2D viscoelastic Zener L=3 FP32 forward)",
+        "value": round(gcell, 4), "unit": "Gcell-upd/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"2D viscoelastic Zener FD {n}^2 (incl. 20-cell CPML), L=3 Q=50, steel plate with "
+                               f"16 seeded voids, 64-element array with a focusing law, 64 receivers",
+                   "grid": [n, n], "n_sls": 3, "fd_order": 2, "parallelism": "single GPU"},
+        "hbm_frac_step": round(gcell * 1e9 * b_step / (HBM_PEAK_GBS * 1e9), 4),
+        "bytes_per_cell_step": round(b_step, 2),
+        "kernel_ms": {"stress": round(k_ms[0], 4), "velocity": round(k_ms[1], 4),
+                      "step_counter": round(k_ms[2], 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "stress2d", "bytes_per_cell": round(b_stress, 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,7 +147,8 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU-oracle sample (0: skip)")
-    ap.add_argument("--workload", choices=("3d", "shots", "adjoint"), default="3d")
+    ap.add_argument("--workload", choices=("3d", "2d", "shots", "adjoint"), default="3d")
+    ap.add_argument("--n2d", type=int, default=2048, help="grid size (--workload 2d)")
     ap.add_argument("--fd-dirs", type=int, default=2, help="finite-difference directions (--workload adjoint)")
     ap.add_argument("--store-all", type=int, default=1,
                     help="also time the gradient with every state stored, no recompute (--workload adjoint)")
@@ -92,6 +174,8 @@ def main():
         sys.exit(0 if rc == 0 else 1)
     if args.workload == "shots":
         return bench_shots(args)
+    if args.workload == "2d":
+        return bench_2d(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
     from taichi_sim_amd import workloads, autotune

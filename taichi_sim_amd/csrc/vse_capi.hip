@@ -45,6 +45,10 @@ struct vse_sim {
   ZenerK zk{};
   unsigned *ctrl = nullptr;       // [0] = v2max bits, [1] = step counter
   unsigned *host_ctrl = nullptr;  // pinned mirror
+  // 2D: running max of |v|^2 per velocity workgroup, merged into ctrl[0]
+  // (fold_v2) before the host reads it
+  unsigned *v2slots = nullptr;
+  int n_slots = 0;
   // sources
   int n_src_pts = 0, src_cols = 1;
   int *src_cell = nullptr, *src_col = nullptr, *src_map = nullptr;
@@ -152,6 +156,12 @@ int vse_create(const vse_desc *desc, vse_sim **out) {
   if (!rc) rc = s->alloc(&s->ctrl, 2);
   if (!rc) rc = s->hip(hipHostMalloc((void **)&s->host_ctrl, 2 * sizeof(unsigned), hipHostMallocDefault),
                        "hipHostMalloc");
+  if (!rc && d.ndim == 2) {
+    // one slot per workgroup of the widest 2D velocity launch (one BX-row group per tile)
+    const long n = vse_tiles2d(d.nx, d.ny, 1);
+    if (n > 0x7fffffffL) rc = s->fail(VSE_ERR_ARG, "grid too large");
+    else { s->n_slots = (int)n; rc = s->alloc(&s->v2slots, s->n_slots); }
+  }
   // Zener constants per mechanism, float32 in the reference's operation order
   // (wgsl2d :881-894): phi = dt(1-alpha)(1/tau_sig)/sum_alpha, h = 0.5 dt/tau_sig,
   // m = 1/(1+h); identical to the per-cell values the reference recomputes.
@@ -354,6 +364,8 @@ int vse_reset(vse_sim *s) {
     if (s->psi[p]) rc = s->hip(hipMemsetAsync(s->psi[p], 0, std::max<long>(s->psi_elems(p), 1) * sizeof(float), s->stream), "reset psi");
   if (!rc) rc = s->hip(hipMemsetAsync(s->traces, 0, (size_t)s->d.n_rows * std::max(s->n_rec, 1) * sizeof(float), s->stream), "reset traces");
   if (!rc) rc = s->hip(hipMemsetAsync(s->ctrl, 0, 2 * sizeof(unsigned), s->stream), "reset ctrl");
+  if (!rc && s->v2slots)
+    rc = s->hip(hipMemsetAsync(s->v2slots, 0, (size_t)s->n_slots * sizeof(unsigned), s->stream), "reset slots");
   if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "reset");
   return rc;
 }
@@ -379,6 +391,7 @@ static void make_args(vse_sim *s, StepArgs &A, SparseArgs &S) {
   A.rho = s->rho; A.cp = s->cp; A.cs = s->cs;
   A.zk = s->zk; A.L = d.n_sls;
   A.v2max = s->ctrl;
+  A.v2slots = s->v2slots; A.n_slots = s->n_slots;
   A.step = (const int *)(s->ctrl + 1);
   A.src_map = s->src_map; A.src = s->src; A.src_cols = s->src_cols; A.src_rows = d.n_rows;
   A.src_rowflag = s->src_rowflag;
@@ -441,8 +454,14 @@ static int get_graph(vse_sim *s, int steps, const StepArgs &A, const SparseArgs 
   return rc;
 }
 
+// ctrl[0] = max(ctrl[0], every workgroup slot) (2D; 3D kernels update ctrl[0])
+static int fold_v2(vse_sim *s) {
+  return s->v2slots ? s->hip(vse_launch_fold_v2(s->v2slots, s->n_slots, s->ctrl, s->stream), "fold v2") : 0;
+}
+
 static int check_stable(vse_sim *s) {
-  int rc = s->hip(hipMemcpyAsync(s->host_ctrl, s->ctrl, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream), "read v2max");
+  int rc = fold_v2(s);
+  if (!rc) rc = s->hip(hipMemcpyAsync(s->host_ctrl, s->ctrl, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream), "read v2max");
   if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "sync");
   if (rc) return rc;
   if (s->host_ctrl[0] >= 0x7f800000u) {
@@ -585,7 +604,8 @@ int vse_read_memory(vse_sim *s, int comp, float *out) {
 
 int vse_read_v2max(vse_sim *s, float *out) {
   if (check_handle(s) || !out) return VSE_ERR_ARG;
-  int rc = s->hip(hipMemcpyAsync(s->host_ctrl, s->ctrl, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream), "read ctrl");
+  int rc = fold_v2(s);
+  if (!rc) rc = s->hip(hipMemcpyAsync(s->host_ctrl, s->ctrl, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream), "read ctrl");
   if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "sync");
   if (!rc) memcpy(out, &s->host_ctrl[0], 4);
   return rc;

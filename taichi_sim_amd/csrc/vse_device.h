@@ -28,6 +28,18 @@ __device__ __forceinline__ float cpml_stretch(const float *a, const float *b, co
   return D / k[j] + pn;
 }
 
+// Max over a wavefront, then one atomicMax per wavefront into its
+// workgroup's own slot.  The result is not used, so no wavefront waits on
+// it, and no two workgroups share an address (a single shared word costs each
+// workgroup a device-scope round trip before it can retire).
+__device__ __forceinline__ void wave_max_to_slot(unsigned v, unsigned *slot) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned o = __shfl_xor(v, off, 64);
+    v = v > o ? v : o;
+  }
+  if (((threadIdx.x + threadIdx.y * blockDim.x) & 63) == 0 && v) atomicMax(slot, v);
+}
+
 // Max over a 64-wide wavefront, then at most one atomicMax per block.  For |v|^2 >= 0
 // the IEEE bit pattern orders like the value; NaN sorts above +inf, so the
 // running max also flags NaN (the reference's `vsn2 > 1e38` test does not).

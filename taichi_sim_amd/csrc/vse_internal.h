@@ -73,6 +73,10 @@ struct StepArgs {
   ZenerK zk;
   int L;
   unsigned *v2max;                   // running max of |v|^2 as float bits
+  // 2D: the velocity kernels keep the running max per workgroup in
+  // v2slots[0..n_slots) instead; vse_launch_fold_v2 merges them into v2max
+  unsigned *v2slots;
+  int n_slots;
   const VSE_G int *step;             // device step counter (0-based)
   // Taichi-variant source (inline in the velocity update)
   const VSE_G int *src_map;          // full grid, -1 or column
@@ -213,4 +217,5 @@ hipError_t vse_launch_batch2d(bool stress, int variant, int visco, int ord, cons
 hipError_t vse_launch_advance_batch(const BatchArgs &B, int n, hipStream_t st);
 hipError_t vse_launch_sparse_batch(const BatchArgs &B, hipStream_t st);
 hipError_t vse_launch_batch_ctrl(const BatchArgs &B, unsigned *out, hipStream_t st);
+hipError_t vse_launch_fold_v2(const unsigned *slots, int n, unsigned *v2max, hipStream_t st);
 long vse_tiles2d(int nx, int ny, int rows);

@@ -339,7 +339,7 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A, int nty, int
   unsigned v2 = velocity2d_cell<VAR, VISCO, ORD>(A, x, y, s);
   source_cell<VAR>(A, x, y, s, v2);
   if (blockIdx.x == 0) record_receivers(A, s);
-  block_max_u32(v2, A.v2max);
+  wave_max_to_slot(v2, A.v2slots + blockIdx.x);
 }
 
 __global__ __launch_bounds__(64) void k_advance(int *step, int n) {
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B, int i
     v2 = v2 > u ? v2 : u;
   }
   if (t == 0) record_receivers(A, s);
-  block_max_u32(v2, A.v2max);
+  wave_max_to_slot(v2, A.v2slots + t);
 }
 
 template <int VAR, bool VISCO>

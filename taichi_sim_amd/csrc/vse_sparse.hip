@@ -70,18 +70,53 @@ __global__ __launch_bounds__(1024) void k_sparse(SparseArgs S) { sparse_body(S);
 // one block per simulation of a batch
 __global__ __launch_bounds__(1024) void k_sparse_batch(BatchArgs B) { sparse_body(B.sparse[blockIdx.x]); }
 
-// (v2max bits, step) of every simulation of a batch into out[2i], out[2i+1]
+// Largest of a workgroup's values (every thread calls it; thread 0 gets it).
+__device__ __forceinline__ unsigned block_reduce_max(unsigned v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned o = __shfl_xor(v, off, 64);
+    v = v > o ? v : o;
+  }
+  __shared__ unsigned w[16];
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  unsigned m = 0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x + 63) / 64; k++) m = m > w[k] ? m : w[k];
+  return m;
+}
+
+// (running max of |v|^2, step) of member m of a batch into out[2m], out[2m+1];
+// one workgroup per member, merging the member's workgroup slots
 __global__ __launch_bounds__(256) void k_batch_ctrl(BatchArgs B, unsigned *out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B.n) return;
-  out[2 * i] = *B.sparse[i].v2max;
-  out[2 * i + 1] = (unsigned)*B.sparse[i].step;
+  const int m = blockIdx.x;
+  const StepArgs &A = B.args[m];
+  unsigned v = 0;
+  for (int k = threadIdx.x; k < A.n_slots; k += blockDim.x) v = v > A.v2slots[k] ? v : A.v2slots[k];
+  v = block_reduce_max(v);
+  if (threadIdx.x == 0) {
+    const unsigned g = *B.sparse[m].v2max;
+    out[2 * m] = g > v ? g : v;
+    out[2 * m + 1] = (unsigned)*B.sparse[m].step;
+  }
+}
+
+// v2max = max(v2max, slots[0..n))
+__global__ __launch_bounds__(256) void k_fold_v2(const unsigned *slots, int n, unsigned *v2max) {
+  unsigned v = 0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) v = v > slots[k] ? v : slots[k];
+  v = block_reduce_max(v);
+  if (threadIdx.x == 0 && v > *v2max) *v2max = v;
 }
 
 }  // namespace
 
+hipError_t vse_launch_fold_v2(const unsigned *slots, int n, unsigned *v2max, hipStream_t st) {
+  hipLaunchKernelGGL(k_fold_v2, dim3(1), dim3(256), 0, st, slots, n, v2max);
+  return hipGetLastError();
+}
+
 hipError_t vse_launch_batch_ctrl(const BatchArgs &B, unsigned *out, hipStream_t st) {
-  hipLaunchKernelGGL(k_batch_ctrl, dim3((B.n + 255) / 256), dim3(256), 0, st, B, out);
+  hipLaunchKernelGGL(k_batch_ctrl, dim3(B.n), dim3(256), 0, st, B, out);
   return hipGetLastError();
 }
 

@@ -194,7 +194,7 @@ def main():
 
     # launch shapes from the autotuner (before the timed region; it advances
     # the state, so the simulation restarts from zero afterwards)
-    cands = [autotune.PLAIN] + [(ty, nt, xc, rm) for ty in (1, 4) for nt in (0, 1) for xc in (4, 8, 16)
+    cands = [autotune.PLAIN] + [(ty, nt, xc, rm) for ty in autotune.TY for nt in (0, 1) for xc in (4, 8, 16)
                                 for rm in (0, 1)]
     tuned = {}
     for math in ("fast", "strict"):

@@ -126,7 +126,7 @@ int vse_profile(vse_sim *s, int n_steps, float *ms);
 /* Launch shape of a kernel (kernel 0 = stress, 1 = velocity).  2D: only remap
  * applies (1, the default: each XCD takes a contiguous band of 64x4 tiles;
  * 0: round-robin order).  3D: workgroup
- * tile of 64 (z) x ty (y) cells, ty in {1,2,4,8}; nt = 1 streams the fields
+ * tile of 64 (z) x ty (y) cells, ty in {1,4,8}; nt = 1 streams the fields
  * touched only at the own cell with non-temporal loads/stores; each
  * workgroup marches through xchunk x-planes (0 = the plain kernel, one cell
  * per thread on a (z, y, x) grid, where ty and nt do not apply);

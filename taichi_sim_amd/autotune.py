@@ -14,7 +14,7 @@ import os
 
 DEFAULT_CACHE = os.environ.get("VSE_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                             "tuning_cache.json"))
-TY = (1, 4)
+TY = (1, 4, 8)
 NT = (0, 1)
 XCHUNK = (1, 2, 4, 8, 16, 32)
 PLAIN = (4, 0, 0, 0)      # xchunk 0: one cell per thread

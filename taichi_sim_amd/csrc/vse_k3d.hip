@@ -155,12 +155,13 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
     if constexpr (PRE) {
 #pragma unroll
       for (int c = 0; c < ORD; c++) {
-        nvy_p[c] = vyP.ld(col + c * sy); nvy_m[c] = vyP.ld(col - (c + 1) * sy);
-        nvz_p[c] = vzP.ld(col + c); nvz_m[c] = vzP.ld(col - c - 1);
-        nxy_p[c] = vxP.ld(col + (c + 1) * sy); nxy_m[c] = vxP.ld(col - c * sy);
-        nxz_p[c] = vxP.ld(col + c + 1); nxz_m[c] = vxP.ld(col - c);
-        nzy_p[c] = vzP.ld(col + (c + 1) * sy); nzy_m[c] = vzP.ld(col - c * sy);
-        nyz_p[c] = vyP.ld(col + c + 1); nyz_m[c] = vyP.ld(col - c);
+        // the own cell of plane x is already in the x windows
+        nvy_p[c] = c == 0 ? wvy[ORD] : vyP.ld(col + c * sy); nvy_m[c] = vyP.ld(col - (c + 1) * sy);
+        nvz_p[c] = c == 0 ? wvz[ORD] : vzP.ld(col + c); nvz_m[c] = vzP.ld(col - c - 1);
+        nxy_p[c] = vxP.ld(col + (c + 1) * sy); nxy_m[c] = c == 0 ? wvx[ORD - 1] : vxP.ld(col - c * sy);
+        nxz_p[c] = vxP.ld(col + c + 1); nxz_m[c] = c == 0 ? wvx[ORD - 1] : vxP.ld(col - c);
+        nzy_p[c] = vzP.ld(col + (c + 1) * sy); nzy_m[c] = c == 0 ? wvz[ORD] : vzP.ld(col - c * sy);
+        nyz_p[c] = vyP.ld(col + c + 1); nyz_m[c] = c == 0 ? wvy[ORD] : vyP.ld(col - c);
       }
       m_r0y = rhoP.ld(col + sy); m_r1y = rho1P.ld(col + sy); m_c0y = csP.ld(col + sy); m_c1y = cs1P.ld(col + sy);
       m_r0z = rhoP.ld(col + 1); m_r1z = rho1P.ld(col + 1); m_c0z = csP.ld(col + 1); m_c1z = cs1P.ld(col + 1);
@@ -401,8 +402,9 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
       float zy_p[ORD], zy_m[ORD], zz_p[ORD], zz_m[ORD];      // vz: syz along y (D-), szz along z (D+)
 #pragma unroll
       for (int c = 0; c < ORD; c++) {
-        xy_p[c] = sxyP.ld(col + c * sy); xy_m[c] = sxyP.ld(col - (c + 1) * sy);
-        xz_p[c] = sxzP.ld(col + c); xz_m[c] = sxzP.ld(col - c - 1);
+        // the own cell of plane x is already in the x windows
+        xy_p[c] = c == 0 ? wxy[ORD - 1] : sxyP.ld(col + c * sy); xy_m[c] = sxyP.ld(col - (c + 1) * sy);
+        xz_p[c] = c == 0 ? wxz[ORD - 1] : sxzP.ld(col + c); xz_m[c] = sxzP.ld(col - c - 1);
         yy_p[c] = syyP.ld(col + (c + 1) * sy); yy_m[c] = syyP.ld(col - c * sy);
         yz_p[c] = syzP.ld(col + c); yz_m[c] = syzP.ld(col - c - 1);
         zy_p[c] = syzP.ld(col + c * sy); zy_m[c] = syzP.ld(col - (c + 1) * sy);
@@ -774,11 +776,13 @@ hipError_t launch_tuned(bool stress, int visco, const StepArgs &A, const KTune &
     }
     return hipGetLastError();
   }
-  // compiled shapes: TY in {1, 4} x non-temporal on/off
+  // compiled shapes: TY in {1, 4, 8} x non-temporal on/off
   if (t.ty == 1) return t.nt ? launch_one<ORD, 1, true, FAST>(stress, visco, A, t.xchunk, t.remap, st)
                              : launch_one<ORD, 1, false, FAST>(stress, visco, A, t.xchunk, t.remap, st);
   if (t.ty == 4) return t.nt ? launch_one<ORD, 4, true, FAST>(stress, visco, A, t.xchunk, t.remap, st)
                              : launch_one<ORD, 4, false, FAST>(stress, visco, A, t.xchunk, t.remap, st);
+  if (t.ty == 8) return t.nt ? launch_one<ORD, 8, true, FAST>(stress, visco, A, t.xchunk, t.remap, st)
+                             : launch_one<ORD, 8, false, FAST>(stress, visco, A, t.xchunk, t.remap, st);
   return hipErrorInvalidValue;
 }
 

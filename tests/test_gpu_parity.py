@@ -124,7 +124,7 @@ def test_3d_launch_shapes_bit_identical():
     g = VSESim.from_simulator(sim, variant="wgsl", att=att)
     ref = None
     for ty, nt, xc, rm in [(4, 0, 0, 0), (4, 0, 1, 0), (1, 0, 7, 1), (1, 1, 16, 0), (4, 1, 3, 1), (4, 1, 64, 1),
-                           (1, 0, 1000, 0)]:
+                           (1, 0, 1000, 0), (8, 0, 8, 1), (8, 1, 5, 0)]:
         g.set_tuning(0, ty, nt, xc, rm)
         g.set_tuning(1, ty, 1 - nt, xc // 2, 1 - rm)
         g.reset()

@@ -31,8 +31,8 @@
 // is two launches in either direction.
 // The forward values the adjoint needs (raw derivatives, CPML-stretched
 // derivatives, memory variables) are recomputed from the stored levels n
-// and n+1 with the forward's own FP32 arithmetic.  Each step's gradient
-// terms (formed in FP64) are added to FP32 per-cell partials; k_grad_flush
+// and n+1 in FP32 (the differences with f32(C/d), see sumT).  Each step's
+// gradient terms are formed in FP32 and added to FP32 per-cell partials; k_grad_flush
 // adds the partials into the FP64 gradients after every step n divisible by
 // VSE_GRAD_FLUSH, so the time accumulation is FP64 at that granularity while
 // every step moves 4-byte partials instead of 8-byte totals.
@@ -52,21 +52,17 @@ struct ZCell {
 // is bit-identical to the in-place kernels; elsewhere phi = rs * dt(1-alpha)
 // / (tau_sigma sum_alpha), h = rs * dt / (2 tau_sigma), m = 1/(1+h) with a
 // hardware reciprocal.  The forward and the adjoint share this function, so
-// the adjoint differentiates exactly the arithmetic the forward ran.
+// the adjoint differentiates exactly the arithmetic the forward ran.  No
+// branch: x * 1.0f is x, and m selects the stored value at s = 1.
 __device__ __forceinline__ ZCell zener_cell(const DiffArgs &A, int l, float rs) {
   ZCell z;
-  if (rs == 1.0f) {
-    z.phi_p = A.zk.phi_p[l]; z.phi_s = A.zk.phi_s[l];
-    z.h_p = A.zk.h_p[l]; z.h_s = A.zk.h_s[l];
-    z.m_p = A.zk.m_p[l]; z.m_s = A.zk.m_s[l];
-  } else {
-    z.phi_p = A.zk.phi_p[l] * rs;
-    z.phi_s = A.zk.phi_s[l] * rs;
-    z.h_p = A.zk.h_p[l] * rs;
-    z.h_s = A.zk.h_s[l] * rs;
-    z.m_p = __builtin_amdgcn_rcpf(1.0f + z.h_p);
-    z.m_s = __builtin_amdgcn_rcpf(1.0f + z.h_s);
-  }
+  const bool one = rs == 1.0f;
+  z.phi_p = A.zk.phi_p[l] * rs;
+  z.phi_s = A.zk.phi_s[l] * rs;
+  z.h_p = A.zk.h_p[l] * rs;
+  z.h_s = A.zk.h_s[l] * rs;
+  z.m_p = one ? A.zk.m_p[l] : __builtin_amdgcn_rcpf(1.0f + z.h_p);
+  z.m_s = one ? A.zk.m_s[l] : __builtin_amdgcn_rcpf(1.0f + z.h_s);
   return z;
 }
 
@@ -151,12 +147,24 @@ __device__ __forceinline__ Taps<ORD> tapsT(const DiffArgs &A, const VSE_G float 
   return t;
 }
 
+// The adjoint kernels multiply by f32(C/d) instead of dividing by d: a
+// rounding-level change (the gradient is checked against FP64 autograd and
+// finite differences, not bit for bit), and two fewer IEEE divisions per
+// operand pair.  The forward kernels keep the reference's C (a - b) / d.
 template <int ORD, int AX>
 __device__ __forceinline__ float sumT(const DiffArgs &A, const Taps<ORD> &t) {
-  const float d = AX == 0 ? A.dx : A.dy;
   float acc = 0.f;
 #pragma unroll
-  for (int c = 0; c < ORD; c++) acc += A.C[c] * (t.a[c] - t.b[c]) / d;
+  for (int c = 0; c < ORD; c++) acc += (t.a[c] - t.b[c]) * A.Cd[AX][c];
+  return acc;
+}
+
+// The forward's staggered difference as the adjoint recomputes it.
+template <int ORD, int AX>
+__device__ __forceinline__ float diffA(const DiffArgs &A, const Taps<ORD> &t) {
+  float acc = 0.f;
+#pragma unroll
+  for (int c = 0; c < ORD; c++) acc += (t.a[c] - t.b[c]) * A.Cd[AX][c];
   return acc;
 }
 
@@ -514,35 +522,35 @@ __global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
     float eb = 0.f;
     if (inY && rhoX > 0.f) {
       eb = vxb * dt / rhoX;
-      const float dx1 = diff<ORD>(A, t1, A.dx), dx2 = diff<ORD>(A, t2, A.dy);
+      const float dx1 = diffA<ORD, 0>(A, t1), dx2 = diffA<ORD, 1>(A, t2);
       // e = D/k + psi', psi' as stored in level n+1
       const float e5 = ex >= 0 ? dx1 / A.ax[0].k[x - off] + p5 : dx1;
       const float e6 = ey >= 0 ? dx2 / A.ax[1].k[y - off] + p6 : dx2;
-      gx = gx + (float)(-((double)(e5 + e6) * dt / ((double)rhoX * rhoX)) * vxb);
+      gx = gx + -(e5 + e6) * eb / rhoX;
     }
     a5b = ex >= 0 ? cpml_back(A.ax[0].a, A.ax[0].b, A.ax[0].k, x - off, b5, eb, b5) : eb;
     a6b = ey >= 0 ? cpml_back(A.ax[1].a, A.ax[1].b, A.ax[1].k, y - off, b6, eb, b6) : eb;
   }
-  double ty = 0.0;
+  float ty = 0.f;
   if (inY) {
     float eb = 0.f;
     const float rhoY = 0.25f * (r00 + r10 + r11 + r01);
     if (rhoY > 0.f) {
       eb = vyb * dt / rhoY;
-      const float dy1 = diff<ORD>(A, t3, A.dx), dy2 = diff<ORD>(A, t4, A.dy);
+      const float dy1 = diffA<ORD, 0>(A, t3), dy2 = diffA<ORD, 1>(A, t4);
       const float e7 = ex >= 0 ? dy1 / A.ax[0].kh[x - off] + p7 : dy1;
       const float e8 = ey >= 0 ? dy2 / A.ax[1].kh[y - off] + p8 : dy2;
-      ty += -((double)(e7 + e8) * dt / ((double)rhoY * rhoY)) * vyb;
+      ty += -(e7 + e8) * eb / rhoY;
     }
     a7b = ex >= 0 ? cpml_back(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, x - off, b7, eb, b7) : eb;
     a8b = ey >= 0 ? cpml_back(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, y - off, b8, eb, b8) : eb;
   }
   // the source term dt*src/rho_vy of step n
-  if (src_on && rho_src > 0.f) ty += -((double)sval * dt / ((double)rho_src * rho_src)) * vyb;
+  if (src_on && rho_src > 0.f) ty += -(sval * dt / rho_src) * vyb / rho_src;
 
   // ---- stores
   if (inXY) A.gf[VSE_GR_RHO_VX][id] = gx;
-  if (inY || src_on) A.gf[VSE_GR_RHO_VY][id] = gy + (float)ty;
+  if (inY || src_on) A.gf[VSE_GR_RHO_VY][id] = gy + ty;
   if (inX) {
     if (ex >= 0) B.psi[SL_SXXX][px] = b5;
     if (ey >= 0) B.psi[SL_SXYY][py] = b6;
@@ -651,11 +659,11 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
   const float P = bsxx + sumT<ORD, 0>(A, u4);
   const float Q = bsyy + sumT<ORD, 1>(A, u7);
   const float R = bsxy + sumT<ORD, 1>(A, u5) + sumT<ORD, 0>(A, u6);
-  double gs = 0.0;
+  float gs = 0.f;
   float a1b = 0.f, a2b = 0.f, a3b = 0.f, a4b = 0.f;
   if (inN) {
     float pn;
-    const float d1 = diff<ORD>(A, t1, A.dx), d2 = diff<ORD>(A, t2, A.dy);
+    const float d1 = diffA<ORD, 0>(A, t1), d2 = diffA<ORD, 1>(A, t2);
     const float e1 = ex >= 0 ? cpml_step(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, x - off, q1, d1, pn) : d1;
     const float e2 = ey >= 0 ? cpml_step(A.ax[1].a, A.ax[1].b, A.ax[1].k, y - off, q2, d2, pn) : d2;
     const float rho_h = 0.5f * (rhoN + rho0);
@@ -685,18 +693,17 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
       e1b += Rxx * z.phi_p * z.m_p + 0.5f * Ryy * z.phi_s * z.m_s;
       e2b += Rxx * z.phi_p * z.m_p - 0.5f * Ryy * z.phi_s * z.m_s;
       // d r'/d s = (m/s)(r h - X phi + r' h)
-      gs += (double)Rxx * ((double)z.m_p * rs) * ((double)oxx * z.h_p - (double)Xp * z.phi_p + (double)nxx * z.h_p);
-      gs += (double)Ryy * ((double)z.m_s * rs) * ((double)oyy * z.h_s - (double)Xs * z.phi_s + (double)nyy * z.h_s);
+      gs += Rxx * (z.m_p * rs) * (oxx * z.h_p - Xp * z.phi_p + nxx * z.h_p);
+      gs += Ryy * (z.m_s * rs) * (oyy * z.h_s - Xs * z.phi_s + nyy * z.h_s);
     }
-    gl = gl + (float)(((double)P + Q) * ((double)e1 + e2 + 0.5 * sum_xx) * dt);
-    gm = gm + (float)(((double)P * (2.0 * e1 + 0.5 * sum_xx + sum_yy) +
-                       (double)Q * (2.0 * e2 + 0.5 * sum_xx - sum_yy)) * dt);
+    gl = gl + (P + Q) * (e1 + e2 + 0.5f * sum_xx) * dt;
+    gm = gm + (P * (2.0f * e1 + 0.5f * sum_xx + sum_yy) + Q * (2.0f * e2 + 0.5f * sum_xx - sum_yy)) * dt;
     a1b = ex >= 0 ? cpml_back(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, x - off, b1, e1b, b1) : e1b;
     a2b = ey >= 0 ? cpml_back(A.ax[1].a, A.ax[1].b, A.ax[1].k, y - off, b2, e2b, b2) : e2b;
   }
   if (inS) {
     float pn;
-    const float d3 = diff<ORD>(A, t3, A.dx), d4 = diff<ORD>(A, t4, A.dy);
+    const float d3 = diffA<ORD, 0>(A, t3), d4 = diffA<ORD, 1>(A, t4);
     const float e3 = ex >= 0 ? cpml_step(A.ax[0].a, A.ax[0].b, A.ax[0].k, x - off, q3, d3, pn) : d3;
     const float e4 = ey >= 0 ? cpml_step(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, y - off, q4, d4, pn) : d4;
     const float rho_h = 0.5f * (rhoS + rho0);
@@ -715,9 +722,9 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
       const float Rl = (LC > 0 ? pb[l] : B.r[2][l * nc + id]) + Sb;
       B.r[2][l * nc + id] = Sb + Rl * ((1.0f - z.h_s) * z.m_s);
       eb += Rl * z.phi_s * z.m_s;
-      gs += (double)Rl * ((double)z.m_s * rs) * ((double)o * z.h_s - (double)X * z.phi_s + (double)n * z.h_s);
+      gs += Rl * (z.m_s * rs) * (o * z.h_s - X * z.phi_s + n * z.h_s);
     }
-    gmxy = gmxy + (float)((double)R * ((double)X + 0.5 * sum_xy) * dt);
+    gmxy = gmxy + R * (X + 0.5f * sum_xy) * dt;
     a3b = ex >= 0 ? cpml_back(A.ax[0].a, A.ax[0].b, A.ax[0].k, x - off, b3, eb, b3) : eb;
     a4b = ey >= 0 ? cpml_back(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, y - off, b4, eb, b4) : eb;
   }
@@ -737,7 +744,7 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
     if (ey >= 0) B.psi[SL_VXY][py] = b4;
     A.gf[VSE_GR_MU_XY][id] = gmxy;
   }
-  if (inN || inS) A.gf[VSE_GR_TAU][id] = gt + (float)gs;
+  if (inN || inS) A.gf[VSE_GR_TAU][id] = gt + gs;
   A.abar[0][id] = a1b;
   A.abar[1][id] = a2b;
   A.abar[2][id] = a3b;

@@ -1055,7 +1055,11 @@ static void make_diff(vse_grad *g, DiffArgs &A) {
   memset(&A, 0, sizeof(A));
   A.nx = d.nx; A.ny = d.ny; A.ord = d.ord; A.ncell = s->ncell;
   A.dx = d.dx; A.dy = d.dy; A.dt = d.dt;
-  for (int c = 0; c < d.ord; c++) A.C[c] = d.coefs[c];
+  for (int c = 0; c < d.ord; c++) {
+    A.C[c] = d.coefs[c];
+    A.Cd[0][c] = d.coefs[c] / d.dx;
+    A.Cd[1][c] = d.coefs[c] / d.dy;
+  }
   A.ax[0] = s->ax[0]; A.ax[1] = s->ax[1];
   A.rho = s->rho; A.cp = s->cp; A.cs = s->cs; A.ts = g->ts;
   A.L = d.n_sls;

@@ -164,6 +164,7 @@ struct DiffArgs {
   long ncell;
   float dx, dy, dt;
   float C[VSE_MAX_ORD];
+  float Cd[2][VSE_MAX_ORD];          // f32(C[c] / d_axis): the adjoint kernels' differences
   AxisProf ax[2];
   const VSE_G float *rho, *cp, *cs;
   const VSE_G float *ts;             // per-cell relaxation-time scale

@@ -49,8 +49,11 @@ HBM_PEAK_GBS = 8000.0
 # family (ty, non-temporal, x-march), measured with rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE (scaled by 2.0 and 1.0 from the copy-kernel calibration); the
 # table and commands are in profiles/visco3d_512cube_kernels.md.
-STRESS_TRAFFIC_512 = {(4, 0, True): 32.76e9, (1, 0, True): 33.65e9, (4, 1, True): 31.68e9,
-                      (1, 1, True): 31.63e9, (4, 0, False): 38.90e9}
+STRESS_TRAFFIC_512 = {(4, 0, True): 32.69e9, (1, 0, True): 33.66e9, (4, 1, True): 31.68e9,
+                      (1, 1, True): 31.64e9, (8, 0, True): 33.68e9, (8, 1, True): 31.65e9,
+                      (4, 0, False): 38.90e9}
+# The same for the 2048^2 2D viscoelastic stress kernel (profiles/visco2d_2048_kernels.md).
+STRESS2D_TRAFFIC_2048 = 489.5e6
 
 
 def bytes_per_cell(g, L):
@@ -132,7 +135,8 @@ def bench_2d(args):
         "kernel_ms": {"stress": round(k_ms[0], 4), "velocity": round(k_ms[1], 4),
                       "step_counter": round(k_ms[2], 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": STRESS2D_TRAFFIC_2048 if n == 2048 else None,
                      "kernel": "stress2d", "bytes_per_cell": round(b_stress, 2)},
         "cpu_baseline": cpu,
     }
@@ -156,6 +160,8 @@ def main():
     ap.add_argument("--sizes", default="512,1024,1536,2048",
                     help="specimen grid sizes (--workload shots), comma separated")
     ap.add_argument("--shots", type=int, default=16, help="array elements = FMC shots per specimen (--workload shots)")
+    ap.add_argument("--group-sweep", action="store_true",
+                    help="also time the batch at cache-group budgets 0..320 MB (--workload shots)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 5 launches): rocprof counter calibration")
     args = ap.parse_args()
@@ -335,6 +341,21 @@ def bench_shots(args):
           file=sys.stderr, flush=True)
     wall_l, _ = timed(lambda n: [g.run(n, check_every=check) for g in sims])
     print(f"# per-shot loop: {wall_l:.3f} s", file=sys.stderr, flush=True)
+    sweep = None
+    if args.group_sweep:
+        # batch Gcell/s at each cache-group budget (MB of member device memory per group)
+        sweep, n_sw = {}, min(steps, 200)
+        for mb in (0, 32, 64, 96, 128, 160, 192, 224, 256, 320):
+            batch.set_group_bytes(mb << 20)
+            key = f"{mb}MB/{len(batch.groups())}g"
+            for rep in range(2):                      # the first run captures the groups' graphs
+                for g in sims:
+                    g.reset()
+                t0 = time.perf_counter()
+                batch.run(n_sw, check_every=n_sw)
+                w = time.perf_counter() - t0
+            sweep[key] = round(cells * n_sw / w / 1e9, 2)
+            print(f"# group budget {key}: {sweep[key]} Gcell/s", file=sys.stderr, flush=True)
     batch.close()
     name = sims[0].device_name()
     for g in sims:
@@ -352,6 +373,7 @@ def bench_shots(args):
                    "shots": len(sims), "cells_per_step": cells, "shot_steps": shot_len,
                    "parallelism": "single GPU, one batch", "cache_groups": len(groups)},
         "per_shot_loop_shots_per_s": round(shots / wall_l, 4),
+        "group_sweep_gcell_per_s": sweep,
         "batch_speedup": round(wall_l / wall_b, 3),
         "gcell_per_s_batched": round(cells * steps / wall_b / 1e9, 3),
         "gcell_per_s_loop": round(cells * steps / wall_l / 1e9, 3),

@@ -176,7 +176,10 @@ __device__ __forceinline__ float diffA(const DiffArgs &A, const Taps<ORD> &t) {
 // variables join the early loads; LC = 0 loops over A.L mechanisms.
 
 // ---------------------------------------------------------------- forward
-template <int ORD, int LC = 0>
+// TS = false: every relaxation-time scale is 1, so s is neither read nor
+// applied (x * 1.0f == x and zener_cell selects the stored constants: the
+// same bits as TS = true over a map of ones).
+template <int ORD, int LC = 0, bool TS = true>
 __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
   const int nx = A.nx, ny = A.ny;
   if (x >= nx || y >= ny) return;
@@ -202,7 +205,7 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
   if (inN || inS) {
     rho0 = A.rho[id];
     cs0 = A.cs[id];
-    ts = A.ts[id];
+    if constexpr (TS) ts = A.ts[id];
   }
   if (inN) {
     tvx = taps<ORD, true>(I.vx, id, ny);
@@ -233,7 +236,7 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
       for (int l = 0; l < LC; l++) oxy[l] = I.r[2][l * nc + id];
     }
   }
-  const float rs = 1.0f / ts;
+  const float rs = TS ? 1.0f / ts : 1.0f;
 
   // ---- normal stresses
   if (inN) {
@@ -423,11 +426,11 @@ __device__ __forceinline__ void record_traces(const DiffArgs &A) {
   }
 }
 
-template <int ORD, int LC>
+template <int ORD, int LC, bool TS>
 __global__ __launch_bounds__(BX * BY) void k_dstress(DiffArgs A) {
   int x, y;
   tile_cell(A, x, y);
-  dstress_cell<ORD, LC>(A, x, y);
+  dstress_cell<ORD, LC, TS>(A, x, y);
 }
 
 template <int ORD>
@@ -568,7 +571,7 @@ __global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
   B.vy[id] = inY ? vyb : 0.f;
 }
 
-template <int ORD, int LC>
+template <int ORD, int LC, bool TS>
 __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
   int x, y;
   tile_cell(A, x, y);
@@ -601,7 +604,7 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
   if (inN || inS) {
     rho0 = A.rho[id];
     cs0 = A.cs[id];
-    ts = A.ts[id];
+    if constexpr (TS) ts = A.ts[id];
     gt = A.gf[VSE_GR_TAU][id];
   }
   if (inN) {
@@ -653,7 +656,7 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
       }
     }
   }
-  const float rs = 1.0f / ts;
+  const float rs = TS ? 1.0f / ts : 1.0f;
 
   // ---- adjoint of the stress pass
   const float P = bsxx + sumT<ORD, 0>(A, u4);
@@ -806,9 +809,10 @@ dim3 grid_of(const DiffArgs &A) { return dim3((unsigned)(((A.ny + BY - 1) / BY) 
 }  // namespace
 
 hipError_t vse_launch_dstress(const DiffArgs &A, hipStream_t st) {
-#define CALL(O)                                                                                 \
-  if (A.L == 3) hipLaunchKernelGGL((k_dstress<O, 3>), grid_of(A), dim3(BY, BX), 0, st, A);      \
-  else hipLaunchKernelGGL((k_dstress<O, 0>), grid_of(A), dim3(BY, BX), 0, st, A)
+#define CALL(O)                                                                                    \
+  if (A.L == 3 && A.ts_unit) hipLaunchKernelGGL((k_dstress<O, 3, false>), grid_of(A), dim3(BY, BX), 0, st, A); \
+  else if (A.L == 3) hipLaunchKernelGGL((k_dstress<O, 3, true>), grid_of(A), dim3(BY, BX), 0, st, A);      \
+  else hipLaunchKernelGGL((k_dstress<O, 0, true>), grid_of(A), dim3(BY, BX), 0, st, A)
   VSE_ORD_SWITCH(A.ord, CALL)
 #undef CALL
   return hipGetLastError();
@@ -829,9 +833,10 @@ hipError_t vse_launch_adj_velocity(const DiffArgs &A, hipStream_t st) {
 }
 
 hipError_t vse_launch_adj_stress(const DiffArgs &A, hipStream_t st) {
-#define CALL(O)                                                                                 \
-  if (A.L == 3) hipLaunchKernelGGL((k_adj_stress<O, 3>), grid_of(A), dim3(BY, BX), 0, st, A);   \
-  else hipLaunchKernelGGL((k_adj_stress<O, 0>), grid_of(A), dim3(BY, BX), 0, st, A)
+#define CALL(O)                                                                                       \
+  if (A.L == 3 && A.ts_unit) hipLaunchKernelGGL((k_adj_stress<O, 3, false>), grid_of(A), dim3(BY, BX), 0, st, A); \
+  else if (A.L == 3) hipLaunchKernelGGL((k_adj_stress<O, 3, true>), grid_of(A), dim3(BY, BX), 0, st, A);      \
+  else hipLaunchKernelGGL((k_adj_stress<O, 0, true>), grid_of(A), dim3(BY, BX), 0, st, A)
   VSE_ORD_SWITCH(A.ord, CALL)
 #undef CALL
   return hipGetLastError();

@@ -997,6 +997,7 @@ struct vse_grad {
   double *g[VSE_GR_COUNT] = {};
   float *gf[VSE_GR_COUNT] = {};   // FP32 per-step partials of g
   float *ts = nullptr, *traces = nullptr, *gbar = nullptr, *obs = nullptr;
+  bool ts_unit = true;            // every tau scale is exactly 1
   int *src_map = nullptr;
   double *misfit = nullptr;
   unsigned *ctrl = nullptr, *host_ctrl = nullptr;
@@ -1061,7 +1062,7 @@ static void make_diff(vse_grad *g, DiffArgs &A) {
     A.Cd[1][c] = d.coefs[c] / d.dy;
   }
   A.ax[0] = s->ax[0]; A.ax[1] = s->ax[1];
-  A.rho = s->rho; A.cp = s->cp; A.cs = s->cs; A.ts = g->ts;
+  A.rho = s->rho; A.cp = s->cp; A.cs = s->cs; A.ts = g->ts; A.ts_unit = g->ts_unit ? 1 : 0;
   A.L = d.n_sls;
   A.zk = s->zk;
   A.bar = g->bar;
@@ -1165,10 +1166,14 @@ int vse_grad_set_tau_scale(vse_grad *g, const float *scale) {
     ones.assign(s->ncell, 1.0f);
     scale = ones.data();
   }
-  for (long i = 0; i < s->ncell; i++)
+  bool unit = true;
+  for (long i = 0; i < s->ncell; i++) {
     if (!(scale[i] > 0.f) || !std::isfinite(scale[i])) return g->fail(VSE_ERR_ARG, "tau scale must be positive and finite");
+    unit = unit && scale[i] == 1.0f;
+  }
   int rc = g->hip(hipMemcpyAsync(g->ts, scale, s->ncell * sizeof(float), hipMemcpyHostToDevice, s->stream), "tau scale");
   if (!rc) rc = g->hip(hipStreamSynchronize(s->stream), "tau scale");
+  if (!rc) g->ts_unit = unit;
   return rc;
 }
 

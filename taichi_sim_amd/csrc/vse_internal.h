@@ -168,6 +168,7 @@ struct DiffArgs {
   AxisProf ax[2];
   const VSE_G float *rho, *cp, *cs;
   const VSE_G float *ts;             // per-cell relaxation-time scale
+  int ts_unit;                       // 1: every scale is exactly 1 (ts is not read)
   int L;
   DLevel in, out;                    // forward: n -> n+1; adjoint: levels n and n+1
   DLevel bar;                        // adjoint state (in place)

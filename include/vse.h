@@ -117,8 +117,9 @@ int vse_run(vse_sim *s, int n_steps, int check_every, float *elapsed_ms);
 
 /* Run n_steps steps launching each kernel between its own HIP events on the
  * library's stream; ms[0..2] receive the average duration of the stress,
- * velocity and sparse (source/receiver/counter) kernels.  In 2D the sparse
- * work runs inside the velocity kernel, so ms[2] is only the event gap.  For roofline
+ * velocity and sparse (source/receiver/counter) kernels.  In 2D the sources
+ * and receivers run inside the velocity kernel, and ms[2] times the one-block
+ * step-counter advance.  For roofline
  * measurement; no reference counterpart (the reference only times the whole
  * loop, sim_webgpu_viscoelastic.py:470,537). */
 int vse_profile(vse_sim *s, int n_steps, float *ms);
@@ -191,9 +192,11 @@ void vse_destroy(vse_sim *s);
  * into groups whose device bytes fit a cache budget (vse_batch_set_group_bytes;
  * default VSE_BATCH_GROUP_BYTES_DEFAULT, sized for the 256 MB Infinity Cache
  * of MI355X).  Each group runs all its steps before the next group starts,
- * as hipGraphs of up to check_every steps in which every step is one stress,
- * one velocity and one sparse launch over the tiles of all the group's
- * members; a group of one member runs vse_run on it.  Members must be
+ * as hipGraphs of up to check_every steps in which every step is one stress
+ * and one velocity launch over the tiles of all the group's members (the
+ * velocity launch also injects their sources and records their receivers;
+ * one launch per graph advances their step counters); a group of one member
+ * runs vse_run on it.  Members must be
  * WGSL-variant simulations (Taichi-variant batches are switched off and
  * return VSE_ERR_ARG) sharing viscoelastic flag, FD order, number of
  * mechanisms and device; they keep their own state,
@@ -209,9 +212,9 @@ int vse_batch_create(vse_sim *const *sims, int n, vse_batch **out);
  * member in vse_batch_last_error() (and in that member's vse_last_error()).
  * elapsed_ms (optional) sums the device time of every group. */
 int vse_batch_run(vse_batch *b, int n_steps, int check_every, float *elapsed_ms);
-/* Average (stress, velocity, sparse) durations in ms of one step of all
- * members in one launch per pass (ignores the grouping), over n_steps steps,
- * each launch bracketed by HIP events.  Advances the members. */
+/* Average (stress, velocity, step-counter advance) durations in ms of one
+ * step of all members in one launch per pass (ignores the grouping), over
+ * n_steps steps, each launch bracketed by HIP events.  Advances the members. */
 int vse_batch_profile(vse_batch *b, int n_steps, float *ms);
 /* Each workgroup of a batched launch covers 64 (y) x 4*rows (x) cells of one
  * member, walking its `rows` groups of 4 x-rows in turn (default 1, 1..64).

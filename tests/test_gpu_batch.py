@@ -37,6 +37,7 @@ def _state(g):
         for c in ("xx", "yy", "xy"):
             f["r" + c] = g.memory(c)
     f["traces"] = g.traces()
+    f["v2max"] = np.array([g.v2max()], dtype=np.float32)     # running max of |v|^2
     return f
 
 
@@ -83,6 +84,7 @@ def test_batch_uneven_specimens_match_solo_runs(variant, grouping):
     o = oracle.Oracle(sim, variant=variant, att=att, visco=variant == "wgsl").run(steps)
     for k in ("vx", "vy", "syy"):
         assert oracle.rel_l2(g.field(k), o.f[k]) <= 1e-5, k
+    assert o.v2max[0] > 0 and abs(g.v2max() - o.v2max[0]) <= 1e-5 * o.v2max[0]
     if np.abs(o.traces).max() > 0:
         assert oracle.rel_l2(g.traces(), o.traces) <= 1e-5
     for m in members:

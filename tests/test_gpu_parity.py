@@ -51,6 +51,8 @@ def _compare(sim, variant, steps, att=None, visco=None, check_f64=True):
         rel = oracle.rel_l2(tr, o.traces)
         worst = max(worst, rel)
         assert rel <= TOL, ("traces", rel)
+    # running max of |v|^2 over every cell and step (the stability check's input)
+    assert o.v2max[0] > 0 and abs(g.v2max() - o.v2max[0]) <= TOL * o.v2max[0], (g.v2max(), o.v2max[0])
     if check_f64:
         o64 = oracle.Oracle(sim, variant=variant, att=att, visco=visco, precision="f64").run(steps)
         key = "syy" if g.ndim == 2 else "szz"

@@ -67,6 +67,8 @@ class VSEGrad:
     def __init__(self, sim, seg_len=0):
         if sim.ndim != 2 or sim.variant != "wgsl" or not sim.visco:
             raise ValueError("gradients need a 2D viscoelastic simulation with the WGSL semantics")
+        if getattr(sim, "_h", None) is None:
+            raise ValueError("the simulation is closed")
         self.sim = sim
         self._lib = sim._lib
         h = ctypes.c_void_p()
@@ -76,6 +78,7 @@ class VSEGrad:
         self._h = h
         self.n_steps = 0
         self.ms = {}
+        sim._dependents.add(self)       # closing the simulation closes this engine first
 
     def _check(self, rc):
         if rc == 0:

@@ -16,6 +16,7 @@ reference's host tables into the library's forms:
   interior where the reference records (sim_taichi_elastic.py:150-190).
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -119,6 +120,7 @@ class VSESim:
         self._h = h
         self.n_rec = 0
         self._bands = {}
+        self._dependents = weakref.WeakSet()   # open batches / gradient engines holding this handle
 
     # ---------------------------------------------------------------- helpers
     def _check(self, rc):
@@ -188,7 +190,9 @@ class VSESim:
 
     def profile(self, n_steps):
         """Average (stress, velocity, sparse) kernel durations in ms over n_steps
-        steps, each kernel bracketed by HIP events on the library's stream."""
+        steps, each kernel bracketed by HIP events on the library's stream.  In
+        2D the velocity kernel does the sources and receivers and the third
+        value times the step-counter advance."""
         ms = (ctypes.c_float * 3)()
         self._check(self._lib.vse_profile(self._h, int(n_steps), ms))
         return tuple(float(v) for v in ms)
@@ -238,6 +242,11 @@ class VSESim:
         return int(self._lib.vse_device_bytes(self._h))
 
     def close(self):
+        """Release the device state; a batch or gradient engine still holding
+        this simulation is closed first (it would otherwise keep a dangling
+        handle)."""
+        for d in list(getattr(self, "_dependents", ())):
+            d.close()
         if getattr(self, "_h", None) is not None and self._h.value:
             self._lib.vse_destroy(self._h)
             self._h = None
@@ -309,6 +318,8 @@ class VSEBatch:
         self.sims = list(sims)
         if not 1 <= len(self.sims) <= _lib.BATCH_MAX:
             raise ValueError(f"a batch holds 1..{_lib.BATCH_MAX} simulations, got {len(self.sims)}")
+        if any(getattr(s, "_h", None) is None for s in self.sims):
+            raise ValueError("batch members must be open simulations")
         arr = (ctypes.c_void_p * len(self.sims))(*[s._h.value for s in self.sims])
         h = ctypes.c_void_p()
         rc = self._lib.vse_batch_create(arr, len(self.sims), ctypes.byref(h))
@@ -316,8 +327,16 @@ class VSEBatch:
             raise _lib.VSEError(rc, "vse_batch_create failed: members must be open 2D WGSL-variant simulations on "
                                     "one device with the same viscoelastic flag, FD order and mechanism count")
         self._h = h
+        for s in self.sims:
+            s._dependents.add(self)
+
+    def _open(self):
+        if getattr(self, "_h", None) is None:
+            raise ValueError("batch is closed")
+        return self._h
 
     def run(self, n_steps, check_every=100, timed=False):
+        self._open()
         ms = ctypes.c_float(0.0)
         rc = self._lib.vse_batch_run(self._h, int(n_steps), int(check_every), ctypes.byref(ms) if timed else None)
         if rc != 0:
@@ -329,28 +348,28 @@ class VSEBatch:
 
     def set_rows(self, rows):
         """x row-groups (of 4 rows) each workgroup walks; speed only."""
-        rc = self._lib.vse_batch_set_rows(self._h, int(rows))
+        rc = self._lib.vse_batch_set_rows(self._open(), int(rows))
         if rc != 0:
             raise _lib.VSEError(rc, self._lib.vse_batch_last_error(self._h).decode(errors="replace"))
 
     def set_group_bytes(self, nbytes):
         """Cache budget per group of members (0: all members in lockstep); speed only."""
-        rc = self._lib.vse_batch_set_group_bytes(self._h, int(nbytes))
+        rc = self._lib.vse_batch_set_group_bytes(self._open(), int(nbytes))
         if rc != 0:
             raise _lib.VSEError(rc, self._lib.vse_batch_last_error(self._h).decode(errors="replace"))
 
     def groups(self):
         """First member index of each group."""
-        n = self._lib.vse_batch_groups(self._h, None, 0)
+        n = self._lib.vse_batch_groups(self._open(), None, 0)
         out = (ctypes.c_int * max(n, 1))()
         self._lib.vse_batch_groups(self._h, out, n)
         return list(out[:n])
 
     def profile(self, n_steps):
-        """Average (stress, velocity, sparse) launch durations in ms of one step of
-        all members in one launch per pass."""
+        """Average (stress, velocity, step-counter advance) launch durations in ms
+        of one step of all members in one launch per pass."""
         ms = (ctypes.c_float * 3)()
-        rc = self._lib.vse_batch_profile(self._h, int(n_steps), ms)
+        rc = self._lib.vse_batch_profile(self._open(), int(n_steps), ms)
         if rc != 0:
             raise _lib.VSEError(rc, self._lib.vse_batch_last_error(self._h).decode(errors="replace"))
         return tuple(float(v) for v in ms)

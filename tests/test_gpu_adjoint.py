@@ -47,6 +47,20 @@ def _smooth_field(shape, seed, amp):
     return amp * f / max(np.abs(f).max(), 1e-30)
 
 
+def test_closing_the_simulation_closes_its_gradient_engine():
+    from taichi_sim_amd._lib import VSEError
+    sim, att = _setup(48, 20)
+    g = VSESim.from_simulator(sim, variant="wgsl", att=att)
+    eng = VSEGrad(g)
+    eng.forward(20)
+    g.close()
+    assert eng._h is None
+    with pytest.raises(VSEError):
+        eng.forward(20)
+    with pytest.raises(ValueError):
+        VSEGrad(g)
+
+
 def test_checkpointed_forward_is_bit_identical_to_run():
     sim, att = _setup(96, 200, voids=2, n_elem=8)
     print("checkpointed forward vs vse_run", flush=True)

@@ -145,6 +145,20 @@ def test_batch_rejects_mixed_members():
     b.close()
 
 
+def test_closing_a_member_closes_its_batch():
+    # the batch keeps raw member handles: closing a member must not leave it dangling
+    gs = [m[3] for m in _shots((96,), 2, 20)]
+    batch = VSEBatch(gs)
+    batch.run(5)
+    gs[0].close()
+    with pytest.raises(ValueError, match="closed"):
+        batch.run(5)
+    with pytest.raises(ValueError):
+        VSEBatch(gs)
+    assert gs[1].step() == 5
+    gs[1].close()
+
+
 def test_run_laws_batched_equals_sequential(tmp_path):
     """Simulator.run() with laws handed over in groups gives the same
     per-law results, in the same order, as one law per implementation()."""

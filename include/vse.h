@@ -14,7 +14,9 @@
  * copied; device memory is owned by the handle; every call is synchronous
  * with respect to host buffers; return 0 on success, a negative VSE_ERR_*
  * code on failure (vse_last_error() gives the message).  One handle per
- * thread.  Grids are C order with the last axis fastest (2D (nx, ny) with
+ * thread.  A handle belongs to the device current at vse_create; every call
+ * on it makes that device current for its duration and restores the
+ * caller's.  Grids are C order with the last axis fastest (2D (nx, ny) with
  * y = the ROI's z axis; 3D (nx, ny, nz)), exactly as the reference lays out
  * its buffers.
  */

@@ -110,6 +110,24 @@ struct vse_sim {
 
 static int check_handle(vse_sim *s) { return s ? 0 : VSE_ERR_ARG; }
 
+// Makes a handle's device current for one C-ABI call and restores the
+// caller's device on return, so a caller that switched devices between calls
+// (torch.cuda.set_device, another library) still reaches the handle's GPU.
+class OnDevice {
+ public:
+  explicit OnDevice(int dev) {
+    if (hipGetDevice(&prev_) != hipSuccess || prev_ == dev || hipSetDevice(dev) != hipSuccess) prev_ = -1;
+  }
+  ~OnDevice() {
+    if (prev_ >= 0) (void)hipSetDevice(prev_);
+  }
+  OnDevice(const OnDevice &) = delete;
+  OnDevice &operator=(const OnDevice &) = delete;
+
+ private:
+  int prev_ = -1;
+};
+
 extern "C" {
 
 int vse_abi_version(void) { return VSE_ABI_VERSION; }
@@ -191,6 +209,7 @@ int vse_create(const vse_desc *desc, vse_sim **out) {
 
 int vse_set_material(vse_sim *s, const float *rho, const float *cp, const float *cs) {
   if (check_handle(s) || !rho || !cp || !cs) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   int rc = 0;
   if (!s->rho) {
     rc = s->alloc(&s->rho, s->ncell);
@@ -210,6 +229,7 @@ int vse_set_material(vse_sim *s, const float *rho, const float *cp, const float 
 int vse_set_cpml(vse_sim *s, int axis, const float *a, const float *b, const float *k, const float *a_h,
                  const float *b_h, const float *k_h) {
   if (check_handle(s) || axis < 0 || axis >= s->d.ndim) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   const float *src[6] = {a, b, k, a_h, b_h, k_h};
   for (int i = 0; i < 6; i++)
     if (!src[i]) return VSE_ERR_ARG;
@@ -253,6 +273,7 @@ int vse_set_cpml(vse_sim *s, int axis, const float *a, const float *b, const flo
 
 int vse_set_sources(vse_sim *s, int n_pts, const int *cells, const int *cols, int n_cols, const float *src) {
   if (check_handle(s) || n_pts < 0 || n_cols < 1 || !src || (n_pts > 0 && (!cells || !cols))) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   for (int i = 0; i < n_pts; i++)
     if (cells[i] < 0 || cells[i] >= s->ncell || cols[i] < 0 || cols[i] >= n_cols)
       return s->fail(VSE_ERR_ARG, "source point out of range");
@@ -296,6 +317,7 @@ int vse_set_sources(vse_sim *s, int n_pts, const int *cells, const int *cols, in
 
 int vse_set_receivers(vse_sim *s, int n_rec, const int *beg, const int *cells, const int *delay, int gate) {
   if (check_handle(s) || n_rec < 0 || (n_rec > 0 && (!beg || !delay))) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   const int n_pts = n_rec > 0 ? beg[n_rec] : 0;
   if (n_rec > 0 && beg[0] != 0) return s->fail(VSE_ERR_ARG, "receiver CSR must start at 0");
   for (int r = 0; r < n_rec; r++)
@@ -354,6 +376,7 @@ static int finalize(vse_sim *s) {
 
 int vse_reset(vse_sim *s) {
   if (check_handle(s)) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   int rc = finalize(s);
   if (rc) return rc;
   for (int f = 0; f < 9 && !rc; f++)
@@ -476,6 +499,7 @@ static int check_stable(vse_sim *s) {
 
 int vse_run(vse_sim *s, int n_steps, int check_every, float *elapsed_ms) {
   if (check_handle(s) || n_steps < 0) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   int rc = finalize(s);
   if (rc) return rc;
   if (n_steps == 0) { if (elapsed_ms) *elapsed_ms = 0.f; return 0; }
@@ -522,6 +546,7 @@ int vse_run(vse_sim *s, int n_steps, int check_every, float *elapsed_ms) {
 
 int vse_profile(vse_sim *s, int n_steps, float *ms) {
   if (check_handle(s) || n_steps < 1 || !ms) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   int rc = finalize(s);
   if (rc) return rc;
   StepArgs A;
@@ -575,6 +600,7 @@ int vse_set_math(vse_sim *s, int mode) {
 
 int vse_read_field(vse_sim *s, int field, float *out) {
   if (check_handle(s) || field < 0 || field > 8 || !out) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   if (!s->field[field]) return s->fail(VSE_ERR_ARG, "field not present in this simulation");
   int rc = s->hip(hipMemcpyAsync(out, s->field[field], s->ncell * sizeof(float), hipMemcpyDeviceToHost, s->stream), "read field");
   if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "read field sync");
@@ -583,6 +609,7 @@ int vse_read_field(vse_sim *s, int field, float *out) {
 
 int vse_read_traces(vse_sim *s, float *out) {
   if (check_handle(s) || !out) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   if (!s->traces) return s->fail(VSE_ERR_STATE, "no receivers");
   int rc = s->hip(hipMemcpyAsync(out, s->traces, (size_t)s->d.n_rows * s->n_rec * sizeof(float), hipMemcpyDeviceToHost, s->stream), "read traces");
   if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "read traces sync");
@@ -591,6 +618,7 @@ int vse_read_traces(vse_sim *s, float *out) {
 
 int vse_read_memory(vse_sim *s, int comp, float *out) {
   if (check_handle(s) || comp < 0 || comp > 5 || !out) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   if (!s->r[comp]) return s->fail(VSE_ERR_ARG, "memory variable not present");
   const int L = s->d.n_sls;
   std::vector<float> planes((size_t)s->ncell * L);
@@ -604,6 +632,7 @@ int vse_read_memory(vse_sim *s, int comp, float *out) {
 
 int vse_read_v2max(vse_sim *s, float *out) {
   if (check_handle(s) || !out) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   int rc = fold_v2(s);
   if (!rc) rc = s->hip(hipMemcpyAsync(s->host_ctrl, s->ctrl, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream), "read ctrl");
   if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "sync");
@@ -613,6 +642,7 @@ int vse_read_v2max(vse_sim *s, float *out) {
 
 int vse_get_step(vse_sim *s, int *out) {
   if (check_handle(s) || !out) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   int rc = s->hip(hipMemcpyAsync(s->host_ctrl, s->ctrl, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream), "read ctrl");
   if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "sync");
   if (!rc) *out = (int)s->host_ctrl[1];
@@ -632,6 +662,7 @@ const char *vse_last_error(vse_sim *s) { return s ? s->err.c_str() : "null handl
 
 void vse_destroy(vse_sim *s) {
   if (!s) return;
+  OnDevice on_dev(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   s->drop_graph();
   for (void *p : s->allocs) (void)hipFree(p);
@@ -844,6 +875,7 @@ extern "C" {
 
 void vse_batch_destroy(vse_batch *b) {
   if (!b) return;
+  OnDevice on_dev(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   b->drop_graphs();
   if (b->d_args) (void)hipFree(b->d_args);
@@ -860,6 +892,7 @@ int vse_batch_create(vse_sim *const *sims, int n, vse_batch **out) {
   *out = nullptr;
   vse_sim *s0 = sims[0];
   if (!s0) return VSE_ERR_ARG;
+  OnDevice on_dev(s0->device);
   // Taichi-variant batches are switched off: a process ended in SIGABRT while
   // tearing down the solo runs that followed one, and no cause has been shown.
   if (s0->d.variant != VSE_VARIANT_WGSL) return s0->fail(VSE_ERR_ARG, "batches take WGSL-variant simulations only");
@@ -878,10 +911,7 @@ int vse_batch_create(vse_sim *const *sims, int n, vse_batch **out) {
   b->visco = s0->d.visco;
   b->ord = s0->d.ord;
   b->group_bytes = VSE_BATCH_GROUP_BYTES_DEFAULT;
-  int cur = 0;
-  int rc = b->hip(hipGetDevice(&cur), "hipGetDevice");
-  if (!rc && cur != b->device) rc = b->fail(VSE_ERR_ARG, "batch must be created on its members' device");
-  if (!rc) rc = b->hip(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+  int rc = b->hip(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
   if (!rc) rc = b->hip(hipMalloc((void **)&b->d_args, n * sizeof(StepArgs)), "hipMalloc");
   if (!rc) rc = b->hip(hipMalloc((void **)&b->d_sparse, n * sizeof(SparseArgs)), "hipMalloc");
   if (!rc) rc = b->hip(hipMalloc((void **)&b->d_ctrl, 2 * n * sizeof(unsigned)), "hipMalloc");
@@ -898,6 +928,7 @@ int vse_batch_create(vse_sim *const *sims, int n, vse_batch **out) {
 
 int vse_batch_run(vse_batch *b, int n_steps, int check_every, float *elapsed_ms) {
   if (!b || n_steps < 0) return VSE_ERR_ARG;
+  OnDevice on_dev(b->device);
   int rc = upload_batch_args(b);
   if (rc) return rc;
   double total = 0.0;
@@ -923,6 +954,7 @@ int vse_batch_run(vse_batch *b, int n_steps, int check_every, float *elapsed_ms)
 
 int vse_batch_profile(vse_batch *b, int n_steps, float *ms) {
   if (!b || n_steps < 1 || !ms) return VSE_ERR_ARG;
+  OnDevice on_dev(b->device);
   int rc = upload_batch_args(b);
   if (rc) return rc;
   const BatchGroup &g = b->all;
@@ -955,12 +987,14 @@ int vse_batch_profile(vse_batch *b, int n_steps, float *ms) {
 
 int vse_batch_set_rows(vse_batch *b, int rows) {
   if (!b || rows < 1 || rows > 64) return VSE_ERR_ARG;
+  OnDevice on_dev(b->device);
   b->rows = rows;
   return regroup(b);
 }
 
 int vse_batch_set_group_bytes(vse_batch *b, size_t bytes) {
   if (!b) return VSE_ERR_ARG;
+  OnDevice on_dev(b->device);
   b->group_bytes = bytes;
   return regroup(b);
 }
@@ -1107,6 +1141,7 @@ extern "C" {
 
 void vse_grad_destroy(vse_grad *g) {
   if (!g) return;
+  OnDevice on_dev(g->s->device);
   if (g->s && g->s->stream) (void)hipStreamSynchronize(g->s->stream);
   for (void *p : g->allocs) (void)hipFree(p);
   if (g->host_ctrl) (void)hipHostFree(g->host_ctrl);
@@ -1115,6 +1150,7 @@ void vse_grad_destroy(vse_grad *g) {
 
 int vse_grad_create(vse_sim *s, int seg_len, vse_grad **out) {
   if (!s || !out || seg_len < 0 || seg_len == 1) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   *out = nullptr;
   if (s->d.ndim != 2 || s->d.variant != VSE_VARIANT_WGSL || !s->d.visco)
     return s->fail(VSE_ERR_ARG, "gradient: 2D viscoelastic WGSL-variant simulations only");
@@ -1160,6 +1196,7 @@ int vse_grad_create(vse_sim *s, int seg_len, vse_grad **out) {
 
 int vse_grad_set_tau_scale(vse_grad *g, const float *scale) {
   if (!g) return VSE_ERR_ARG;
+  OnDevice on_dev(g->s->device);
   vse_sim *s = g->s;
   std::vector<float> ones;
   if (!scale) {
@@ -1179,6 +1216,7 @@ int vse_grad_set_tau_scale(vse_grad *g, const float *scale) {
 
 int vse_grad_forward(vse_grad *g, int n_steps, int check_every, float *traces, float *elapsed_ms) {
   if (!g || n_steps < 1 || n_steps > g->s->d.n_rows) return VSE_ERR_ARG;
+  OnDevice on_dev(g->s->device);
   vse_sim *s = g->s;
   int rc = finalize(s);
   if (rc) return g->fail(rc, s->err);
@@ -1232,6 +1270,7 @@ int vse_grad_forward(vse_grad *g, int n_steps, int check_every, float *traces, f
 
 int vse_grad_misfit(vse_grad *g, const float *observed, double *misfit) {
   if (!g || !observed || !misfit) return VSE_ERR_ARG;
+  OnDevice on_dev(g->s->device);
   if (g->n_fwd < 1) return g->fail(VSE_ERR_STATE, "vse_grad_forward not run");
   vse_sim *s = g->s;
   const long n = (long)g->n_fwd * s->n_rec;
@@ -1245,6 +1284,7 @@ int vse_grad_misfit(vse_grad *g, const float *observed, double *misfit) {
 
 int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms) {
   if (!g) return VSE_ERR_ARG;
+  OnDevice on_dev(g->s->device);
   const int N = g->n_fwd;
   if (N < 1) return g->fail(VSE_ERR_STATE, "vse_grad_forward not run");
   vse_sim *s = g->s;
@@ -1297,6 +1337,7 @@ int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms) 
 
 int vse_grad_read(vse_grad *g, int which, double *out) {
   if (!g || which < 0 || which >= VSE_GR_COUNT || !out) return VSE_ERR_ARG;
+  OnDevice on_dev(g->s->device);
   vse_sim *s = g->s;
   int rc = g->hip(hipMemcpyAsync(out, g->g[which], s->ncell * sizeof(double), hipMemcpyDeviceToHost, s->stream), "read gradient");
   if (!rc) rc = g->hip(hipStreamSynchronize(s->stream), "read gradient");
@@ -1305,6 +1346,7 @@ int vse_grad_read(vse_grad *g, int which, double *out) {
 
 int vse_grad_read_state(vse_grad *g, int field, float *out) {
   if (!g || field < 0 || field > 4 || !out) return VSE_ERR_ARG;
+  OnDevice on_dev(g->s->device);
   if (g->n_fwd < 1 || !g->last_seg_stored) return g->fail(VSE_ERR_STATE, "no forward state (run vse_grad_forward)");
   const int m = g->n_fwd, last = (m - 1) / g->K;
   const DLevel &lv = g->lev(last, m - last * g->K);    // where the forward ended

@@ -586,6 +586,7 @@ int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap
   if (check_handle(s) || kernel < 0 || kernel > 1) return VSE_ERR_ARG;
   if (!(ty == 1 || ty == 4 || ty == 8) || (nt != 0 && nt != 1) || xchunk < 0 || (remap != 0 && remap != 1))
     return s->fail(VSE_ERR_ARG, "tuning: ty in {1,4,8}, nt in {0,1}, xchunk >= 0, remap in {0,1}");
+  OnDevice on_dev(s->device);
   s->tune[kernel] = KTune{ty, nt, xchunk, remap};
   s->drop_graph();
   return VSE_OK;
@@ -593,6 +594,7 @@ int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap
 
 int vse_set_math(vse_sim *s, int mode) {
   if (check_handle(s) || (mode != VSE_MATH_STRICT && mode != VSE_MATH_FAST)) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
   s->math = mode;
   s->drop_graph();
   return VSE_OK;

@@ -12,6 +12,22 @@ namespace {
 
 constexpr int BY = 64, BX = 4;
 
+// A whole field as a raw buffer: a scalar (SGPR) resource descriptor over its
+// bytes, addressed with a 32-bit per-lane offset (no 64-bit address per
+// load).  The hardware range-checks every access: an index outside
+// [0, ncell) -- a negative one wraps above the range -- reads 0, so a
+// cell's stencil operands can be loaded unconditionally; operands read
+// outside an update's interior are never used.  Word 3 0x00020000 is the
+// gfx9-family raw 32-bit buffer format.
+struct FieldBuf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ FieldBuf(const float *base, long ncell)
+      : r(__builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)(ncell * 4), 0x00020000)) {}
+  __device__ __forceinline__ float operator[](int i) const {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)((unsigned)i * 4u), 0, 0));
+  }
+};
+
 // LC > 0: the mechanism count is the compile-time LC and every memory-variable
 // load of a point is issued before its arithmetic (same operations, same
 // order: bit-identical to the runtime-L loop, which waits on each
@@ -158,40 +174,38 @@ __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, in
     const int jx = x - off, jy = y - off;
     const int bx = band_index(A.ax[0], jx), by = band_index(A.ax[1], jy);
     const long pxi = (long)bx * ny + y, pyi = (long)x * A.ax[1].nb + by;
-    // ---- loads
-    float vxn = A.vx[id], vyn = A.vy[id];
+    // ---- loads: every stencil operand unconditionally through the field
+    // buffers (reads outside the grid give 0; operands outside an update's
+    // interior are loaded but not used), the CPML band and source map only
+    // where they apply
+    const FieldBuf Bvx(A.vx, A.ncell), Bvy(A.vy, A.ncell), Bsxx(A.sxx, A.ncell), Bsxy(A.sxy, A.ncell),
+        Bsyy(A.syy, A.ncell), Brho(A.rho, A.ncell);
+    const int i = (int)id;
+    float vxn = Bvx[i], vyn = Bvy[i];
     float xx_p[ORD], xx_m[ORD], xy_p[ORD], xy_m[ORD];     // vx: sxx along x, sxy along y (D-)
     float yx_p[ORD], yx_m[ORD], yy_p[ORD], yy_m[ORD];     // vy: sxy along x, syy along y (D+)
+#pragma unroll
+    for (int c = 0; c < ORD; c++) {
+      xx_p[c] = Bsxx[i + c * ny];
+      xx_m[c] = Bsxx[i - (c + 1) * ny];
+      xy_p[c] = Bsxy[i + c];
+      xy_m[c] = Bsxy[i - c - 1];
+      yx_p[c] = Bsxy[i + (c + 1) * ny];
+      yx_m[c] = Bsxy[i - c * ny];
+      yy_p[c] = Bsyy[i + c + 1];
+      yy_m[c] = Bsyy[i - c];
+    }
+    // in the vy range x+1 and y+1 are inside the grid
+    const float r00 = Brho[i], r10 = Brho[i + ny], r11 = Brho[i + ny + 1], r01 = Brho[i + 1];
     float q1 = 0.f, q2 = 0.f, q3 = 0.f, q4 = 0.f;
-    float r00 = 0.f, r10 = 0.f, r11 = 0.f, r01 = 0.f;
     int is = -1;
     if (in_vx) {
-#pragma unroll
-      for (int c = 0; c < ORD; c++) {
-        xx_p[c] = A.sxx[id + (long)c * ny];
-        xx_m[c] = A.sxx[id - (long)(c + 1) * ny];
-        xy_p[c] = A.sxy[id + c];
-        xy_m[c] = A.sxy[id - c - 1];
-      }
       if (bx >= 0) q1 = A.psi[P_DSXX_DX][pxi];
       if (by >= 0) q2 = A.psi[P_DSXY_DY][pyi];
-      r00 = A.rho[id];
     }
     if (in_vy) {
-#pragma unroll
-      for (int c = 0; c < ORD; c++) {
-        yx_p[c] = A.sxy[id + (long)(c + 1) * ny];
-        yx_m[c] = A.sxy[id - (long)c * ny];
-        yy_p[c] = A.syy[id + c + 1];
-        yy_m[c] = A.syy[id - c];
-      }
       if (bx >= 0) q3 = A.psi[P_DSXY_DX][pxi];
       if (by >= 0) q4 = A.psi[P_DSYY_DY][pyi];
-      // in the vy range x+1 and y+1 are inside the grid
-      r00 = A.rho[id];
-      r10 = A.rho[id + ny];
-      r11 = A.rho[id + ny + 1];
-      r01 = A.rho[id + 1];
       if constexpr (VAR == VSE_VAR_TAICHI) is = A.src_map[id];
     }
     // ---- vx (wgsl2d :998-1025)

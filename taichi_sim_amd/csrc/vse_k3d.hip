@@ -63,6 +63,26 @@ struct PB {
   }
 };
 
+// A whole field (fewer than 2^30 cells) as one raw buffer, indexed by
+// plane * sx + in-plane offset in elements.  The descriptor is built once per
+// kernel, so a plane step costs vector offset adds shared by every field
+// instead of a scalar descriptor per plane and field.  Every access is
+// range-checked against the field: an index past its end, or a negative one
+// (which wraps above it), reads 0.
+struct FBuf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ FBuf(const float *base, long ncell)
+      : r(__builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)(unsigned)(ncell * 4), 0x00020000)) {}
+  template <int CP = 0>
+  __device__ __forceinline__ float ld(unsigned i) const {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4u), 0, CP));
+  }
+  template <int CP = 0>
+  __device__ __forceinline__ void st(unsigned i, float v) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)(i * 4u), 0, CP);
+  }
+};
+
 // psi recursion psi <- b psi + a D, returns D/k + psi (wgsl3d :1372-1378);
 // FAST multiplies by the precomputed 1/k.
 template <int CP, bool FAST>
@@ -357,7 +377,12 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
   const unsigned col = y * sy + z;
   const float dt = A.dt;
   const int x0 = T.bxc * xchunk, x1 = min(nx, x0 + xchunk);
-  auto P = [&](const float *f, int x) { return PB(f + (long)(x < 0 ? 0 : (x >= nx ? nx - 1 : x)) * sx, pbytes); };
+  // whole-field buffers (the host launches this kernel for fewer than 2^30 cells)
+  const FBuf Fsxx(A.sxx, A.ncell), Fsxy(A.sxy, A.ncell), Fsxz(A.sxz, A.ncell), Fsyy(A.syy, A.ncell);
+  const FBuf Fsyz(A.syz, A.ncell), Fszz(A.szz, A.ncell), Frho(A.rho, A.ncell);
+  const FBuf Fvx(A.vx, A.ncell), Fvy(A.vy, A.ncell), Fvz(A.vz, A.ncell);
+  // first element of plane x, clamped into the grid (clamped planes feed no update)
+  auto pl = [&](int x) { return (unsigned)((x < 0 ? 0 : (x >= nx ? nx - 1 : x)) * sx); };
   unsigned v2bits = 0;
   if (valid) {
     const int jy = y - off, jz = z - off;
@@ -372,45 +397,43 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
     float wxx[2 * ORD], wxy[2 * ORD], wxz[2 * ORD];
 #pragma unroll
     for (int i = 0; i < 2 * ORD - 1; i++) {
-      wxx[i + 1] = P(A.sxx, x0 - ORD + i).ld(col);
-      wxy[i + 1] = P(A.sxy, x0 - (ORD - 1) + i).ld(col);
-      wxz[i + 1] = P(A.sxz, x0 - (ORD - 1) + i).ld(col);
+      wxx[i + 1] = Fsxx.ld(pl(x0 - ORD + i) + col);
+      wxy[i + 1] = Fsxy.ld(pl(x0 - (ORD - 1) + i) + col);
+      wxz[i + 1] = Fsxz.ld(pl(x0 - (ORD - 1) + i) + col);
     }
-    float rho_n = P(A.rho, x0).ld(col);
+    float rho_n = Frho.ld(pl(x0) + col);
 #pragma unroll 1
     for (int x = x0; x < x1; x++) {
 #pragma unroll
       for (int i = 0; i < 2 * ORD - 1; i++) { wxx[i] = wxx[i + 1]; wxy[i] = wxy[i + 1]; wxz[i] = wxz[i + 1]; }
-      wxx[2 * ORD - 1] = P(A.sxx, x + ORD - 1).ld(col);
-      wxy[2 * ORD - 1] = P(A.sxy, x + ORD).ld(col);
-      wxz[2 * ORD - 1] = P(A.sxz, x + ORD).ld(col);
+      wxx[2 * ORD - 1] = Fsxx.ld(pl(x + ORD - 1) + col);
+      wxy[2 * ORD - 1] = Fsxy.ld(pl(x + ORD) + col);
+      wxz[2 * ORD - 1] = Fsxz.ld(pl(x + ORD) + col);
       const float rho0 = rho_n;
-      rho_n = P(A.rho, x + 1).ld(col);
+      rho_n = Frho.ld(pl(x + 1) + col);
       const int jx = x - off;
       const int bx = band_index(A.ax[0], jx);
       const long rofs = (long)x * sx;
-      const PB sxyP(A.sxy + rofs, pbytes), sxzP(A.sxz + rofs, pbytes), syyP(A.syy + rofs, pbytes);
-      const PB syzP(A.syz + rofs, pbytes), szzP(A.szz + rofs, pbytes), rhoP(A.rho + rofs, pbytes);
-      const PB vxP(A.vx + rofs, pbytes), vyP(A.vy + rofs, pbytes), vzP(A.vz + rofs, pbytes);
-      // Every load of the plane first, unconditionally: buffer loads are
-      // range-checked per plane (an offset outside it reads 0, and such
-      // values are used only outside the interiors, where they are not),
-      // so the wavefront has all of them in flight before any arithmetic.
-      float vxn = vxP.ld<CP>(col), vyn = vyP.ld<CP>(col), vzn = vzP.ld<CP>(col);
+      const unsigned i0 = (unsigned)rofs + col;      // the cell in plane x
+      // Every load of the plane first, unconditionally: a neighbour index
+      // outside the plane reads the adjacent plane, or 0 outside the field,
+      // and such values are used only outside the interiors, where they are
+      // not; so the wavefront has all of them in flight before any arithmetic.
+      float vxn = Fvx.ld<CP>(i0), vyn = Fvy.ld<CP>(i0), vzn = Fvz.ld<CP>(i0);
       float xy_p[ORD], xy_m[ORD], xz_p[ORD], xz_m[ORD];      // vx: sxy along y, sxz along z (D-)
       float yy_p[ORD], yy_m[ORD], yz_p[ORD], yz_m[ORD];      // vy: syy along y (D+), syz along z (D-)
       float zy_p[ORD], zy_m[ORD], zz_p[ORD], zz_m[ORD];      // vz: syz along y (D-), szz along z (D+)
 #pragma unroll
       for (int c = 0; c < ORD; c++) {
         // the own cell of plane x is already in the x windows
-        xy_p[c] = c == 0 ? wxy[ORD - 1] : sxyP.ld(col + c * sy); xy_m[c] = sxyP.ld(col - (c + 1) * sy);
-        xz_p[c] = c == 0 ? wxz[ORD - 1] : sxzP.ld(col + c); xz_m[c] = sxzP.ld(col - c - 1);
-        yy_p[c] = syyP.ld(col + (c + 1) * sy); yy_m[c] = syyP.ld(col - c * sy);
-        yz_p[c] = syzP.ld(col + c); yz_m[c] = syzP.ld(col - c - 1);
-        zy_p[c] = syzP.ld(col + c * sy); zy_m[c] = syzP.ld(col - (c + 1) * sy);
-        zz_p[c] = szzP.ld(col + c + 1); zz_m[c] = szzP.ld(col - c);
+        xy_p[c] = c == 0 ? wxy[ORD - 1] : Fsxy.ld(i0 + c * sy); xy_m[c] = Fsxy.ld(i0 - (c + 1) * sy);
+        xz_p[c] = c == 0 ? wxz[ORD - 1] : Fsxz.ld(i0 + c); xz_m[c] = Fsxz.ld(i0 - c - 1);
+        yy_p[c] = Fsyy.ld(i0 + (c + 1) * sy); yy_m[c] = Fsyy.ld(i0 - c * sy);
+        yz_p[c] = Fsyz.ld(i0 + c); yz_m[c] = Fsyz.ld(i0 - c - 1);
+        zy_p[c] = Fsyz.ld(i0 + c * sy); zy_m[c] = Fsyz.ld(i0 - (c + 1) * sy);
+        zz_p[c] = Fszz.ld(i0 + c + 1); zz_m[c] = Fszz.ld(i0 - c);
       }
-      const float rho_y1 = rhoP.ld(col + sy);
+      const float rho_y1 = Frho.ld(i0 + sy);
       // vx (wgsl3d :1647-1681)
       if (x >= last + 1 && x < nx - last && yi_vx && zi_vx) {
         float a = 0.f, b = 0.f, c_ = 0.f;
@@ -459,9 +482,9 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
       } else {
         vxn = 0.f; vyn = 0.f; vzn = 0.f;
       }
-      vxP.st<CP>(col, vxn);
-      vyP.st<CP>(col, vyn);
-      vzP.st<CP>(col, vzn);
+      Fvx.st<CP>(i0, vxn);
+      Fvy.st<CP>(i0, vyn);
+      Fvz.st<CP>(i0, vzn);
       const unsigned u = __float_as_uint(vxn * vxn + vyn * vyn + vzn * vzn);
       v2bits = v2bits > u ? v2bits : u;
     }
@@ -759,7 +782,9 @@ hipError_t launch_one(bool stress, int visco, const StepArgs &A, int xchunk, int
 
 template <int ORD, bool FAST>
 hipError_t launch_tuned(bool stress, int visco, const StepArgs &A, const KTune &t, hipStream_t st) {
-  if (t.xchunk == 0) {     // plain one-cell-per-thread kernels
+  // the x-march velocity kernel addresses whole fields with 32-bit offsets
+  const bool plain = t.xchunk == 0 || (!stress && A.ncell >= (1L << 30));
+  if (plain) {             // plain one-cell-per-thread kernels
     const dim3 block(BZ, BYY), grid((A.nz + BZ - 1) / BZ, (A.ny + BYY - 1) / BYY, A.nx);
     if (stress) {
       bool done = false;

@@ -132,7 +132,8 @@ int vse_profile(vse_sim *s, int n_steps, float *ms);
  * tile of 64 (z) x ty (y) cells, ty in {1,4,8}; nt = 1 streams the fields
  * touched only at the own cell with non-temporal loads/stores; each
  * workgroup marches through xchunk x-planes (0 = the plain kernel, one cell
- * per thread on a (z, y, x) grid, where ty and nt do not apply);
+ * per thread on a (z, y, x) grid, where ty and nt do not apply; the velocity
+ * pass of a grid of 2^30 cells or more always runs the plain kernel);
  * remap = 1 gives each XCD a contiguous block of tiles instead of the
  * round-robin dispatch order.  Results are identical for every shape; only speed changes.  Picked by the
  * autotuner (taichi_sim_amd/autotune.py); no reference counterpart (the

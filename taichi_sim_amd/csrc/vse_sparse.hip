@@ -100,18 +100,21 @@ __global__ __launch_bounds__(256) void k_batch_ctrl(BatchArgs B, unsigned *out) 
   }
 }
 
-// v2max = max(v2max, slots[0..n))
+// v2max = max(v2max, slots[0..n)): each workgroup folds a strided share of
+// the slots and adds its maximum with one atomicMax
 __global__ __launch_bounds__(256) void k_fold_v2(const unsigned *slots, int n, unsigned *v2max) {
   unsigned v = 0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) v = v > slots[k] ? v : slots[k];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    v = v > slots[k] ? v : slots[k];
   v = block_reduce_max(v);
-  if (threadIdx.x == 0 && v > *v2max) *v2max = v;
+  if (threadIdx.x == 0 && v) atomicMax(v2max, v);
 }
 
 }  // namespace
 
 hipError_t vse_launch_fold_v2(const unsigned *slots, int n, unsigned *v2max, hipStream_t st) {
-  hipLaunchKernelGGL(k_fold_v2, dim3(1), dim3(256), 0, st, slots, n, v2max);
+  const int blocks = n / 2048 + 1;      // about 8 slots per thread, at most a few hundred workgroups
+  hipLaunchKernelGGL(k_fold_v2, dim3(blocks < 512 ? blocks : 512), dim3(256), 0, st, slots, n, v2max);
   return hipGetLastError();
 }
 

@@ -13,6 +13,22 @@ __device__ __forceinline__ float fd_term(float C, float a, float b, float d, flo
   else return C * (a - b) / d;
 }
 
+// A whole field as a raw buffer: a scalar (SGPR) resource descriptor over its
+// bytes, addressed with a 32-bit per-lane offset (no 64-bit address per
+// load).  The hardware range-checks every access: an index outside
+// [0, ncell) -- a negative one wraps above the range -- reads 0, so a
+// cell's stencil operands can be loaded unconditionally; operands read
+// outside an update's interior are never used.  Word 3 0x00020000 is the
+// gfx9-family raw 32-bit buffer format.
+struct FieldBuf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ FieldBuf(const float *base, long ncell)
+      : r(__builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)(ncell * 4), 0x00020000)) {}
+  __device__ __forceinline__ float operator[](int i) const {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)((unsigned)i * 4u), 0, 0));
+  }
+};
+
 // Compact band index of profile index j, or -1 when j is outside the band.
 __device__ __forceinline__ int band_index(const AxisProf &P, int j) {
   return j < P.lo ? j : (j >= P.hi ? P.lo + (j - P.hi) : -1);

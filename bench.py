@@ -164,6 +164,8 @@ def main():
                     help="also time the batch at cache-group budgets 0..320 MB (--workload shots)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 5 launches): rocprof counter calibration")
+    ap.add_argument("--bandwidth-probe", action="store_true",
+                    help="only measure the in-place read/write mixes of the stencil passes (vse_stream_bandwidth)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the shot/slab sharding "
@@ -178,6 +180,8 @@ def main():
         rc = _lib.load().vse_copy_bandwidth(1 << 30, 5, ctypes.byref(gbs))
         print(json.dumps({"copy_bytes_per_launch": 2 << 30, "launches": 6, "gbs": round(gbs.value, 1), "rc": rc}))
         sys.exit(0 if rc == 0 else 1)
+    if args.bandwidth_probe:
+        return bandwidth_probe()
     if args.workload == "shots":
         return bench_shots(args)
     if args.workload == "2d":
@@ -288,6 +292,31 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def bandwidth_probe():
+    """GB/s the device sustains at each stencil pass's read/write mix, without the
+    arithmetic (vse_stream_bandwidth: n_read read-only arrays, n_rw arrays read and
+    rewritten in place, 512^3 floats each), beside the plain copy kernel."""
+    import ctypes
+    from taichi_sim_amd import _lib
+    lib = _lib.load()
+    n = 512 ** 3
+    mixes = {"3D stress pass (6 read-only, 24 read and written)": (6, 24),
+             "3D velocity pass (7 read-only, 3 read and written)": (7, 3),
+             "6 read-only, 12 read and written": (6, 12),
+             "2D stress pass (5 read-only, 12 read and written)": (5, 12),
+             "2D velocity pass (4 read-only, 2 read and written)": (4, 2),
+             "24 read and written only": (0, 24)}
+    out = {}
+    gbs = ctypes.c_float(0.0)
+    rc = lib.vse_copy_bandwidth(1 << 30, 10, ctypes.byref(gbs))
+    out["float4 copy, 2 x 1 GiB"] = round(gbs.value, 1) if rc == 0 else None
+    for name, (r, w) in mixes.items():
+        rc = lib.vse_stream_bandwidth(n, r, w, 5, ctypes.byref(gbs))
+        out[name] = round(gbs.value, 1) if rc == 0 else None
+        print(f"# {name}: {out[name]} GB/s", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "sustained GB/s by read/write mix", "floats_per_array": n, "gbs": out}))
 
 
 def bench_shots(args):

@@ -185,6 +185,16 @@ const char *vse_last_error(vse_sim *s);
  * calibrates rocprof's memory counters. */
 int vse_copy_bandwidth(size_t bytes, int iters, float *gbs);
 
+/* Throughput (GB/s of bytes read plus written) of the read/write mix of a
+ * stencil pass without its arithmetic: n_read read-only arrays and n_rw
+ * arrays read and rewritten in place, n floats each (one element of every
+ * array per thread), iters launches on the current device.  No reference
+ * counterpart: the in-place ceiling the stencil kernels are read against
+ * (e.g. the 3D stress pass: 6 read-only, 24 read-and-written).
+ * n_read + n_rw <= VSE_STREAMS_MAX. */
+#define VSE_STREAMS_MAX 32
+int vse_stream_bandwidth(size_t n, int n_read, int n_rw, int iters, float *gbs);
+
 void vse_destroy(vse_sim *s);
 
 /* ---- batches: many 2D simulations per launch ---------------------------

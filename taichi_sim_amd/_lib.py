@@ -55,6 +55,7 @@ EXPORTS = ["vse_abi_version", "vse_create", "vse_set_material", "vse_set_cpml", 
            "vse_set_receivers", "vse_reset", "vse_run", "vse_profile", "vse_set_tuning", "vse_set_math", "vse_read_field", "vse_read_traces",
            "vse_read_memory", "vse_read_v2max", "vse_get_step", "vse_field_device_ptr",
            "vse_device_bytes", "vse_device_name", "vse_last_error", "vse_destroy", "vse_copy_bandwidth",
+           "vse_stream_bandwidth",
            "vse_batch_create", "vse_batch_run", "vse_batch_profile", "vse_batch_set_rows", "vse_batch_set_group_bytes", "vse_batch_groups",
            "vse_batch_size", "vse_grad_create", "vse_grad_set_tau_scale", "vse_grad_forward", "vse_grad_misfit",
            "vse_grad_backward", "vse_grad_read", "vse_grad_read_state", "vse_grad_segment", "vse_grad_device_bytes",
@@ -114,6 +115,7 @@ def load():
         "vse_last_error": (ctypes.c_char_p, [P]),
         "vse_destroy": (None, [P]),
         "vse_copy_bandwidth": (I, [ctypes.c_size_t, I, fp]),
+        "vse_stream_bandwidth": (I, [ctypes.c_size_t, I, I, I, fp]),
         "vse_batch_create": (I, [ctypes.POINTER(P), I, ctypes.POINTER(P)]),
         "vse_batch_run": (I, [P, I, I, fp]),
         "vse_batch_profile": (I, [P, I, fp]),

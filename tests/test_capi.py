@@ -53,3 +53,9 @@ def test_abi_version_and_arg_validation_without_gpu():
     assert lib.vse_batch_size(None) == 0
     assert lib.vse_batch_last_error(None) == b"null handle"
     assert lib.vse_batch_run(None, 1, 1, None) == _lib.VSE_ERR_ARG
+    # bandwidth probes: the limits are checked before anything is allocated
+    g = ctypes.c_float(0.0)
+    assert lib.vse_stream_bandwidth(1 << 20, 30, 3, 1, ctypes.byref(g)) == _lib.VSE_ERR_ARG   # > 32 arrays
+    assert lib.vse_stream_bandwidth(1 << 20, 2, 0, 1, ctypes.byref(g)) == _lib.VSE_ERR_ARG    # nothing written
+    assert lib.vse_stream_bandwidth(100, 2, 2, 1, ctypes.byref(g)) == _lib.VSE_ERR_ARG        # too small
+    assert lib.vse_copy_bandwidth(100, 1, ctypes.byref(g)) == _lib.VSE_ERR_ARG

@@ -219,3 +219,15 @@ def test_reset_and_rerun_is_deterministic():
     g.close()
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_bandwidth_probes_run():
+    # the roofline's reference points: plain copy and the stencil passes' read/write mixes
+    import ctypes
+    from taichi_sim_amd import _lib
+    lib = _lib.load()
+    g = ctypes.c_float(0.0)
+    assert lib.vse_copy_bandwidth(64 << 20, 3, ctypes.byref(g)) == 0 and 100.0 < g.value < 20000.0
+    for r, w in ((6, 24), (7, 3), (0, 1), (0, 32)):
+        assert lib.vse_stream_bandwidth(1 << 22, r, w, 3, ctypes.byref(g)) == 0, (r, w)
+        assert 100.0 < g.value < 20000.0, (r, w, g.value)

@@ -217,7 +217,6 @@ hipError_t vse_launch_sparse(const SparseArgs &S, hipStream_t st);
 hipError_t vse_launch_batch2d(bool stress, int variant, int visco, int ord, const BatchArgs &B, int total_tiles,
                               int i, hipStream_t st);
 hipError_t vse_launch_advance_batch(const BatchArgs &B, int n, hipStream_t st);
-hipError_t vse_launch_sparse_batch(const BatchArgs &B, hipStream_t st);
 hipError_t vse_launch_batch_ctrl(const BatchArgs &B, unsigned *out, hipStream_t st);
 hipError_t vse_launch_fold_v2(const unsigned *slots, int n, unsigned *v2max, hipStream_t st);
 long vse_tiles2d(int nx, int ny, int rows);

@@ -67,9 +67,6 @@ __device__ __forceinline__ void sparse_body(const SparseArgs &S) {
 
 __global__ __launch_bounds__(1024) void k_sparse(SparseArgs S) { sparse_body(S); }
 
-// one block per simulation of a batch
-__global__ __launch_bounds__(1024) void k_sparse_batch(BatchArgs B) { sparse_body(B.sparse[blockIdx.x]); }
-
 // Largest of a workgroup's values (every thread calls it; thread 0 gets it).
 __device__ __forceinline__ unsigned block_reduce_max(unsigned v) {
   for (int off = 32; off > 0; off >>= 1) {
@@ -120,11 +117,6 @@ hipError_t vse_launch_fold_v2(const unsigned *slots, int n, unsigned *v2max, hip
 
 hipError_t vse_launch_batch_ctrl(const BatchArgs &B, unsigned *out, hipStream_t st) {
   hipLaunchKernelGGL(k_batch_ctrl, dim3(B.n), dim3(256), 0, st, B, out);
-  return hipGetLastError();
-}
-
-hipError_t vse_launch_sparse_batch(const BatchArgs &B, hipStream_t st) {
-  hipLaunchKernelGGL(k_sparse_batch, dim3(B.n), dim3(1024), 0, st, B);
   return hipGetLastError();
 }
 

@@ -90,6 +90,7 @@ def bench_2d(args):
     sim = Simulator(cfg)
     att = attenuation_from_config(cfg)
     g = VSESim.from_simulator(sim, variant="wgsl", att=att)
+    g.set_math(args.math2d)
     ncell = int(np.prod(g.shape))
     print(f"# setup done: grid {g.shape}, {g.device_bytes() / 2**30:.2f} GiB on {g.device_name()}",
           file=sys.stderr, flush=True)
@@ -129,7 +130,8 @@ def bench_2d(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"2D viscoelastic Zener FD {n}^2 (incl. 20-cell CPML), L=3 Q=50, steel plate with "
                                f"16 seeded voids, 64-element array with a focusing law, 64 receivers",
-                   "grid": [n, n], "n_sls": 3, "fd_order": 2, "parallelism": "single GPU"},
+                   "grid": [n, n], "n_sls": 3, "fd_order": 2, "parallelism": "single GPU",
+                   "math": args.math2d},
         "hbm_frac_step": round(gcell * 1e9 * b_step / (HBM_PEAK_GBS * 1e9), 4),
         "bytes_per_cell_step": round(b_step, 2),
         "kernel_ms": {"stress": round(k_ms[0], 4), "velocity": round(k_ms[1], 4),
@@ -153,6 +155,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU-oracle sample (0: skip)")
     ap.add_argument("--workload", choices=("3d", "2d", "shots", "adjoint"), default="3d")
     ap.add_argument("--n2d", type=int, default=2048, help="grid size (--workload 2d)")
+    ap.add_argument("--math2d", choices=("strict", "fast"), default="strict",
+                    help="arithmetic mode (--workload 2d): strict = the reference's operation order")
     ap.add_argument("--fd-dirs", type=int, default=2, help="finite-difference directions (--workload adjoint)")
     ap.add_argument("--store-all", type=int, default=1,
                     help="also time the gradient with every state stored, no recompute (--workload adjoint)")

@@ -141,11 +141,14 @@ int vse_profile(vse_sim *s, int n_steps, float *ms);
  * sim_webgpu_3D_viscoelastic.py:40-42). */
 int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap);
 
-/* Arithmetic mode of the 3D kernels.  VSE_MATH_STRICT (default) keeps the
+/* Arithmetic mode of the forward kernels.  VSE_MATH_STRICT (default) keeps the
  * reference's FP32 operation order term for term (coef*(a-b)/dx as in
- * shader_webgpu_3D_viscoelastic.wgsl:1367-1378).  VSE_MATH_FAST multiplies
- * by f32(coef/dx), 1/k and 1/3 instead of dividing: one extra rounding per
- * term, within the FP32 noise of the reference. */
+ * shader_webgpu_3D_viscoelastic.wgsl:1367-1378 and
+ * shader_webgpu_viscoelastic.wgsl:850 in 2D).  VSE_MATH_FAST multiplies
+ * by f32(coef/dx), 1/k and (3D) 1/3 instead of dividing: one extra rounding
+ * per term, within the FP32 noise of the reference.  2D: the WGSL variant
+ * only (the Taichi variant already multiplies by f32(1/dx)); a batch takes
+ * its members' mode, which must be the same for all of them. */
 #define VSE_MATH_STRICT 0
 #define VSE_MATH_FAST 1
 int vse_set_math(vse_sim *s, int mode);

@@ -442,9 +442,9 @@ static hipError_t enqueue_step(vse_sim *s, const StepArgs &A, const SparseArgs &
   const vse_desc &d = s->d;
   hipError_t e;
   if (d.ndim == 2) {
-    e = vse_launch_stress2d(d.variant, d.visco, d.ord, A, s->tune[0], s->stream);
+    e = vse_launch_stress2d(d.variant, d.visco, d.ord, s->math, A, s->tune[0], s->stream);
     if (e != hipSuccess) return e;
-    return vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->tune[1], i, s->stream);
+    return vse_launch_velocity2d(d.variant, d.visco, d.ord, s->math, A, s->tune[1], i, s->stream);
   } else {
     e = vse_launch_stress3d(d.visco, d.ord, s->math, A, s->tune[0], s->stream);
     if (e != hipSuccess) return e;
@@ -558,10 +558,10 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
   double acc[3] = {0, 0, 0};
   for (int it = 0; it < n_steps && !rc; it++) {
     hipError_t e = hipEventRecord(ev[0], s->stream);
-    if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_stress2d(d.variant, d.visco, d.ord, A, s->tune[0], s->stream)
+    if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_stress2d(d.variant, d.visco, d.ord, s->math, A, s->tune[0], s->stream)
                                          : vse_launch_stress3d(d.visco, d.ord, s->math, A, s->tune[0], s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[1], s->stream);
-    if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_velocity2d(d.variant, d.visco, d.ord, A, s->tune[1], 0, s->stream)
+    if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_velocity2d(d.variant, d.visco, d.ord, s->math, A, s->tune[1], 0, s->stream)
                                          : vse_launch_velocity3d(d.visco, d.ord, s->math, A, s->tune[1], s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
     if (e == hipSuccess) e = d.ndim == 2 ? advance_steps(s, 1) : vse_launch_sparse(S, s->stream);
@@ -698,6 +698,7 @@ struct vse_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   int variant = 0, visco = 0, ord = 0;
+  int math = VSE_MATH_STRICT;                       // the members' arithmetic mode (they must agree)
   int rows = 1;
   size_t group_bytes = 0;
   std::vector<BatchGroup> groups;
@@ -769,8 +770,9 @@ static int regroup(vse_batch *b) {
 // does the members' sources and receivers, and vse_launch_advance_batch
 // moves their counters after the k steps.
 static hipError_t enqueue_group_step(vse_batch *b, const BatchGroup &g, int i) {
-  hipError_t e = vse_launch_batch2d(true, b->variant, b->visco, b->ord, g.B, g.total_tiles, 0, b->stream);
-  if (e == hipSuccess) e = vse_launch_batch2d(false, b->variant, b->visco, b->ord, g.B, g.total_tiles, i, b->stream);
+  hipError_t e = vse_launch_batch2d(true, b->variant, b->visco, b->ord, b->math, g.B, g.total_tiles, 0, b->stream);
+  if (e == hipSuccess)
+    e = vse_launch_batch2d(false, b->variant, b->visco, b->ord, b->math, g.B, g.total_tiles, i, b->stream);
   return e;
 }
 
@@ -778,6 +780,13 @@ static hipError_t enqueue_group_step(vse_batch *b, const BatchGroup &g, int i) {
 // replaced by vse_set_sources / vse_set_receivers since the last run).
 static int upload_batch_args(vse_batch *b) {
   const int n = (int)b->sims.size();
+  const int math = b->sims[0]->math;
+  for (int i = 1; i < n; i++)
+    if (b->sims[i]->math != math) return b->fail(VSE_ERR_ARG, "batch members differ in arithmetic mode");
+  if (math != b->math) {
+    b->drop_graphs();
+    b->math = math;
+  }
   std::vector<StepArgs> A(n);
   std::vector<SparseArgs> S(n);
   for (int i = 0; i < n; i++) {
@@ -965,9 +974,11 @@ int vse_batch_profile(vse_batch *b, int n_steps, float *ms) {
   double acc[3] = {0, 0, 0};
   for (int it = 0; it < n_steps && !rc; it++) {
     hipError_t e = hipEventRecord(ev[0], b->stream);
-    if (e == hipSuccess) e = vse_launch_batch2d(true, b->variant, b->visco, b->ord, g.B, g.total_tiles, 0, b->stream);
+    if (e == hipSuccess)
+      e = vse_launch_batch2d(true, b->variant, b->visco, b->ord, b->math, g.B, g.total_tiles, 0, b->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[1], b->stream);
-    if (e == hipSuccess) e = vse_launch_batch2d(false, b->variant, b->visco, b->ord, g.B, g.total_tiles, 0, b->stream);
+    if (e == hipSuccess)
+      e = vse_launch_batch2d(false, b->variant, b->visco, b->ord, b->math, g.B, g.total_tiles, 0, b->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[2], b->stream);
     if (e == hipSuccess) e = vse_launch_advance_batch(g.B, 1, b->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[3], b->stream);

@@ -44,6 +44,18 @@ __device__ __forceinline__ float cpml_stretch(const float *a, const float *b, co
   return D / k[j] + pn;
 }
 
+// The same with the FAST choice: D times the precomputed f32(1/k) instead of
+// D / k (a rounding-level difference).
+template <bool FAST>
+__device__ __forceinline__ float cpml_stretch_k(const float *a, const float *b, const float *k, const float *ki,
+                                                float *psi, long pidx, int j, bool in_band, float D) {
+  if (!in_band) return D;
+  float pn = b[j] * psi[pidx] + a[j] * D;
+  psi[pidx] = pn;
+  if constexpr (FAST) return D * ki[j] + pn;
+  else return D / k[j] + pn;
+}
+
 // Max over a wavefront, then one atomicMax per wavefront into its
 // workgroup's own slot.  The result is not used, so no wavefront waits on
 // it, and no two workgroups share an address (a single shared word costs each

@@ -201,11 +201,14 @@ hipError_t vse_launch_misfit(const float *traces, const float *obs, float *resid
 // launchers (vse_k2d.hip / vse_k3d.hip)
 // 2D: t.remap = 1 hands each XCD a contiguous band of tiles (other fields of
 // t are not used in 2D)
-hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st);
+// math = VSE_MATH_FAST takes the WGSL variant's differences as (a-b)*f32(C/d)
+// and its CPML 1/k as a product
+hipError_t vse_launch_stress2d(int variant, int visco, int ord, int math, const StepArgs &A, const KTune &t,
+                               hipStream_t st);
 // 2D velocity (single simulation): advances step *A.step + i and does that
 // step's sources and receivers; vse_launch_advance(step, k) moves the counter
 // after k such launches
-hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, int i,
+hipError_t vse_launch_velocity2d(int variant, int visco, int ord, int math, const StepArgs &A, const KTune &t, int i,
                                  hipStream_t st);
 hipError_t vse_launch_advance(int *step, int n, hipStream_t st);
 hipError_t vse_launch_stress3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
@@ -214,8 +217,8 @@ hipError_t vse_launch_sparse(const SparseArgs &S, hipStream_t st);
 // the velocity pass (stress = false) advances every member's step *step + i
 // and does its sources and receivers; vse_launch_advance_batch moves the
 // members' counters after k such steps
-hipError_t vse_launch_batch2d(bool stress, int variant, int visco, int ord, const BatchArgs &B, int total_tiles,
-                              int i, hipStream_t st);
+hipError_t vse_launch_batch2d(bool stress, int variant, int visco, int ord, int math, const BatchArgs &B,
+                              int total_tiles, int i, hipStream_t st);
 hipError_t vse_launch_advance_batch(const BatchArgs &B, int n, hipStream_t st);
 hipError_t vse_launch_batch_ctrl(const BatchArgs &B, unsigned *out, hipStream_t st);
 hipError_t vse_launch_fold_v2(const unsigned *slots, int n, unsigned *v2max, hipStream_t st);

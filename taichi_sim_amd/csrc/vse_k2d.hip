@@ -12,12 +12,28 @@ namespace {
 
 constexpr int BY = 64, BX = 4;
 
+// One staggered-difference term along axis AX: the variant's own form, or
+// with FAST (WGSL variant only) (a - b) * f32(C/d), a rounding-level
+// difference from the reference order C (a - b) / d.
+template <int VAR, bool FAST, int AX>
+__device__ __forceinline__ float dterm2(const StepArgs &A, int c, float a, float b) {
+  if constexpr (FAST) return (a - b) * A.Cd[AX][c];
+  else return fd_term<VAR>(A.C[c], a, b, AX == 0 ? A.dx : A.dy, AX == 0 ? A.one_dx : A.one_dy);
+}
+
+// D / k, or with FAST D times the precomputed f32(1/k)
+template <bool FAST>
+__device__ __forceinline__ float kdiv(float D, const float *k, const float *ki, int j) {
+  if constexpr (FAST) return D * ki[j];
+  else return D / k[j];
+}
+
 
 // LC > 0: the mechanism count is the compile-time LC and every memory-variable
 // load of a point is issued before its arithmetic (same operations, same
 // order: bit-identical to the runtime-L loop, which waits on each
 // mechanism's loads in turn).
-template <int VAR, bool VISCO, int ORD, int LC = 0>
+template <int VAR, bool VISCO, int ORD, int LC = 0, bool FAST = false>
 __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
   const int nx = A.nx, ny = A.ny;
   if (x >= nx || y >= ny) return;
@@ -30,13 +46,15 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
     float dvx = 0.f, dvy = 0.f;
 #pragma unroll
     for (int c = 0; c < ORD; c++) {
-      dvx += fd_term<VAR>(A.C[c], A.vx[id + (long)(c + 1) * ny], A.vx[id - (long)c * ny], A.dx, A.one_dx);
-      dvy += fd_term<VAR>(A.C[c], A.vy[id + c], A.vy[id - c - 1], A.dy, A.one_dy);
+      dvx += dterm2<VAR, FAST, 0>(A, c, A.vx[id + (long)(c + 1) * ny], A.vx[id - (long)c * ny]);
+      dvy += dterm2<VAR, FAST, 1>(A, c, A.vy[id + c], A.vy[id - c - 1]);
     }
     const int jx = x - off, jy = y - off;
     const int bx = band_index(A.ax[0], jx), by = band_index(A.ax[1], jy);
-    dvx = cpml_stretch(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, A.psi[P_DVX_DX], (long)bx * ny + y, jx, bx >= 0, dvx);
-    dvy = cpml_stretch(A.ax[1].a, A.ax[1].b, A.ax[1].k, A.psi[P_DVY_DY], (long)x * A.ax[1].nb + by, jy, by >= 0, dvy);
+    dvx = cpml_stretch_k<FAST>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, A.ax[0].khi, A.psi[P_DVX_DX], (long)bx * ny + y, jx,
+                               bx >= 0, dvx);
+    dvy = cpml_stretch_k<FAST>(A.ax[1].a, A.ax[1].b, A.ax[1].k, A.ax[1].ki, A.psi[P_DVY_DY], (long)x * A.ax[1].nb + by,
+                               jy, by >= 0, dvy);
 
     const float rho_h = 0.5f * (A.rho[id + ny] + A.rho[id]);
     const float cp_h = 0.5f * (A.cp[id + ny] + A.cp[id]);
@@ -99,13 +117,15 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
     float dvydx = 0.f, dvxdy = 0.f;
 #pragma unroll
     for (int c = 0; c < ORD; c++) {
-      dvydx += fd_term<VAR>(A.C[c], A.vy[id + (long)c * ny], A.vy[id - (long)(c + 1) * ny], A.dx, A.one_dx);
-      dvxdy += fd_term<VAR>(A.C[c], A.vx[id + c + 1], A.vx[id - c], A.dy, A.one_dy);
+      dvydx += dterm2<VAR, FAST, 0>(A, c, A.vy[id + (long)c * ny], A.vy[id - (long)(c + 1) * ny]);
+      dvxdy += dterm2<VAR, FAST, 1>(A, c, A.vx[id + c + 1], A.vx[id - c]);
     }
     const int jx = x - off, jy = y - off;
     const int bx = band_index(A.ax[0], jx), by = band_index(A.ax[1], jy);
-    dvydx = cpml_stretch(A.ax[0].a, A.ax[0].b, A.ax[0].k, A.psi[P_DVY_DX], (long)bx * ny + y, jx, bx >= 0, dvydx);
-    dvxdy = cpml_stretch(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, A.psi[P_DVX_DY], (long)x * A.ax[1].nb + by, jy, by >= 0, dvxdy);
+    dvydx = cpml_stretch_k<FAST>(A.ax[0].a, A.ax[0].b, A.ax[0].k, A.ax[0].ki, A.psi[P_DVY_DX], (long)bx * ny + y, jx,
+                                 bx >= 0, dvydx);
+    dvxdy = cpml_stretch_k<FAST>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, A.ax[1].khi, A.psi[P_DVX_DY],
+                                 (long)x * A.ax[1].nb + by, jy, by >= 0, dvxdy);
 
     const float rho_h = 0.5f * (A.rho[id + 1] + A.rho[id]);
     const float csa = A.cs[id + 1], csb = A.cs[id];
@@ -145,7 +165,7 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
 // results are the same; a division right after a load would otherwise make
 // the wavefront wait on it before issuing the loads that follow.
 // s: the step being advanced (the Taichi-variant source row).
-template <int VAR, bool VISCO, int ORD>
+template <int VAR, bool VISCO, int ORD, bool FAST = false>
 __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, int y, int s) {
   const int nx = A.nx, ny = A.ny;
   const bool valid = x < nx && y < ny;
@@ -198,18 +218,18 @@ __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, in
       float d1 = 0.f, d2 = 0.f;
 #pragma unroll
       for (int c = 0; c < ORD; c++) {
-        d1 += fd_term<VAR>(A.C[c], xx_p[c], xx_m[c], A.dx, A.one_dx);
-        d2 += fd_term<VAR>(A.C[c], xy_p[c], xy_m[c], A.dy, A.one_dy);
+        d1 += dterm2<VAR, FAST, 0>(A, c, xx_p[c], xx_m[c]);
+        d2 += dterm2<VAR, FAST, 1>(A, c, xy_p[c], xy_m[c]);
       }
       if (bx >= 0) {
         const float pn = A.ax[0].b[jx] * q1 + A.ax[0].a[jx] * d1;
         A.psi[P_DSXX_DX][pxi] = pn;
-        d1 = d1 / A.ax[0].k[jx] + pn;
+        d1 = kdiv<FAST>(d1, A.ax[0].k, A.ax[0].ki, jx) + pn;
       }
       if (by >= 0) {
         const float pn = A.ax[1].b[jy] * q2 + A.ax[1].a[jy] * d2;
         A.psi[P_DSXY_DY][pyi] = pn;
-        d2 = d2 / A.ax[1].k[jy] + pn;
+        d2 = kdiv<FAST>(d2, A.ax[1].k, A.ax[1].ki, jy) + pn;
       }
       if (r00 > 0.f) {
         if constexpr (VAR == VSE_VAR_TAICHI) vxn += (d1 + d2) * dt / r00;
@@ -223,18 +243,18 @@ __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, in
       float d1 = 0.f, d2 = 0.f;
 #pragma unroll
       for (int c = 0; c < ORD; c++) {
-        d1 += fd_term<VAR>(A.C[c], yx_p[c], yx_m[c], A.dx, A.one_dx);
-        d2 += fd_term<VAR>(A.C[c], yy_p[c], yy_m[c], A.dy, A.one_dy);
+        d1 += dterm2<VAR, FAST, 0>(A, c, yx_p[c], yx_m[c]);
+        d2 += dterm2<VAR, FAST, 1>(A, c, yy_p[c], yy_m[c]);
       }
       if (bx >= 0) {
         const float pn = A.ax[0].bh[jx] * q3 + A.ax[0].ah[jx] * d1;
         A.psi[P_DSXY_DX][pxi] = pn;
-        d1 = d1 / A.ax[0].kh[jx] + pn;
+        d1 = kdiv<FAST>(d1, A.ax[0].kh, A.ax[0].khi, jx) + pn;
       }
       if (by >= 0) {
         const float pn = A.ax[1].bh[jy] * q4 + A.ax[1].ah[jy] * d2;
         A.psi[P_DSYY_DY][pyi] = pn;
-        d2 = d2 / A.ax[1].kh[jy] + pn;
+        d2 = kdiv<FAST>(d2, A.ax[1].kh, A.ax[1].khi, jy) + pn;
       }
       if constexpr (VAR == VSE_VAR_TAICHI) {
         const float rho = 0.25f * (r01 + r00 + r11 + r10);
@@ -275,10 +295,10 @@ __device__ __forceinline__ Tile2 tile2d(int nty, int remap) {
   return Tile2{t % nty, t / nty};
 }
 
-template <int VAR, bool VISCO, int ORD, int LC = 0>
+template <int VAR, bool VISCO, int ORD, int LC = 0, bool FAST = false>
 __global__ __launch_bounds__(BX * BY) void k_stress2d(StepArgs A, int nty, int remap) {
   const Tile2 t = tile2d(nty, remap);
-  stress2d_cell<VAR, VISCO, ORD, LC>(A, t.tx * BX + threadIdx.y, t.ty * BY + threadIdx.x);
+  stress2d_cell<VAR, VISCO, ORD, LC, FAST>(A, t.tx * BX + threadIdx.y, t.ty * BY + threadIdx.x);
 }
 
 // The step's sparse work, done inside the velocity pass (vse_sparse.hip does
@@ -330,12 +350,12 @@ __device__ __forceinline__ void record_receivers(const StepArgs &A, int s) {
 }
 
 // Velocity pass of one simulation, with its step's sparse work.
-template <int VAR, bool VISCO, int ORD>
+template <int VAR, bool VISCO, int ORD, bool FAST = false>
 __global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A, int nty, int remap, int i) {
   const Tile2 t = tile2d(nty, remap);
   const int x = t.tx * BX + threadIdx.y, y = t.ty * BY + threadIdx.x;
   const int s = *A.step + i;
-  unsigned v2 = velocity2d_cell<VAR, VISCO, ORD>(A, x, y, s);
+  unsigned v2 = velocity2d_cell<VAR, VISCO, ORD, FAST>(A, x, y, s);
   source_cell<VAR>(A, x, y, s, v2);
   if (blockIdx.x == 0) record_receivers(A, s);
   wave_max_to_slot(v2, A.v2slots + blockIdx.x);
@@ -366,7 +386,7 @@ __device__ __forceinline__ int batch_instance(const BatchArgs &B, int b) {
   return lo;
 }
 
-template <int VAR, bool VISCO, int ORD, int LC = 0>
+template <int VAR, bool VISCO, int ORD, int LC = 0, bool FAST = false>
 __global__ __launch_bounds__(BX * BY) void k_stress2d_batch(BatchArgs B) {
   const int b = xcd_tile(blockIdx.x, gridDim.x, 1);     // XCD-contiguous tile order, as tile2d
   const int i = batch_instance(B, b);
@@ -375,12 +395,12 @@ __global__ __launch_bounds__(BX * BY) void k_stress2d_batch(BatchArgs B) {
   const int nyb = (A.ny + BY - 1) / BY;
   const int y = (t % nyb) * BY + threadIdx.x;
   const int x0 = (t / nyb) * BX * B.rows + threadIdx.y;
-  for (int k = 0; k < B.rows; k++) stress2d_cell<VAR, VISCO, ORD, LC>(A, x0 + k * BX, y);
+  for (int k = 0; k < B.rows; k++) stress2d_cell<VAR, VISCO, ORD, LC, FAST>(A, x0 + k * BX, y);
 }
 
 // One launch over the tiles of every member; each member's step is
 // *step + i and its first tile records its receivers.
-template <int VAR, bool VISCO, int ORD>
+template <int VAR, bool VISCO, int ORD, bool FAST = false>
 __global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B, int i) {
   const int b = xcd_tile(blockIdx.x, gridDim.x, 1);
   const int m = batch_instance(B, b);
@@ -392,7 +412,7 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B, int i
   unsigned v2 = 0;
   const int s = *A.step + i;
   for (int k = 0; k < B.rows; k++) {
-    unsigned u = velocity2d_cell<VAR, VISCO, ORD>(A, x0 + k * BX, y, s);
+    unsigned u = velocity2d_cell<VAR, VISCO, ORD, FAST>(A, x0 + k * BX, y, s);
     source_cell<VAR>(A, x0 + k * BX, y, s, u);
     v2 = v2 > u ? v2 : u;
   }
@@ -400,7 +420,7 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B, int i
   wave_max_to_slot(v2, A.v2slots + t);
 }
 
-template <int VAR, bool VISCO>
+template <int VAR, bool VISCO, bool FAST>
 hipError_t launch_ord(bool stress, int ord, const StepArgs &A, const KTune &t, int i, hipStream_t st) {
   const dim3 block(BY, BX);
   const int nty = (A.ny + BY - 1) / BY, ntx = (A.nx + BX - 1) / BX;
@@ -409,9 +429,9 @@ hipError_t launch_ord(bool stress, int ord, const StepArgs &A, const KTune &t, i
   const bool l3 = VISCO && VAR == VSE_VAR_WGSL && A.L == 3;
 #define VSE_CASE(O)                                                                                        \
   case O:                                                                                                  \
-    if (stress && l3) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O, 3>), grid, block, 0, st, A, nty, rm); \
-    else if (stress) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O>), grid, block, 0, st, A, nty, rm);     \
-    else hipLaunchKernelGGL((k_velocity2d<VAR, VISCO, O>), grid, block, 0, st, A, nty, rm, i);            \
+    if (stress && l3) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O, 3, FAST>), grid, block, 0, st, A, nty, rm); \
+    else if (stress) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O, 0, FAST>), grid, block, 0, st, A, nty, rm); \
+    else hipLaunchKernelGGL((k_velocity2d<VAR, VISCO, O, FAST>), grid, block, 0, st, A, nty, rm, i);          \
     break;
   switch (ord) {
     VSE_CASE(2) VSE_CASE(3) VSE_CASE(4) VSE_CASE(5) VSE_CASE(6)
@@ -421,15 +441,15 @@ hipError_t launch_ord(bool stress, int ord, const StepArgs &A, const KTune &t, i
   return hipGetLastError();
 }
 
-template <int VAR, bool VISCO>
+template <int VAR, bool VISCO, bool FAST>
 hipError_t launch_batch_ord(bool stress, int ord, const BatchArgs &B, int total_tiles, int i, hipStream_t st) {
   dim3 block(BY, BX), grid(total_tiles);
   const bool l3 = VISCO && VAR == VSE_VAR_WGSL && B.L == 3;
 #define VSE_CASE(O)                                                                              \
   case O:                                                                                        \
-    if (stress && l3) hipLaunchKernelGGL((k_stress2d_batch<VAR, VISCO, O, 3>), grid, block, 0, st, B); \
-    else if (stress) hipLaunchKernelGGL((k_stress2d_batch<VAR, VISCO, O>), grid, block, 0, st, B); \
-    else hipLaunchKernelGGL((k_velocity2d_batch<VAR, VISCO, O>), grid, block, 0, st, B, i);      \
+    if (stress && l3) hipLaunchKernelGGL((k_stress2d_batch<VAR, VISCO, O, 3, FAST>), grid, block, 0, st, B); \
+    else if (stress) hipLaunchKernelGGL((k_stress2d_batch<VAR, VISCO, O, 0, FAST>), grid, block, 0, st, B); \
+    else hipLaunchKernelGGL((k_velocity2d_batch<VAR, VISCO, O, FAST>), grid, block, 0, st, B, i);          \
     break;
   switch (ord) {
     VSE_CASE(2) VSE_CASE(3) VSE_CASE(4) VSE_CASE(5) VSE_CASE(6)
@@ -439,20 +459,28 @@ hipError_t launch_batch_ord(bool stress, int ord, const BatchArgs &B, int total_
   return hipGetLastError();
 }
 
-hipError_t dispatch(bool stress, int variant, int visco, int ord, const StepArgs &A, const KTune &t, int i,
+// FAST applies to the WGSL variant (the Taichi variant already multiplies by f32(1/d)).
+hipError_t dispatch(bool stress, int variant, int visco, int ord, int math, const StepArgs &A, const KTune &t, int i,
                     hipStream_t st) {
-  if (variant == VSE_VAR_TAICHI) return launch_ord<VSE_VAR_TAICHI, false>(stress, ord, A, t, i, st);
-  if (visco) return launch_ord<VSE_VAR_WGSL, true>(stress, ord, A, t, i, st);
-  return launch_ord<VSE_VAR_WGSL, false>(stress, ord, A, t, i, st);
+  const bool fast = math == VSE_MATH_FAST;
+  if (variant == VSE_VAR_TAICHI) return launch_ord<VSE_VAR_TAICHI, false, false>(stress, ord, A, t, i, st);
+  if (visco) return fast ? launch_ord<VSE_VAR_WGSL, true, true>(stress, ord, A, t, i, st)
+                         : launch_ord<VSE_VAR_WGSL, true, false>(stress, ord, A, t, i, st);
+  return fast ? launch_ord<VSE_VAR_WGSL, false, true>(stress, ord, A, t, i, st)
+              : launch_ord<VSE_VAR_WGSL, false, false>(stress, ord, A, t, i, st);
 }
 
 }  // namespace
 
-hipError_t vse_launch_batch2d(bool stress, int variant, int visco, int ord, const BatchArgs &B, int total_tiles,
-                              int i, hipStream_t st) {
-  if (variant == VSE_VAR_TAICHI) return launch_batch_ord<VSE_VAR_TAICHI, false>(stress, ord, B, total_tiles, i, st);
-  if (visco) return launch_batch_ord<VSE_VAR_WGSL, true>(stress, ord, B, total_tiles, i, st);
-  return launch_batch_ord<VSE_VAR_WGSL, false>(stress, ord, B, total_tiles, i, st);
+hipError_t vse_launch_batch2d(bool stress, int variant, int visco, int ord, int math, const BatchArgs &B,
+                              int total_tiles, int i, hipStream_t st) {
+  const bool fast = math == VSE_MATH_FAST;
+  if (variant == VSE_VAR_TAICHI)
+    return launch_batch_ord<VSE_VAR_TAICHI, false, false>(stress, ord, B, total_tiles, i, st);
+  if (visco) return fast ? launch_batch_ord<VSE_VAR_WGSL, true, true>(stress, ord, B, total_tiles, i, st)
+                         : launch_batch_ord<VSE_VAR_WGSL, true, false>(stress, ord, B, total_tiles, i, st);
+  return fast ? launch_batch_ord<VSE_VAR_WGSL, false, true>(stress, ord, B, total_tiles, i, st)
+              : launch_batch_ord<VSE_VAR_WGSL, false, false>(stress, ord, B, total_tiles, i, st);
 }
 
 hipError_t vse_launch_advance_batch(const BatchArgs &B, int n, hipStream_t st) {
@@ -464,12 +492,13 @@ long vse_tiles2d(int nx, int ny, int rows) {
   return (long)((ny + BY - 1) / BY) * (((nx + BX - 1) / BX + rows - 1) / rows);
 }
 
-hipError_t vse_launch_stress2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, hipStream_t st) {
-  return dispatch(true, variant, visco, ord, A, t, 0, st);
+hipError_t vse_launch_stress2d(int variant, int visco, int ord, int math, const StepArgs &A, const KTune &t,
+                               hipStream_t st) {
+  return dispatch(true, variant, visco, ord, math, A, t, 0, st);
 }
-hipError_t vse_launch_velocity2d(int variant, int visco, int ord, const StepArgs &A, const KTune &t, int i,
+hipError_t vse_launch_velocity2d(int variant, int visco, int ord, int math, const StepArgs &A, const KTune &t, int i,
                                  hipStream_t st) {
-  return dispatch(false, variant, visco, ord, A, t, i, st);
+  return dispatch(false, variant, visco, ord, math, A, t, i, st);
 }
 hipError_t vse_launch_advance(int *step, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, step, n);

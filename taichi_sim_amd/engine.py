@@ -202,7 +202,12 @@ class VSESim:
         self._check(self._lib.vse_set_tuning(self._h, int(kernel), int(ty), int(nt), int(xchunk), int(remap)))
 
     def set_math(self, mode):
-        """'strict' (reference operation order, default) or 'fast' (3D kernels)."""
+        """'strict' (reference operation order, default) or 'fast'.
+
+        'fast' multiplies by f32(coef/d) and f32(1/k) instead of dividing (one
+        extra FP32 rounding per term).  In 2D it applies to the WGSL variant; the
+        members of a batch must all use the same mode.
+        """
         self._check(self._lib.vse_set_math(self._h, {"strict": 0, "fast": 1}[mode]))
 
     def band_cells(self):

@@ -194,3 +194,36 @@ def test_batch_row_groups_bit_identical():
                 _assert_same(a, c, f"rows {rows}")
         for g in gs:
             g.close()
+
+
+def test_batch_fast_math_matches_solo_fast_runs():
+    """A batch takes its members' arithmetic mode: FAST batch members end exactly
+    where FAST solo runs end, and members whose modes differ are refused."""
+    steps = 150
+    print("batch fast math", flush=True)
+    members = _shots((96, 131), 2, steps)
+    gs = [m[3] for m in members]
+    for g in gs:
+        g.set_math("fast")
+    batch = VSEBatch(gs)
+    batch.set_group_bytes(0)
+    batch.run(steps, check_every=64)
+    for sim, law, att, g in members:
+        sim.select_law(law)
+        solo = VSESim.from_simulator(sim, variant="wgsl", att=att)
+        solo.set_math("fast")
+        solo.run(steps, check_every=64)
+        _assert_same(_state(g), _state(solo), ("fast", sim._nx, law))
+        strict = VSESim.from_simulator(sim, variant="wgsl", att=att)
+        strict.run(steps, check_every=64)
+        # one extra FP32 rounding per term: close to, but not the same bits as, the strict run
+        rel = oracle.rel_l2(_state(g)["syy"], _state(strict)["syy"])
+        assert rel <= 1e-5, rel
+        solo.close()
+        strict.close()
+    gs[0].set_math("strict")
+    with pytest.raises(Exception, match="arithmetic mode"):
+        batch.run(5)
+    batch.close()
+    for g in gs:
+        g.close()

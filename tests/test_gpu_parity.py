@@ -162,6 +162,30 @@ def test_3d_fast_math_within_tolerance(visco):
     g.close()
 
 
+@pytest.mark.parametrize("acc", [2, 4])
+def test_2d_fast_math_within_tolerance(acc):
+    """FAST arithmetic in 2D (precomputed coef/d, 1/k) stays within 1e-5 of the oracle."""
+    cfg = workloads.visco_2d_array(n=(120, 104), steps=300, n_elem=8, n_voids=3)
+    cfg["simul_params"]["acc"] = acc
+    sim = Simulator(cfg)
+    att = attenuation_from_config(cfg)
+    o = oracle.Oracle(sim, variant="wgsl", att=att).run(300)
+    g = VSESim.from_simulator(sim, variant="wgsl", att=att)
+    g.set_math("fast")
+    g.run(300, check_every=64)
+    worst = 0.0
+    for k in o.f:
+        rel = oracle.rel_l2(g.field(k), o.f[k])
+        worst = max(worst, rel)
+        assert rel <= TOL, (k, rel)
+    for ci, comp in ((0, "xx"), (1, "yy"), (3, "xy")):
+        assert oracle.rel_l2(g.memory(comp), o.r[ci]) <= TOL, comp
+    rel = oracle.rel_l2(g.traces(), o.traces)
+    assert rel <= TOL, rel
+    print(f"2D fast math worst field rel-L2 {worst:.2e}, traces {rel:.2e}")
+    g.close()
+
+
 def test_3d_ragged_order3():
     cfg = workloads.visco_3d_block(n=(55, 53, 50), steps=50, n_incl=2, n_rec_side=3)
     cfg["simul_params"]["acc"] = 3

@@ -45,13 +45,19 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch of the 512^3 viscoelastic stress kernel, by launch-shape
-# family (ty, non-temporal, x-march), measured with rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE (scaled by 2.0 and 1.0 from the copy-kernel calibration); the
-# table and commands are in profiles/visco3d_512cube_kernels.md.
-STRESS_TRAFFIC_512 = {(4, 0, True): 32.69e9, (1, 0, True): 33.66e9, (4, 1, True): 31.68e9,
-                      (1, 1, True): 31.64e9, (8, 0, True): 33.68e9, (8, 1, True): 31.65e9,
-                      (4, 0, False): 38.90e9}
+# Launch shapes (ty, non-temporal, x-chunk, tile order) the 3D autotuner
+# chooses from in this benchmark: the fastest ones on MI355X, each with its
+# stress-kernel HBM traffic measured below.
+SHAPES_3D = [(4, 0, 4, 2), (4, 0, 8, 2), (4, 0, 16, 2), (1, 0, 4, 2), (1, 0, 8, 2), (1, 0, 16, 2),
+             (4, 0, 4, 1), (4, 0, 8, 1), (1, 0, 4, 1), (1, 0, 8, 1)]
+# HBM bytes per launch of the 512^3 viscoelastic FAST stress kernel by launch
+# shape, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over
+# `bench.py --shape-sweep` (scaled by 2.0 and 1.0 from the copy-kernel
+# calibration); the table and commands are in profiles/visco3d_512cube_kernels.md.
+STRESS_TRAFFIC_512 = {(4, 0, 4, 2): 32.05e9, (4, 0, 8, 2): 32.38e9, (4, 0, 16, 2): 33.20e9,
+                      (1, 0, 4, 2): 31.97e9, (1, 0, 8, 2): 32.03e9, (1, 0, 16, 2): 32.80e9,
+                      (4, 0, 4, 1): 32.29e9, (4, 0, 8, 1): 32.54e9, (1, 0, 4, 1): 32.22e9,
+                      (1, 0, 8, 1): 32.14e9}
 # The same for the 2048^2 2D viscoelastic stress kernel (profiles/visco2d_2048_kernels.md).
 STRESS2D_TRAFFIC_2048 = 489.5e6
 
@@ -145,6 +151,25 @@ def bench_2d(args):
     print(json.dumps(line), flush=True)
 
 
+def shape_sweep(g, k):
+    """k stress and velocity launches (HIP-event timed) of the FAST stress pass
+    at each shape of SHAPES_3D in turn, the velocity pass fixed: the run that
+    rocprofv3 --pmc counts per dispatch (after 2 warm-up dispatches at the
+    first shape, dispatch 2 + i of the stress kernel has shape SHAPES_3D[i // k])."""
+    g.set_math("fast")
+    g.set_tuning(0, *SHAPES_3D[0])
+    g.set_tuning(1, *SHAPES_3D[0])
+    g.profile(2)                                  # warm-up, not counted per shape
+    out = []
+    for shape in SHAPES_3D:
+        g.set_tuning(0, *shape)
+        ms = g.profile(k)
+        out.append({"shape": list(shape), "stress_ms": round(ms[0], 4)})
+        print(f"# stress shape {shape}: {ms[0]:.4f} ms", file=sys.stderr, flush=True)
+    g.close()
+    print(json.dumps({"shape_sweep": out, "dispatches_per_shape": k}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +179,12 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU-oracle sample (0: skip)")
     ap.add_argument("--workload", choices=("3d", "2d", "shots", "adjoint"), default="3d")
+    ap.add_argument("--tuning", default="",
+                    help="3D: pin the launch shapes instead of autotuning, "
+                         "ty,nt,xchunk,remap/ty,nt,xchunk,remap (stress/velocity)")
+    ap.add_argument("--shape-sweep", type=int, default=0,
+                    help="3D: only time k launches of the stress pass at each shape of SHAPES_3D "
+                         "(the run the traffic table is counted on)")
     ap.add_argument("--n2d", type=int, default=2048, help="grid size (--workload 2d)")
     ap.add_argument("--math2d", choices=("strict", "fast"), default="strict",
                     help="arithmetic mode (--workload 2d): strict = the reference's operation order")
@@ -208,14 +239,21 @@ def main():
 
     # launch shapes from the autotuner (before the timed region; it advances
     # the state, so the simulation restarts from zero afterwards)
-    cands = [autotune.PLAIN] + [(ty, nt, xc, rm) for ty in autotune.TY for nt in (0, 1) for xc in (4, 8, 16)
-                                for rm in (0, 1)]
+    if args.shape_sweep:
+        return shape_sweep(g, args.shape_sweep)
     tuned = {}
-    for math in ("fast", "strict"):
-        g.set_math(math)
-        best = autotune.tune(g, steps=2, candidates=cands, cache=None)
-        tuned[math] = {k: best[k] for k in ("stress", "velocity")}
-    print(f"# tuned launch shapes (ty, nt, xchunk, remap, step ms): {tuned}", file=sys.stderr, flush=True)
+    if args.tuning:
+        pinned = [tuple(int(v) for v in part.split(",")) for part in args.tuning.split("/")]
+        if len(pinned) != 2 or any(len(p) != 4 for p in pinned):
+            raise SystemExit("--tuning takes ty,nt,xchunk,remap/ty,nt,xchunk,remap (stress/velocity)")
+        for math in ("fast", "strict"):
+            tuned[math] = {"stress": pinned[0] + (None,), "velocity": pinned[1] + (None,)}
+    else:
+        for math in ("fast", "strict"):
+            g.set_math(math)
+            best = autotune.tune(g, steps=2, candidates=SHAPES_3D, cache=None)
+            tuned[math] = {k: best[k] for k in ("stress", "velocity")}
+    print(f"# launch shapes (ty, nt, xchunk, remap, step ms): {tuned}", file=sys.stderr, flush=True)
 
     def timed_run(math, steps):
         g.set_math(math)
@@ -243,8 +281,7 @@ def main():
 
     # per-kernel durations (HIP events around each launch), after the timed loop
     k_ms = g.profile(args.profile_steps)
-    ty, nt, xc = tuned["fast"]["stress"][:3]
-    traffic = STRESS_TRAFFIC_512.get((ty, nt if xc else 0, xc > 0)) if args.n == 512 else None
+    traffic = STRESS_TRAFFIC_512.get(tuple(tuned["fast"]["stress"][:4])) if args.n == 512 else None
     b_step, b_stress, b_vel = bytes_per_cell(g, g.n_sls)
     achieved = b_stress * ncell / (k_ms[0] * 1e-3) / 1e9
     step_frac = gcell * 1e9 * b_step / (HBM_PEAK_GBS * 1e9)

@@ -135,7 +135,8 @@ int vse_profile(vse_sim *s, int n_steps, float *ms);
  * per thread on a (z, y, x) grid, where ty and nt do not apply; the velocity
  * pass of a grid of 2^30 cells or more always runs the plain kernel);
  * remap = 1 gives each XCD a contiguous block of tiles instead of the
- * round-robin dispatch order.  Results are identical for every shape; only speed changes.  Picked by the
+ * round-robin dispatch order; remap = 2 gives each XCD a contiguous band of
+ * y-tiles that it walks x-chunk after x-chunk (2D: 2 acts as 1).  Results are identical for every shape; only speed changes.  Picked by the
  * autotuner (taichi_sim_amd/autotune.py); no reference counterpart (the
  * reference fixes its workgroup to gcd(n, 16) per axis,
  * sim_webgpu_3D_viscoelastic.py:40-42). */

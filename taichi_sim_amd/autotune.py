@@ -18,7 +18,7 @@ TY = (1, 4, 8)
 NT = (0, 1)
 XCHUNK = (1, 2, 4, 8, 16, 32)
 PLAIN = (4, 0, 0, 0)      # xchunk 0: one cell per thread
-REMAP = (0, 1)
+REMAP = (0, 1, 2)
 
 
 def problem_key(g):

@@ -584,8 +584,8 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
 
 int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap) {
   if (check_handle(s) || kernel < 0 || kernel > 1) return VSE_ERR_ARG;
-  if (!(ty == 1 || ty == 4 || ty == 8) || (nt != 0 && nt != 1) || xchunk < 0 || (remap != 0 && remap != 1))
-    return s->fail(VSE_ERR_ARG, "tuning: ty in {1,4,8}, nt in {0,1}, xchunk >= 0, remap in {0,1}");
+  if (!(ty == 1 || ty == 4 || ty == 8) || (nt != 0 && nt != 1) || xchunk < 0 || remap < 0 || remap > 2)
+    return s->fail(VSE_ERR_ARG, "tuning: ty in {1,4,8}, nt in {0,1}, xchunk >= 0, remap in {0,1,2}");
   OnDevice on_dev(s->device);
   s->tune[kernel] = KTune{ty, nt, xchunk, remap};
   s->drop_graph();

@@ -2,8 +2,8 @@
 
 For one simulation handle it times every candidate (tile height ty, non-
 temporal streaming nt, x-chunk length, tile order) of the stress and velocity
-passes with HIP events (``VSESim.profile``) and keeps the one with the fastest
-*step* (stress + velocity back to back: one pass's speed depends on what the
+passes with HIP events (``VSESim.profile``), re-times the fastest few over
+more steps, and keeps the one with the fastest *step* (stress + velocity back to back: one pass's speed depends on what the
 other leaves in L2 / MALL), tuning stress first, then velocity, and caches the
 choice per problem key (ndim, grid, FD order, visco, L) in a JSON file.  All
 shapes compute identical results, so tuning only changes speed.  The handle's
@@ -33,8 +33,14 @@ def _load(path):
         return {}
 
 
-def tune(g, steps=3, candidates=None, cache=DEFAULT_CACHE, verbose=False, use_cache=True):
+def tune(g, steps=3, candidates=None, cache=DEFAULT_CACHE, verbose=False, use_cache=True, finalists=3,
+         final_steps=8):
     """Pick and apply the fastest launch shape of each 3D pass of ``g``.
+
+    Every candidate is timed over ``steps`` steps; the ``finalists`` fastest are
+    timed again, in turn, over ``final_steps`` steps (shapes within a few percent
+    of each other are not told apart reliably by a short timing), and the fastest
+    of that second round is kept.
 
     Returns {"stress": (ty, nt, xchunk, remap, step_ms), "velocity": (...), "table": [...]}.
     """
@@ -61,7 +67,15 @@ def tune(g, steps=3, candidates=None, cache=DEFAULT_CACHE, verbose=False, use_ca
             table.append((name, ty, nt, xc, rm, ms))
             if verbose:
                 print(f"  {name:8s} ty={ty} nt={nt} xchunk={xc:3d} remap={rm}: {ms:.3f} ms", flush=True)
-        ms, ty, nt, xc, rm = min(results)
+        final = []
+        for _, ty, nt, xc, rm in sorted(results)[:max(1, finalists)]:
+            g.set_tuning(k, ty, nt, xc, rm)
+            t = g.profile(final_steps)
+            final.append((t[0] + t[1], ty, nt, xc, rm))
+            if verbose:
+                print(f"  {name:8s} finalist ty={ty} nt={nt} xchunk={xc:3d} remap={rm}: {final[-1][0]:.3f} ms",
+                      flush=True)
+        ms, ty, nt, xc, rm = min(final)
         g.set_tuning(k, ty, nt, xc, rm)
         best[name] = (ty, nt, xc, rm, ms)
     if cache:

@@ -117,6 +117,25 @@ def test_3d_block(visco):
     _compare(Simulator3D(cfg), "wgsl", 80, att=attenuation_from_config(cfg), visco=visco)
 
 
+@pytest.mark.parametrize("acc", [4, 5, 6])
+def test_3d_high_order(acc):
+    # FD orders 4-6 on a ragged grid, x-march in tile order 2 for both passes
+    cfg = workloads.visco_3d_block(n=(50, 48, 52), steps=40, n_incl=2, n_rec_side=3)
+    cfg["simul_params"]["acc"] = acc
+    sim = Simulator3D(cfg)
+    att = attenuation_from_config(cfg)
+    o = oracle.Oracle(sim, variant="wgsl", att=att).run(40)
+    g = VSESim.from_simulator(sim, variant="wgsl", att=att)
+    g.set_tuning(0, 4, 0, 4, 2)
+    g.set_tuning(1, 1, 0, 8, 2)
+    g.run(40)
+    for k in o.f:
+        if np.abs(o.f[k]).max() > 0:
+            assert oracle.rel_l2(g.field(k), o.f[k]) <= TOL, (acc, k)
+    assert oracle.rel_l2(g.traces(), o.traces) <= TOL
+    g.close()
+
+
 def test_3d_launch_shapes_bit_identical():
     """Every 3D launch shape the autotuner may pick gives the same bits."""
     cfg = workloads.visco_3d_block(n=(70, 64, 130), steps=40, n_incl=3, n_rec_side=3)     # ny/ty % 8 == 0

@@ -40,7 +40,7 @@ def tune(g, steps=3, candidates=None, cache=DEFAULT_CACHE, verbose=False, use_ca
     Every candidate is timed over ``steps`` steps; the ``finalists`` fastest are
     timed again, in turn, over ``final_steps`` steps (shapes within a few percent
     of each other are not told apart reliably by a short timing), and the fastest
-    of that second round is kept.
+    of that second timing pass is kept.
 
     Returns {"stress": (ty, nt, xchunk, remap, step_ms), "velocity": (...), "table": [...]}.
     """

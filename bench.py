@@ -164,7 +164,9 @@ def shape_sweep(g, k):
     """k stress and velocity launches (HIP-event timed) of the FAST stress pass
     at each shape of SHAPES_3D in turn, the velocity pass fixed: the run that
     rocprofv3 --pmc counts per dispatch (after 2 warm-up dispatches at the
-    first shape, dispatch 2 + i of the stress kernel has shape SHAPES_3D[i // k])."""
+    first shape, dispatch 2 + i of the stress kernel has shape SHAPES_3D[i // k]);
+    then the same for the velocity pass (its dispatch 2 + k * len(SHAPES_3D) + i
+    has shape SHAPES_3D[i // k])."""
     g.set_math("fast")
     g.set_tuning(0, *SHAPES_3D[0])
     g.set_tuning(1, *SHAPES_3D[0])
@@ -175,8 +177,16 @@ def shape_sweep(g, k):
         ms = g.profile(k)
         out.append({"shape": list(shape), "stress_ms": round(ms[0], 4)})
         print(f"# stress shape {shape}: {ms[0]:.4f} ms", file=sys.stderr, flush=True)
+    # then the velocity pass at each shape, the stress pass fixed at the first
+    g.set_tuning(0, *SHAPES_3D[0])
+    vel = []
+    for shape in SHAPES_3D:
+        g.set_tuning(1, *shape)
+        ms = g.profile(k)
+        vel.append({"shape": list(shape), "velocity_ms": round(ms[1], 4)})
+        print(f"# velocity shape {shape}: {ms[1]:.4f} ms", file=sys.stderr, flush=True)
     g.close()
-    emit({"shape_sweep": out, "dispatches_per_shape": k})
+    emit({"shape_sweep": out, "velocity_sweep": vel, "dispatches_per_shape": k})
 
 
 def main():
@@ -192,8 +202,8 @@ def main():
                     help="3D: pin the launch shapes instead of autotuning, "
                          "ty,nt,xchunk,remap/ty,nt,xchunk,remap (stress/velocity)")
     ap.add_argument("--shape-sweep", type=int, default=0,
-                    help="3D: only time k launches of the stress pass at each shape of SHAPES_3D "
-                         "(the run the traffic table is counted on)")
+                    help="3D: only time k launches of the stress pass, then of the velocity pass, "
+                         "at each shape of SHAPES_3D (the run the traffic table is counted on)")
     ap.add_argument("--n2d", type=int, default=2048, help="grid size (--workload 2d)")
     ap.add_argument("--math2d", choices=("strict", "fast"), default="strict",
                     help="arithmetic mode (--workload 2d): strict = the reference's operation order")

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define VSE_ABI_VERSION 1
+#define VSE_ABI_VERSION 2
 
 #define VSE_OK 0
 #define VSE_ERR_ARG (-1)        /* invalid argument / shape */
@@ -101,10 +101,16 @@ int vse_set_sources(vse_sim *s, int n_pts, const int *cells, const int *cols,
 /* Receivers: n_rec receivers; receiver r owns the point cells
  * cells[beg[r] .. beg[r+1]) (beg has n_rec+1 entries) and records from the
  * first step s with s + gate >= delay[r] (gate 0: WGSL, 1: Taichi).
+ * channel: the field (VSE_F_*) the receivers sum at the end of each step;
+ * -1 = the reference's own, sigma_yy in 2D and sigma_zz in 3D
+ * (shader_webgpu_viscoelastic.wgsl:1130, shader_webgpu_3D_viscoelastic.wgsl:1839).
+ * A velocity channel (2D VSE_F_VY is the ROI's vertical "vz") is recorded
+ * after the velocity update and its source term.  The gradient handle takes
+ * sigma_yy only.
  * Replaces b_delay_rec, b_info_rec_pt, b_offset_sensors and the trace
  * buffers (sim_webgpu_viscoelastic.py:174-191). */
 int vse_set_receivers(vse_sim *s, int n_rec, const int *beg, const int *cells,
-                      const int *delay, int gate);
+                      const int *delay, int gate, int channel);
 
 /* Zero every state array, the traces and the step counter. */
 int vse_reset(vse_sim *s);
@@ -199,6 +205,10 @@ int vse_copy_bandwidth(size_t bytes, int iters, float *gbs);
 #define VSE_STREAMS_MAX 32
 int vse_stream_bandwidth(size_t n, int n_read, int n_rw, int iters, float *gbs);
 
+/* Release the handle.  Batches and gradient handles built on s may be
+ * destroyed before or after it: destroying s first leaves them detached
+ * (a gradient handle frees its device memory at once), and every later call
+ * on them except their destroy returns VSE_ERR_STATE. */
 void vse_destroy(vse_sim *s);
 
 /* ---- batches: many 2D simulations per launch ---------------------------
@@ -212,15 +222,16 @@ void vse_destroy(vse_sim *s);
  * as hipGraphs of up to check_every steps in which every step is one stress
  * and one velocity launch over the tiles of all the group's members (the
  * velocity launch also injects their sources and records their receivers;
- * one launch per graph advances their step counters); a group of one member
- * runs vse_run on it.  Members must be
- * WGSL-variant simulations (Taichi-variant batches are switched off and
- * return VSE_ERR_ARG) sharing viscoelastic flag, FD order, number of
+ * one launch per graph advances their step counters; members recording a
+ * velocity channel are gathered by one more launch after each velocity
+ * pass); a group of one member runs vse_run on it.  Members must be 2D
+ * simulations sharing variant, viscoelastic flag, FD order, number of
  * mechanisms and device; they keep their own state,
  * so vse_read_* on a member reads its results, and every member's result is
- * identical to vse_run on it alone.  A batch does not own its members:
- * destroy it before them, and do not call a member while a batch call is in
- * flight on another thread.  No reference counterpart. */
+ * identical to vse_run on it alone.  A batch does not own its members; a
+ * batch one of whose members was destroyed returns VSE_ERR_STATE.  Do not
+ * call a member while a batch call is in flight on another thread.  No
+ * reference counterpart. */
 #define VSE_BATCH_MAX 256
 #define VSE_BATCH_GROUP_BYTES_DEFAULT ((size_t)224 << 20)
 typedef struct vse_batch vse_batch;

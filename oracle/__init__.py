@@ -48,8 +48,12 @@ class OracleParams(ctypes.Structure):
         ("n_rec_el", ctypes.c_int), ("n_rec_pt", ctypes.c_int),
         ("info_rec_pt", ctypes.c_void_p), ("offset_sensors", ctypes.c_void_p),
         ("delay_rec", ctypes.c_void_p), ("rec_map", ctypes.c_void_p),
-        ("nthreads", ctypes.c_int),
+        ("rec_channel", ctypes.c_int), ("nthreads", ctypes.c_int),
     ]
+
+
+# receiver channels, in the state's field order (vse_oracle.h ORC_CH_*)
+CHANNEL = {"vx": 0, "vy": 1, "vz": 2, "sxx": 3, "syy": 4, "szz": 5, "sxy": 6, "sxz": 7, "syz": 8}
 
 
 class OracleState(ctypes.Structure):
@@ -74,7 +78,7 @@ def lib():
         if not os.path.exists(_LIB_PATH):
             build()
         _lib = ctypes.CDLL(_LIB_PATH)
-        assert _lib.oracle_abi_version() == 1
+        assert _lib.oracle_abi_version() == 2, "stale oracle build: run oracle.build()"
         _lib.oracle_params_size.restype = ctypes.c_size_t
         assert _lib.oracle_params_size() == ctypes.sizeof(OracleParams), "OracleParams layout mismatch"
         for sfx in ("f32", "f64"):
@@ -95,7 +99,7 @@ class Oracle:
     reference's own choice per variant."""
 
     def __init__(self, sim, variant="wgsl", precision="f32", visco=None, att=None, n_threads=None,
-                 src_stride=None):
+                 src_stride=None, channel=None):
         self.ndim = 3 if hasattr(sim, "_rho_grid_vx") and sim._rho_grid_vx.ndim == 3 else 2
         self.dtype = np.float32 if precision == "f32" else np.float64
         self.sfx = precision
@@ -156,6 +160,11 @@ class Oracle:
         p.offset_sensors = _ptr(keep32(sim._offset_sensors, np.int32))
         p.delay_rec = _ptr(keep32(sim._delay_recv, np.int32))
         p.rec_map = _ptr(keep32(sim._pos_sensors, np.int32))
+        if channel is None:
+            channel = "syy" if self.ndim == 2 else "szz"
+        if self.ndim == 2 and channel not in ("vx", "vy", "sxx", "syy", "sxy"):
+            raise ValueError(f"2D receivers cannot record {channel}")
+        p.rec_channel = CHANNEL[channel]
         p.nthreads = int(n_threads or os.cpu_count() or 1)
         self.p = p
 

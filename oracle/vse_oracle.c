@@ -18,5 +18,5 @@
 #undef REAL
 #undef SFX
 
-int oracle_abi_version(void) { return 1; }
+int oracle_abi_version(void) { return 2; }
 size_t oracle_params_size(void) { return sizeof(OracleParams); }

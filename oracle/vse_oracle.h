@@ -63,7 +63,11 @@ typedef struct {
     const int *offset_sensors; /* (n_rec_el) */
     const int *delay_rec;   /* (n_rec_el) */
     const int *rec_map;     /* Taichi: full grid, -1 or receiver id */
+    int rec_channel;        /* ORC_CH_*: the field the receivers sum (the reference's: sigma_yy / sigma_zz) */
     int nthreads;
 } OracleParams;
+
+/* Receiver channels (field order of the state). */
+enum { ORC_CH_VX = 0, ORC_CH_VY, ORC_CH_VZ, ORC_CH_SXX, ORC_CH_SYY, ORC_CH_SZZ, ORC_CH_SXY, ORC_CH_SXZ, ORC_CH_SYZ };
 
 #endif

@@ -277,10 +277,23 @@ static void SFX(velocity2d)(const OracleParams *p, SFX(OracleState) *st, int it)
 }
 
 /* Receivers: WGSL table loop (wgsl2d :1117-1134, wgsl3d :1823-1843). */
+/* The field receivers sum: the reference's sigma_yy (2D) / sigma_zz (3D)
+ * (wgsl2d :1130, wgsl3d :1839), or the channel a test asks for. */
+static const REAL *SFX(rec_field)(const OracleParams *p, const SFX(OracleState) *st)
+{
+    REAL *const f[9] = {st->vx, st->vy, st->vz, st->sxx, st->syy, st->szz, st->sxy, st->sxz, st->syz};
+    return f[p->rec_channel];
+}
+
+static int SFX(velocity_channel)(const OracleParams *p)
+{
+    return p->rec_channel == ORC_CH_VX || p->rec_channel == ORC_CH_VY || p->rec_channel == ORC_CH_VZ;
+}
+
 static void SFX(sensors_wgsl)(const OracleParams *p, SFX(OracleState) *st, int it)
 {
     const int cols = p->ndim + 1;
-    const REAL *f = p->ndim == 2 ? st->syy : st->szz;
+    const REAL *f = SFX(rec_field)(p, st);
     for (int s = 0; s < p->n_rec_el; s++) {
         int pt = p->offset_sensors[s];
         for (;;) {
@@ -307,12 +320,14 @@ static void SFX(sensors_wgsl)(const OracleParams *p, SFX(OracleState) *st, int i
 static void SFX(sensors_taichi)(const OracleParams *p, SFX(OracleState) *st, int it)
 {
     const int nx = p->nx, ny = p->ny, last = p->ord - 1;
+    const REAL *f = SFX(rec_field)(p, st);
+    if (it < 1 || it > p->n_iter) return;
     for (int x = last; x < nx - last - 1; x++)
         for (int y = last + 1; y < ny - last; y++) {
             long id = (long)y + (long)x * ny;
             int s = p->rec_map[id];
-            if (s != -1 && it >= p->delay_rec[s])
-                st->traces[(long)(it - 1) * p->n_rec_el + s] += st->syy[id];
+            if (s >= 0 && s < p->n_rec_el && it >= p->delay_rec[s])
+                st->traces[(long)(it - 1) * p->n_rec_el + s] += f[id];
         }
 }
 
@@ -517,8 +532,11 @@ int SFX(oracle_run)(const OracleParams *p, SFX(OracleState) *st, int s0, int n)
         if (p->ndim == 2) {
             SFX(stress2d)(p, st);
             if (p->variant == ORC_VARIANT_TAICHI) {
-                SFX(sensors_taichi)(p, st, s + 1);
+                /* the reference gathers sigma_yy inside the stress pass; a
+                 * velocity channel is read after the velocity pass */
+                if (!SFX(velocity_channel)(p)) SFX(sensors_taichi)(p, st, s + 1);
                 SFX(velocity2d)(p, st, s + 1);
+                if (SFX(velocity_channel)(p)) SFX(sensors_taichi)(p, st, s + 1);
             } else {
                 SFX(velocity2d)(p, st, s);
                 SFX(sensors_wgsl)(p, st, s);

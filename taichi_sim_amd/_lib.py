@@ -8,11 +8,10 @@ dynamic linker resolves those names to whichever copy the process loaded
 first, and the torch wheel ships its own copies (an older ROCm release) under
 the same names.  Load this library before importing torch to run it on the
 ROCm runtime it was compiled with; ``runtime_path()`` names the copy in use
-and ``load()`` warns when it lies outside that ROCm installation.
+and ``load()`` raises when it lies outside that ROCm installation.
 """
 import ctypes
 import os
-import warnings
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvse.so")
@@ -61,6 +60,7 @@ EXPORTS = ["vse_abi_version", "vse_create", "vse_set_material", "vse_set_cpml", 
            "vse_grad_backward", "vse_grad_read", "vse_grad_read_state", "vse_grad_segment", "vse_grad_device_bytes",
            "vse_grad_last_error", "vse_grad_destroy", "vse_batch_last_error", "vse_batch_destroy"]
 BATCH_MAX = 256
+ABI_VERSION = 2
 
 
 class VseDesc(ctypes.Structure):
@@ -98,7 +98,7 @@ def load():
         "vse_set_material": (I, [P, fp, fp, fp]),
         "vse_set_cpml": (I, [P, I, fp, fp, fp, fp, fp, fp]),
         "vse_set_sources": (I, [P, I, ip, ip, I, fp]),
-        "vse_set_receivers": (I, [P, I, ip, ip, ip, I]),
+        "vse_set_receivers": (I, [P, I, ip, ip, ip, I, I]),
         "vse_reset": (I, [P]),
         "vse_run": (I, [P, I, I, fp]),
         "vse_profile": (I, [P, I, fp]),
@@ -141,12 +141,23 @@ def load():
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
-    if lib.vse_abi_version() != 1:
-        raise ImportError("libvse ABI version mismatch")
+    if lib.vse_abi_version() != ABI_VERSION:
+        raise ImportError(f"libvse ABI version {lib.vse_abi_version()}, this binding expects {ABI_VERSION}: rebuild it")
     rt = runtime_path(lib)
     if rt is not None and not rt.startswith(ROCM_PATH + os.sep):
-        warnings.warn(f"libvse is bound to the HIP runtime {rt}, loaded earlier by another package, not the one "
-                      f"under {ROCM_PATH} it was built with; import taichi_sim_amd before torch to use the latter",
-                      RuntimeWarning, stacklevel=2)
+        raise ImportError(f"libvse is bound to the HIP runtime {rt}, loaded earlier by another package, not the one "
+                          f"under {ROCM_PATH} it was built with; import taichi_sim_amd (or call "
+                          f"taichi_sim_amd._lib.load()) before that package, e.g. before torch")
     _lib = lib
     return lib
+
+
+def mapped_runtimes():
+    """Files of every HIP and HSA runtime mapped into this process (Linux).
+    A second copy (another package's) stays idle as long as only libvse's is called."""
+    try:
+        with open("/proc/self/maps") as f:
+            names = {ln.split()[-1] for ln in f if "libamdhip64" in ln or "libhsa-runtime64" in ln}
+    except OSError:
+        return []
+    return sorted(n for n in names if n.startswith("/"))

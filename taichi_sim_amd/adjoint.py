@@ -77,6 +77,7 @@ class VSEGrad:
             raise _lib.VSEError(rc, sim._lib.vse_last_error(sim._h).decode(errors="replace"))
         self._h = h
         self.n_steps = 0
+        self.forward_count = 0          # forwards run so far (ViscoTraces checks it in backward)
         self.ms = {}
         sim._dependents.add(self)       # closing the simulation closes this engine first
 
@@ -110,6 +111,7 @@ class VSEGrad:
         n = int(n_steps or self.sim.n_rows)
         out = np.zeros((n, max(self.sim.n_rec, 1)), dtype=np.float32)
         ms = ctypes.c_float(0.0)
+        self.forward_count += 1
         self._check(self._lib.vse_grad_forward(self._h, n, int(check_every),
                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(ms)))
         self.n_steps = n
@@ -182,26 +184,39 @@ def chain_to_cells(coef_grads, rho, cp, cs):
 class ViscoTraces(torch.autograd.Function):
     """traces (n_steps, n_rec) = forward(rho, cp, cs, tscale) through libvse;
     backward runs the adjoint.  Inputs are (nx, ny) tensors (any float dtype,
-    CPU); the wavefields are FP32 on the GPU, the traces are returned as FP64."""
+    CPU); tscale may be None (every relaxation time at scale 1).  The
+    wavefields are FP32 on the GPU, the traces are returned as FP64.
+
+    The engine keeps the checkpoints of its latest forward only, so a node's
+    backward must come before the engine's next forward; backward raises
+    otherwise instead of differentiating the other model."""
 
     @staticmethod
     def forward(ctx, eng, rho, cp, cs, tscale):
         eng.set_model(rho, cp, cs, tscale)
         tr = eng.forward()
         ctx.eng = eng
+        ctx.forward_id = eng.forward_count
         ctx.save_for_backward(rho, cp, cs)
         return torch.from_numpy(tr.astype(np.float64))
 
     @staticmethod
     def backward(ctx, g_tr):
         eng = ctx.eng
+        if eng.forward_count != ctx.forward_id:
+            raise RuntimeError("ViscoTraces: the engine ran another forward since this node's; its checkpoints "
+                               "are gone -- call backward before the next apply() on the same engine")
         rho, cp, cs = ctx.saved_tensors
         grads = eng.backward(g_tr.detach().cpu().numpy())
-        with torch.enable_grad():
-            gr, gc, gs = chain_to_cells(grads, rho, cp, cs)
-        gt = torch.as_tensor(grads["tau"], dtype=torch.float64)
-        cast = lambda g, like: g.to(like.dtype) if torch.is_tensor(like) else g
-        return None, cast(gr, rho), cast(gc, cp), cast(gs, cs), gt
+        need = ctx.needs_input_grad
+        gr = gc = gs = gt = None
+        if any(need[1:4]):
+            with torch.enable_grad():
+                gr, gc, gs = chain_to_cells(grads, rho, cp, cs)
+            gr, gc, gs = (g.to(like.dtype) if n else None for g, like, n in zip((gr, gc, gs), (rho, cp, cs), need[1:4]))
+        if need[4]:
+            gt = torch.as_tensor(grads["tau"], dtype=torch.float64)
+        return None, gr, gc, gs, gt
 
 
 def velocities(rho, lam, mu):

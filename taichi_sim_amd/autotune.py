@@ -22,7 +22,11 @@ REMAP = (0, 1, 2)
 
 
 def problem_key(g):
-    return f"{g.ndim}d:{'x'.join(map(str, g.shape))}:ord{g.ord}:visco{int(g.visco)}:L{g.n_sls}"
+    """Cache key: the device (name, architecture, CU count) and arithmetic mode
+    as well as the problem, so a shape tuned on one part is not reused on another."""
+    dev = g.device_name() if hasattr(g, "device_name") else "any"
+    math = getattr(g, "math", "strict")
+    return f"{dev}|{g.ndim}d:{'x'.join(map(str, g.shape))}:ord{g.ord}:visco{int(g.visco)}:L{g.n_sls}:{math}"
 
 
 def _load(path):

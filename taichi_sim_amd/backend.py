@@ -64,8 +64,8 @@ class HIPBackend:
         simulation per law, advanced as one batch (one launch per pass per
         step for all of them).  Each law's result equals implementation()'s;
         "sim_time" is the batch's device time divided by the number of laws."""
-        if len(laws) < 2 or np.asarray(self._rho_grid_vx).ndim != 2 or self._variant != "wgsl":
-            return super().implementation_many(laws)       # Taichi-variant batches are switched off
+        if len(laws) < 2 or np.asarray(self._rho_grid_vx).ndim != 2:
+            return super().implementation_many(laws)       # batches are 2D
         att = attenuation_from_config(self._configs)
         sims = []
         try:

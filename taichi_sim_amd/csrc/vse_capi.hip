@@ -56,6 +56,7 @@ struct vse_sim {
   float *src = nullptr;
   // receivers
   int n_rec = 0, rec_gate = 0;
+  int rec_channel = VSE_F_SYY;    // field the receivers sample (VSE_F_*)
   int *rec_beg = nullptr, *rec_cell = nullptr, *rec_delay = nullptr;
   float *traces = nullptr;
   // launch shapes of the 3D stress / velocity kernels
@@ -63,6 +64,10 @@ struct vse_sim {
   // cached graph
   hipGraphExec_t gexec = nullptr;
   int gexec_steps = 0;
+  // handles built on this simulation; vse_destroy detaches them, so any
+  // destroy order is safe (a detached handle reports VSE_ERR_STATE)
+  std::vector<vse_batch *> batches;
+  std::vector<vse_grad *> grads;
 
   int fail(int code, const std::string &m) { err = m; return code; }
   int hip(hipError_t e, const char *what) {
@@ -110,6 +115,36 @@ struct vse_sim {
 
 static int check_handle(vse_sim *s) { return s ? 0 : VSE_ERR_ARG; }
 
+// vse_destroy(s) tells the handles built on s that it is gone (defined below
+// with their types): a batch forgets the member, a gradient handle releases
+// its device memory and reports VSE_ERR_STATE from then on.
+static void batch_forget_member(vse_batch *b, vse_sim *s);
+static void grad_detach(vse_grad *g);
+
+// Waits for the stream when it goes out of scope.  A setter that queues
+// host-to-device copies from its own local buffers declares one first, so that
+// every return -- an error return included -- leaves no copy reading freed memory.
+class SyncOnExit {
+ public:
+  explicit SyncOnExit(hipStream_t st) : st_(st) {}
+  ~SyncOnExit() {
+    if (st_) (void)hipStreamSynchronize(st_);
+  }
+  SyncOnExit(const SyncOnExit &) = delete;
+  SyncOnExit &operator=(const SyncOnExit &) = delete;
+
+ private:
+  hipStream_t st_;
+};
+
+// Field ids a simulation holds (2D: vx, vy, sxx, syy, sxy; 3D: all nine).
+static bool field_present(const vse_desc &d, int f) {
+  if (f < 0 || f > 8) return false;
+  return d.ndim == 3 || f == VSE_F_VX || f == VSE_F_VY || f == VSE_F_SXX || f == VSE_F_SYY || f == VSE_F_SXY;
+}
+
+static bool velocity_field(int f) { return f == VSE_F_VX || f == VSE_F_VY || f == VSE_F_VZ; }
+
 // Makes a handle's device current for one C-ABI call and restores the
 // caller's device on return, so a caller that switched devices between calls
 // (torch.cuda.set_device, another library) still reaches the handle's GPU.
@@ -145,12 +180,18 @@ int vse_create(const vse_desc *desc, vse_sim **out) {
   if (d.nx < minn || d.ny < minn || (d.ndim == 3 && d.nz < minn)) return VSE_ERR_ARG;
   if (d.n_rows < 1 || !(d.dt > 0.f) || !(d.dx > 0.f) || !(d.dy > 0.f) || (d.ndim == 3 && !(d.dz > 0.f)))
     return VSE_ERR_ARG;
+  // 32-bit buffer addressing: the 2D velocity kernels index whole fields
+  // (byte offsets and ranges below 2^32), the 3D kernels index x-planes
+  if (d.ndim == 2 && (long)d.nx * d.ny >= (1L << 30)) return VSE_ERR_ARG;
+  if (d.ndim == 3 && (long)d.ny * d.nz >= (1L << 30)) return VSE_ERR_ARG;
 
   vse_sim *s = new vse_sim();
   s->d = d;
   if (d.ndim == 2) {
     s->d.nz = 1;
     s->tune[0] = s->tune[1] = KTune{4, 0, 0, 1};   // XCD-contiguous tile order
+  } else {
+    s->rec_channel = VSE_F_SZZ;
   }
   s->ncell = (long)s->d.nx * s->d.ny * s->d.nz;
   int rc = s->hip(hipGetDevice(&s->device), "hipGetDevice");
@@ -210,6 +251,7 @@ int vse_create(const vse_desc *desc, vse_sim **out) {
 int vse_set_material(vse_sim *s, const float *rho, const float *cp, const float *cs) {
   if (check_handle(s) || !rho || !cp || !cs) return VSE_ERR_ARG;
   OnDevice on_dev(s->device);
+  SyncOnExit sync(s->stream);
   int rc = 0;
   if (!s->rho) {
     rc = s->alloc(&s->rho, s->ncell);
@@ -249,6 +291,7 @@ int vse_set_cpml(vse_sim *s, int axis, const float *a, const float *b, const flo
   }
   int rc = 0;
   std::vector<float> inv_k(np), inv_kh(np);
+  SyncOnExit sync(s->stream);     // declared after the vectors: waits before they are freed
   for (int j = 0; j < np; j++) {
     inv_k[j] = 1.0f / k[j];
     inv_kh[j] = 1.0f / k_h[j];
@@ -284,17 +327,22 @@ int vse_set_sources(vse_sim *s, int n_pts, const int *cells, const int *cols, in
     if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
       return s->fail(VSE_ERR_ARG, "source cell listed twice");
   }
+  std::vector<int> map, rows;
+  SyncOnExit sync(s->stream);     // after the vectors: waits before they are freed
+  // the cached graph holds the old buffers' addresses: drop it before they go
+  s->drop_graph();
+  (void)hipStreamSynchronize(s->stream);
   s->release(s->src_cell); s->release(s->src_col); s->release(s->src); s->release(s->src_map);
   s->release(s->src_rowflag);
   s->src_cell = s->src_col = s->src_map = s->src_rowflag = nullptr;
   s->src = nullptr;
+  s->n_src_pts = 0;
   int rc = s->alloc(&s->src_cell, n_pts);
   if (!rc) rc = s->alloc(&s->src_col, n_pts);
   if (!rc) rc = s->alloc(&s->src, (size_t)s->d.n_rows * n_cols);
   if (!rc && n_pts) rc = s->hip(hipMemcpyAsync(s->src_cell, cells, n_pts * sizeof(int), hipMemcpyHostToDevice, s->stream), "src cells");
   if (!rc && n_pts) rc = s->hip(hipMemcpyAsync(s->src_col, cols, n_pts * sizeof(int), hipMemcpyHostToDevice, s->stream), "src cols");
   if (!rc) rc = s->hip(hipMemcpyAsync(s->src, src, (size_t)s->d.n_rows * n_cols * sizeof(float), hipMemcpyHostToDevice, s->stream), "src");
-  std::vector<int> map, rows;
   if (!rc && s->d.ndim == 2) {     // 2D velocity kernels inject through a per-cell map
     map.assign(s->ncell, -1);
     rows.assign(s->d.nx, 0);
@@ -315,9 +363,12 @@ int vse_set_sources(vse_sim *s, int n_pts, const int *cells, const int *cols, in
   return VSE_OK;
 }
 
-int vse_set_receivers(vse_sim *s, int n_rec, const int *beg, const int *cells, const int *delay, int gate) {
+int vse_set_receivers(vse_sim *s, int n_rec, const int *beg, const int *cells, const int *delay, int gate,
+                      int channel) {
   if (check_handle(s) || n_rec < 0 || (n_rec > 0 && (!beg || !delay))) return VSE_ERR_ARG;
   OnDevice on_dev(s->device);
+  if (channel < 0) channel = s->d.ndim == 2 ? VSE_F_SYY : VSE_F_SZZ;
+  if (!field_present(s->d, channel)) return s->fail(VSE_ERR_ARG, "receiver channel is not a field of this simulation");
   const int n_pts = n_rec > 0 ? beg[n_rec] : 0;
   if (n_rec > 0 && beg[0] != 0) return s->fail(VSE_ERR_ARG, "receiver CSR must start at 0");
   for (int r = 0; r < n_rec; r++)
@@ -325,9 +376,14 @@ int vse_set_receivers(vse_sim *s, int n_rec, const int *beg, const int *cells, c
   if (n_pts > 0 && !cells) return VSE_ERR_ARG;
   for (int p = 0; p < n_pts; p++)
     if (cells[p] < 0 || cells[p] >= s->ncell) return s->fail(VSE_ERR_ARG, "receiver point out of range");
+  SyncOnExit sync(s->stream);
+  // the cached graph holds the old buffers' addresses: drop it before they go
+  s->drop_graph();
+  (void)hipStreamSynchronize(s->stream);
   s->release(s->rec_beg); s->release(s->rec_cell); s->release(s->rec_delay); s->release(s->traces);
   s->rec_beg = s->rec_cell = s->rec_delay = nullptr;
   s->traces = nullptr;
+  s->n_rec = 0;
   int rc = s->alloc(&s->rec_beg, n_rec + 1);
   if (!rc) rc = s->alloc(&s->rec_cell, n_pts);
   if (!rc) rc = s->alloc(&s->rec_delay, n_rec);
@@ -339,7 +395,7 @@ int vse_set_receivers(vse_sim *s, int n_rec, const int *beg, const int *cells, c
   if (rc) return rc;
   s->n_rec = n_rec;
   s->rec_gate = gate;
-  s->drop_graph();
+  s->rec_channel = channel;
   return VSE_OK;
 }
 
@@ -420,11 +476,13 @@ static void make_args(vse_sim *s, StepArgs &A, SparseArgs &S) {
   A.src_rowflag = s->src_rowflag;
   A.n_rec = s->n_rec; A.rec_gate = s->rec_gate; A.trace_rows = d.n_rows;
   A.rec_beg = s->rec_beg; A.rec_cell = s->rec_cell; A.rec_delay = s->rec_delay; A.traces = s->traces;
+  A.rec_field = s->field[s->rec_channel];
+  A.rec_after = velocity_field(s->rec_channel) ? 1 : 0;
 
   memset(&S, 0, sizeof(S));
   S.ndim = d.ndim; S.nx = d.nx; S.ny = d.ny; S.nz = d.nz; S.dt = d.dt;
   S.vx = A.vx; S.vy = A.vy; S.vz = A.vz;
-  S.rec_field = d.ndim == 2 ? A.syy : A.szz;
+  S.rec_field = s->field[s->rec_channel];
   S.rho = s->rho;
   S.v2max = s->ctrl;
   S.step = (int *)(s->ctrl + 1);
@@ -444,7 +502,9 @@ static hipError_t enqueue_step(vse_sim *s, const StepArgs &A, const SparseArgs &
   if (d.ndim == 2) {
     e = vse_launch_stress2d(d.variant, d.visco, d.ord, s->math, A, s->tune[0], s->stream);
     if (e != hipSuccess) return e;
-    return vse_launch_velocity2d(d.variant, d.visco, d.ord, s->math, A, s->tune[1], i, s->stream);
+    e = vse_launch_velocity2d(d.variant, d.visco, d.ord, s->math, A, s->tune[1], i, s->stream);
+    if (e == hipSuccess && A.rec_after) e = vse_launch_record2d(A, i, s->stream);
+    return e;
   } else {
     e = vse_launch_stress3d(d.visco, d.ord, s->math, A, s->tune[0], s->stream);
     if (e != hipSuccess) return e;
@@ -564,6 +624,7 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
     if (e == hipSuccess) e = d.ndim == 2 ? vse_launch_velocity2d(d.variant, d.visco, d.ord, s->math, A, s->tune[1], 0, s->stream)
                                          : vse_launch_velocity3d(d.visco, d.ord, s->math, A, s->tune[1], s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
+    if (e == hipSuccess && d.ndim == 2 && A.rec_after) e = vse_launch_record2d(A, 0, s->stream);
     if (e == hipSuccess) e = d.ndim == 2 ? advance_steps(s, 1) : vse_launch_sparse(S, s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[3], s->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
@@ -666,6 +727,11 @@ void vse_destroy(vse_sim *s) {
   if (!s) return;
   OnDevice on_dev(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  // detach the batches and gradient handles built on s (they may be destroyed later)
+  for (vse_batch *b : s->batches) batch_forget_member(b, s);
+  s->batches.clear();
+  for (vse_grad *g : s->grads) grad_detach(g);
+  s->grads.clear();
   s->drop_graph();
   for (void *p : s->allocs) (void)hipFree(p);
   s->allocs.clear();
@@ -689,6 +755,7 @@ struct BatchGroup {
   int first = 0, n = 0;
   BatchArgs B{};
   int total_tiles = 0;
+  bool rec_after = false;     // a member records a velocity channel: one gather launch after each velocity pass
   hipGraphExec_t gexec = nullptr;
   int gexec_steps = 0;
 };
@@ -706,6 +773,7 @@ struct vse_batch {
   StepArgs *d_args = nullptr;
   SparseArgs *d_sparse = nullptr;
   unsigned *d_ctrl = nullptr, *h_ctrl = nullptr;   // (v2max bits, step) per member
+  bool orphan = false;                              // a member was destroyed: the batch can only be destroyed
   std::string err;
 
   int fail(int code, const std::string &m) { err = m; return code; }
@@ -718,7 +786,30 @@ struct vse_batch {
     for (auto &g : groups)
       if (g.gexec) { (void)hipGraphExecDestroy(g.gexec); g.gexec = nullptr; g.gexec_steps = 0; }
   }
+  int usable() { return orphan ? fail(VSE_ERR_STATE, "a member of this batch was destroyed") : 0; }
 };
+
+static void batch_forget_member(vse_batch *b, vse_sim *s) {
+  for (auto &m : b->sims)
+    if (m == s) m = nullptr;
+  b->orphan = true;
+}
+
+// Whether each group (and the all-member group) needs the velocity-channel
+// gather after its velocity passes; a group whose need changed recaptures its graph.
+static void refresh_rec_flags(vse_batch *b) {
+  auto need = [&](const BatchGroup &g) {
+    for (int i = 0; i < g.n; i++)
+      if (velocity_field(b->sims[g.first + i]->rec_channel)) return true;
+    return false;
+  };
+  for (auto &g : b->groups) {
+    const bool r = need(g);
+    if (r != g.rec_after && g.gexec) { (void)hipGraphExecDestroy(g.gexec); g.gexec = nullptr; g.gexec_steps = 0; }
+    g.rec_after = r;
+  }
+  b->all.rec_after = need(b->all);
+}
 
 static int build_group(vse_batch *b, BatchGroup &g, int first, int n) {
   g.first = first;
@@ -763,6 +854,7 @@ static int regroup(vse_batch *b) {
     b->groups.emplace_back();
     rc = build_group(b, b->groups.back(), first, n - first);
   }
+  if (!rc) refresh_rec_flags(b);
   return rc;
 }
 
@@ -773,6 +865,7 @@ static hipError_t enqueue_group_step(vse_batch *b, const BatchGroup &g, int i) {
   hipError_t e = vse_launch_batch2d(true, b->variant, b->visco, b->ord, b->math, g.B, g.total_tiles, 0, b->stream);
   if (e == hipSuccess)
     e = vse_launch_batch2d(false, b->variant, b->visco, b->ord, b->math, g.B, g.total_tiles, i, b->stream);
+  if (e == hipSuccess && g.rec_after) e = vse_launch_record2d_batch(g.B, i, b->stream);
   return e;
 }
 
@@ -797,9 +890,11 @@ static int upload_batch_args(vse_batch *b) {
     if (rc) return b->fail(rc, "member " + std::to_string(i) + ": " + s->err);
     make_args(s, A[i], S[i]);
   }
+  SyncOnExit sync(b->stream);     // after the vectors: waits before they are freed
   int rc = b->hip(hipMemcpyAsync(b->d_args, A.data(), n * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream), "upload args");
   if (!rc) rc = b->hip(hipMemcpyAsync(b->d_sparse, S.data(), n * sizeof(SparseArgs), hipMemcpyHostToDevice, b->stream), "upload args");
   if (!rc) rc = b->hip(hipStreamSynchronize(b->stream), "upload args");
+  if (!rc) refresh_rec_flags(b);
   return rc;
 }
 
@@ -888,6 +983,8 @@ void vse_batch_destroy(vse_batch *b) {
   if (!b) return;
   OnDevice on_dev(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (vse_sim *s : b->sims)
+    if (s) s->batches.erase(std::remove(s->batches.begin(), s->batches.end(), b), s->batches.end());
   b->drop_graphs();
   if (b->d_args) (void)hipFree(b->d_args);
   if (b->d_sparse) (void)hipFree(b->d_sparse);
@@ -904,9 +1001,6 @@ int vse_batch_create(vse_sim *const *sims, int n, vse_batch **out) {
   vse_sim *s0 = sims[0];
   if (!s0) return VSE_ERR_ARG;
   OnDevice on_dev(s0->device);
-  // Taichi-variant batches are switched off: a process ended in SIGABRT while
-  // tearing down the solo runs that followed one, and no cause has been shown.
-  if (s0->d.variant != VSE_VARIANT_WGSL) return s0->fail(VSE_ERR_ARG, "batches take WGSL-variant simulations only");
   for (int i = 0; i < n; i++) {
     vse_sim *s = sims[i];
     if (!s || s->d.ndim != 2 || s->d.variant != s0->d.variant || s->d.visco != s0->d.visco ||
@@ -917,6 +1011,7 @@ int vse_batch_create(vse_sim *const *sims, int n, vse_batch **out) {
   }
   vse_batch *b = new vse_batch();
   b->sims.assign(sims, sims + n);
+  for (vse_sim *s : b->sims) s->batches.push_back(b);   // vse_batch_destroy (also on failure below) unregisters
   b->device = s0->device;
   b->variant = s0->d.variant;
   b->visco = s0->d.visco;
@@ -940,7 +1035,8 @@ int vse_batch_create(vse_sim *const *sims, int n, vse_batch **out) {
 int vse_batch_run(vse_batch *b, int n_steps, int check_every, float *elapsed_ms) {
   if (!b || n_steps < 0) return VSE_ERR_ARG;
   OnDevice on_dev(b->device);
-  int rc = upload_batch_args(b);
+  int rc = b->usable();
+  if (!rc) rc = upload_batch_args(b);
   if (rc) return rc;
   double total = 0.0;
   for (size_t k = 0; k < b->groups.size() && !rc && n_steps > 0; k++) {
@@ -966,7 +1062,8 @@ int vse_batch_run(vse_batch *b, int n_steps, int check_every, float *elapsed_ms)
 int vse_batch_profile(vse_batch *b, int n_steps, float *ms) {
   if (!b || n_steps < 1 || !ms) return VSE_ERR_ARG;
   OnDevice on_dev(b->device);
-  int rc = upload_batch_args(b);
+  int rc = b->usable();
+  if (!rc) rc = upload_batch_args(b);
   if (rc) return rc;
   const BatchGroup &g = b->all;
   hipEvent_t ev[4] = {};
@@ -980,6 +1077,7 @@ int vse_batch_profile(vse_batch *b, int n_steps, float *ms) {
     if (e == hipSuccess)
       e = vse_launch_batch2d(false, b->variant, b->visco, b->ord, b->math, g.B, g.total_tiles, 0, b->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[2], b->stream);
+    if (e == hipSuccess && g.rec_after) e = vse_launch_record2d_batch(g.B, 0, b->stream);
     if (e == hipSuccess) e = vse_launch_advance_batch(g.B, 1, b->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[3], b->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
@@ -1001,6 +1099,7 @@ int vse_batch_profile(vse_batch *b, int n_steps, float *ms) {
 int vse_batch_set_rows(vse_batch *b, int rows) {
   if (!b || rows < 1 || rows > 64) return VSE_ERR_ARG;
   OnDevice on_dev(b->device);
+  if (int rc = b->usable()) return rc;
   b->rows = rows;
   return regroup(b);
 }
@@ -1008,6 +1107,7 @@ int vse_batch_set_rows(vse_batch *b, int rows) {
 int vse_batch_set_group_bytes(vse_batch *b, size_t bytes) {
   if (!b) return VSE_ERR_ARG;
   OnDevice on_dev(b->device);
+  if (int rc = b->usable()) return rc;
   b->group_bytes = bytes;
   return regroup(b);
 }
@@ -1037,6 +1137,7 @@ const char *vse_batch_last_error(vse_batch *b) { return b ? b->err.c_str() : "nu
 struct vse_grad {
   vse_sim *s = nullptr;
   int K = 0, n_ckpt = 0, n_fwd = 0;
+  int n_rec = 0;                  // receivers of s when the trace buffers were sized
   bool last_seg_stored = false;   // the forward left the last segment's levels in seg[]
   std::vector<DLevel> ckpt, seg;
   DLevel bar{};
@@ -1152,12 +1253,36 @@ static int grad_src_map(vse_grad *g) {
 
 extern "C" {
 
+// Releases g's device memory (on its simulation's device and stream) and
+// leaves it without a simulation: every later call but vse_grad_destroy
+// returns VSE_ERR_STATE.
+static void grad_detach(vse_grad *g) {
+  vse_sim *s = g->s;
+  if (!s) return;
+  OnDevice on_dev(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (void *p : g->allocs) (void)hipFree(p);
+  g->allocs.clear();
+  if (g->host_ctrl) (void)hipHostFree(g->host_ctrl);
+  g->host_ctrl = nullptr;
+  g->s = nullptr;
+}
+
+// 0 if g can run, else VSE_ERR_ARG (null handle) or VSE_ERR_STATE (its
+// simulation was destroyed, or its receivers changed after the trace buffers
+// were sized)
+static int grad_usable(vse_grad *g) {
+  if (!g) return VSE_ERR_ARG;
+  if (!g->s) return g->fail(VSE_ERR_STATE, "the simulation of this gradient handle was destroyed");
+  if (g->s->n_rec != g->n_rec || g->s->rec_channel != VSE_F_SYY)
+    return g->fail(VSE_ERR_STATE, "the simulation's receivers changed after vse_grad_create");
+  return 0;
+}
+
 void vse_grad_destroy(vse_grad *g) {
   if (!g) return;
-  OnDevice on_dev(g->s->device);
-  if (g->s && g->s->stream) (void)hipStreamSynchronize(g->s->stream);
-  for (void *p : g->allocs) (void)hipFree(p);
-  if (g->host_ctrl) (void)hipHostFree(g->host_ctrl);
+  if (vse_sim *s = g->s) s->grads.erase(std::remove(s->grads.begin(), s->grads.end(), g), s->grads.end());
+  grad_detach(g);
   delete g;
 }
 
@@ -1167,10 +1292,13 @@ int vse_grad_create(vse_sim *s, int seg_len, vse_grad **out) {
   *out = nullptr;
   if (s->d.ndim != 2 || s->d.variant != VSE_VARIANT_WGSL || !s->d.visco)
     return s->fail(VSE_ERR_ARG, "gradient: 2D viscoelastic WGSL-variant simulations only");
+  if (s->rec_channel != VSE_F_SYY)
+    return s->fail(VSE_ERR_ARG, "gradient: receivers must record sigma_yy (the adjoint injects its residual there)");
   int rc = finalize(s);
   if (rc) return rc;
   vse_grad *g = new vse_grad();
   g->s = s;
+  s->grads.push_back(g);      // vse_grad_destroy (also on failure below) unregisters
   const int N = s->d.n_rows;
   g->K = seg_len ? seg_len : std::max(2, (int)std::ceil(std::sqrt((double)N)));
   g->n_ckpt = (N + g->K - 1) / g->K;
@@ -1186,6 +1314,7 @@ int vse_grad_create(vse_sim *s, int seg_len, vse_grad **out) {
   for (int i = 0; i < VSE_GR_COUNT && !rc; i++) rc = g->alloc(&g->gf[i], s->ncell);
   if (!rc) rc = g->alloc(&g->ts, s->ncell);
   if (!rc) rc = g->alloc(&g->src_map, s->ncell);
+  g->n_rec = s->n_rec;
   const size_t nt = (size_t)N * std::max(s->n_rec, 1);
   if (!rc) rc = g->alloc(&g->traces, nt);
   if (!rc) rc = g->alloc(&g->gbar, nt);
@@ -1208,7 +1337,7 @@ int vse_grad_create(vse_sim *s, int seg_len, vse_grad **out) {
 }
 
 int vse_grad_set_tau_scale(vse_grad *g, const float *scale) {
-  if (!g) return VSE_ERR_ARG;
+  if (int rc = grad_usable(g)) return rc;
   OnDevice on_dev(g->s->device);
   vse_sim *s = g->s;
   std::vector<float> ones;
@@ -1228,7 +1357,8 @@ int vse_grad_set_tau_scale(vse_grad *g, const float *scale) {
 }
 
 int vse_grad_forward(vse_grad *g, int n_steps, int check_every, float *traces, float *elapsed_ms) {
-  if (!g || n_steps < 1 || n_steps > g->s->d.n_rows) return VSE_ERR_ARG;
+  if (int rc = grad_usable(g)) return rc;
+  if (n_steps < 1 || n_steps > g->s->d.n_rows) return VSE_ERR_ARG;
   OnDevice on_dev(g->s->device);
   vse_sim *s = g->s;
   int rc = finalize(s);
@@ -1282,7 +1412,8 @@ int vse_grad_forward(vse_grad *g, int n_steps, int check_every, float *traces, f
 }
 
 int vse_grad_misfit(vse_grad *g, const float *observed, double *misfit) {
-  if (!g || !observed || !misfit) return VSE_ERR_ARG;
+  if (int rc = grad_usable(g)) return rc;
+  if (!observed || !misfit) return VSE_ERR_ARG;
   OnDevice on_dev(g->s->device);
   if (g->n_fwd < 1) return g->fail(VSE_ERR_STATE, "vse_grad_forward not run");
   vse_sim *s = g->s;
@@ -1296,7 +1427,7 @@ int vse_grad_misfit(vse_grad *g, const float *observed, double *misfit) {
 }
 
 int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms) {
-  if (!g) return VSE_ERR_ARG;
+  if (int rc = grad_usable(g)) return rc;
   OnDevice on_dev(g->s->device);
   const int N = g->n_fwd;
   if (N < 1) return g->fail(VSE_ERR_STATE, "vse_grad_forward not run");
@@ -1349,7 +1480,8 @@ int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms) 
 }
 
 int vse_grad_read(vse_grad *g, int which, double *out) {
-  if (!g || which < 0 || which >= VSE_GR_COUNT || !out) return VSE_ERR_ARG;
+  if (int rc = grad_usable(g)) return rc;
+  if (which < 0 || which >= VSE_GR_COUNT || !out) return VSE_ERR_ARG;
   OnDevice on_dev(g->s->device);
   vse_sim *s = g->s;
   int rc = g->hip(hipMemcpyAsync(out, g->g[which], s->ncell * sizeof(double), hipMemcpyDeviceToHost, s->stream), "read gradient");
@@ -1358,7 +1490,8 @@ int vse_grad_read(vse_grad *g, int which, double *out) {
 }
 
 int vse_grad_read_state(vse_grad *g, int field, float *out) {
-  if (!g || field < 0 || field > 4 || !out) return VSE_ERR_ARG;
+  if (int rc = grad_usable(g)) return rc;
+  if (field < 0 || field > 4 || !out) return VSE_ERR_ARG;
   OnDevice on_dev(g->s->device);
   if (g->n_fwd < 1 || !g->last_seg_stored) return g->fail(VSE_ERR_STATE, "no forward state (run vse_grad_forward)");
   const int m = g->n_fwd, last = (m - 1) / g->K;

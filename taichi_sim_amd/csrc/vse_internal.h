@@ -89,6 +89,11 @@ struct StepArgs {
   int n_rec, rec_gate, trace_rows;
   const VSE_G int *rec_beg, *rec_cell, *rec_delay;
   VSE_G float *traces;               // (trace_rows, n_rec)
+  // the field the receivers sample (sigma_yy by default); a velocity channel
+  // is written by the velocity pass itself, so rec_after = 1 moves its
+  // gather into a one-workgroup launch after that pass
+  const VSE_G float *rec_field;
+  int rec_after;
 };
 
 // Sparse per-step work: source injection (WGSL variants), receiver gather,
@@ -97,7 +102,7 @@ struct SparseArgs {
   int ndim, nx, ny, nz;
   float dt;
   float *vx, *vy, *vz;
-  const float *rec_field;            // sigma_yy (2D) / sigma_zz (3D)
+  const float *rec_field;            // the receivers' channel (sigma_yy / sigma_zz by default)
   const float *rho;
   unsigned *v2max;
   int *step;
@@ -211,6 +216,10 @@ hipError_t vse_launch_stress2d(int variant, int visco, int ord, int math, const 
 hipError_t vse_launch_velocity2d(int variant, int visco, int ord, int math, const StepArgs &A, const KTune &t, int i,
                                  hipStream_t st);
 hipError_t vse_launch_advance(int *step, int n, hipStream_t st);
+// receivers of a velocity channel (A.rec_after): the gather of step *A.step + i
+// after the velocity pass; the batch form gathers every member with rec_after
+hipError_t vse_launch_record2d(const StepArgs &A, int i, hipStream_t st);
+hipError_t vse_launch_record2d_batch(const BatchArgs &B, int i, hipStream_t st);
 hipError_t vse_launch_stress3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
 hipError_t vse_launch_velocity3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
 hipError_t vse_launch_sparse(const SparseArgs &S, hipStream_t st);

@@ -307,10 +307,12 @@ __global__ __launch_bounds__(BX * BY) void k_stress2d(StepArgs A, int nty, int r
 //   after the update, anywhere on the grid, rho the vy-grid density with
 //   out-of-grid neighbours 0 (wgsl2d :1059-1065); |v|^2 of the cell is
 //   counted before and after;
-// * receivers of step s, by one workgroup: traces[s, r] += sigma_yy over the
-//   receiver's points in table order when s + gate >= delay (wgsl2d
-//   :1117-1134; sim_taichi_elastic.py:187-190).  sigma_yy is final after the
-//   stress pass and this pass does not write it.
+// * receivers of step s, by one workgroup: traces[s, r] += the channel
+//   (sigma_yy by default) over the receiver's points in table order when
+//   s + gate >= delay (wgsl2d :1117-1134; sim_taichi_elastic.py:187-190).  A
+//   stress channel is final after the stress pass and this pass does not
+//   write it; a velocity channel (rec_after) is gathered by k_record2d after
+//   this pass instead.
 // The step is s = *A.step + i: a graph of k steps passes i = 0..k-1 and ends
 // with an advance of the counter by k, so it moves once per graph launch and
 // no workgroup waits for another (wgsl2d :1136-1141 advances it per step).
@@ -343,7 +345,7 @@ __device__ __forceinline__ void record_receivers(const StepArgs &A, int s) {
     if (s + A.rec_gate >= A.rec_delay[r]) {
       VSE_G float *tr = A.traces + (long)s * A.n_rec + r;
       float acc = *tr;
-      for (int p = A.rec_beg[r]; p < A.rec_beg[r + 1]; p++) acc = acc + A.syy[A.rec_cell[p]];
+      for (int p = A.rec_beg[r]; p < A.rec_beg[r + 1]; p++) acc = acc + A.rec_field[A.rec_cell[p]];
       *tr = acc;
     }
   }
@@ -357,8 +359,19 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A, int nty, int
   const int s = *A.step + i;
   unsigned v2 = velocity2d_cell<VAR, VISCO, ORD, FAST>(A, x, y, s);
   source_cell<VAR>(A, x, y, s, v2);
-  if (blockIdx.x == 0) record_receivers(A, s);
+  if (blockIdx.x == 0 && !A.rec_after) record_receivers(A, s);
   wave_max_to_slot(v2, A.v2slots + blockIdx.x);
+}
+
+// Receivers of a velocity channel, after the velocity pass that wrote it
+// (one workgroup; step *A.step + i as in k_velocity2d).
+__global__ __launch_bounds__(256) void k_record2d(StepArgs A, int i) { record_receivers(A, *A.step + i); }
+
+// The same for every member of a batch that records a velocity channel
+// (workgroup m serves member m).
+__global__ __launch_bounds__(256) void k_record2d_batch(BatchArgs B, int i) {
+  const StepArgs &A = B.args[blockIdx.x];
+  if (A.rec_after) record_receivers(A, *A.step + i);
 }
 
 __global__ __launch_bounds__(64) void k_advance(int *step, int n) {
@@ -416,7 +429,7 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B, int i
     source_cell<VAR>(A, x0 + k * BX, y, s, u);
     v2 = v2 > u ? v2 : u;
   }
-  if (t == 0) record_receivers(A, s);
+  if (t == 0 && !A.rec_after) record_receivers(A, s);
   wave_max_to_slot(v2, A.v2slots + t);
 }
 
@@ -502,5 +515,13 @@ hipError_t vse_launch_velocity2d(int variant, int visco, int ord, int math, cons
 }
 hipError_t vse_launch_advance(int *step, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, step, n);
+  return hipGetLastError();
+}
+hipError_t vse_launch_record2d(const StepArgs &A, int i, hipStream_t st) {
+  hipLaunchKernelGGL(k_record2d, dim3(1), dim3(256), 0, st, A, i);
+  return hipGetLastError();
+}
+hipError_t vse_launch_record2d_batch(const BatchArgs &B, int i, hipStream_t st) {
+  hipLaunchKernelGGL(k_record2d_batch, dim3((unsigned)B.n), dim3(256), 0, st, B, i);
   return hipGetLastError();
 }

@@ -119,6 +119,8 @@ class VSESim:
             raise _lib.VSEError(rc, "vse_create failed (invalid description or HIP error)")
         self._h = h
         self.n_rec = 0
+        self.channel = "syy" if ndim == 2 else "szz"
+        self.math = "strict"
         self._bands = {}
         self._dependents = weakref.WeakSet()   # open batches / gradient engines holding this handle
 
@@ -168,15 +170,22 @@ class VSESim:
         self._check(self._lib.vse_set_sources(self._h, cells.size, _iptr(cells), _iptr(cols),
                                               src.shape[1], _fptr(src)))
 
-    def set_receivers(self, beg, cells, delay, gate):
+    def set_receivers(self, beg, cells, delay, gate, channel=None):
+        """Receiver CSR (see ``vse_set_receivers``).  channel: the field the
+        receivers record ('syy', 'vy', ...; None = the reference's, sigma_yy in
+        2D and sigma_zz in 3D).  In 2D the grid's y axis is the ROI's vertical
+        z axis, so its vertical particle velocity is 'vy'."""
         beg = np.ascontiguousarray(beg, dtype=np.int32)
         cells = np.ascontiguousarray(cells, dtype=np.int32)
         delay = np.ascontiguousarray(delay, dtype=np.int32)
         n_rec = beg.size - 1
         if delay.size != n_rec:
             raise ValueError("one delay per receiver")
-        self._check(self._lib.vse_set_receivers(self._h, n_rec, _iptr(beg), _iptr(cells), _iptr(delay), int(gate)))
+        ch = -1 if channel is None else _lib.FIELD[channel]
+        self._check(self._lib.vse_set_receivers(self._h, n_rec, _iptr(beg), _iptr(cells), _iptr(delay), int(gate),
+                                                ch))
         self.n_rec = n_rec
+        self.channel = channel or ("syy" if self.ndim == 2 else "szz")
 
     # -------------------------------------------------------------------- run
     def reset(self):
@@ -209,6 +218,7 @@ class VSESim:
         members of a batch must all use the same mode.
         """
         self._check(self._lib.vse_set_math(self._h, {"strict": 0, "fast": 1}[mode]))
+        self.math = mode
 
     def band_cells(self):
         """Cells per psi slot-axis band: {axis: band width} from the CPML profiles."""
@@ -264,8 +274,9 @@ class VSESim:
 
     # ------------------------------------------------------ from a Simulator
     @classmethod
-    def from_simulator(cls, sim, variant="wgsl", visco=None, att=None):
-        """Build from an object carrying the reference's Simulator attributes."""
+    def from_simulator(cls, sim, variant="wgsl", visco=None, att=None, channel=None):
+        """Build from an object carrying the reference's Simulator attributes;
+        channel: the receivers' field (None = the reference's σyy / σzz)."""
         ndim = 3 if np.asarray(sim._rho_grid_vx).ndim == 3 else 2
         if visco is None:
             visco = bool(sim._configs.get("simul_configs", {}).get("viscoelastic_attn", 0))
@@ -292,7 +303,7 @@ class VSESim:
         delay = np.zeros(int(sim._n_rec), dtype=np.int32)
         dr = np.asarray(sim._delay_recv, dtype=np.int32).reshape(-1)
         delay[:min(dr.size, delay.size)] = dr[:delay.size]
-        self.set_receivers(beg, cells, delay, gate)
+        self.set_receivers(beg, cells, delay, gate, channel)
         self.reset()
         return self
 
@@ -329,8 +340,8 @@ class VSEBatch:
         h = ctypes.c_void_p()
         rc = self._lib.vse_batch_create(arr, len(self.sims), ctypes.byref(h))
         if rc != 0 or not h.value:
-            raise _lib.VSEError(rc, "vse_batch_create failed: members must be open 2D WGSL-variant simulations on "
-                                    "one device with the same viscoelastic flag, FD order and mechanism count")
+            raise _lib.VSEError(rc, "vse_batch_create failed: members must be open 2D simulations on one device "
+                                    "with the same variant, viscoelastic flag, FD order and mechanism count")
         self._h = h
         for s in self.sims:
             s._dependents.add(self)

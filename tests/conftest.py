@@ -25,6 +25,25 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU and the built libvse.so")
 
 
+def pytest_report_header(config):
+    # which HIP runtime libvse is bound to, and every runtime mapped in this process
+    try:
+        from taichi_sim_amd import _lib
+        lib = _lib.load()
+        return [f"libvse HIP runtime: {_lib.runtime_path(lib)}", f"mapped HIP/HSA runtimes: {_lib.mapped_runtimes()}"]
+    except ImportError as e:
+        return [f"libvse not loaded: {e}"]
+
+
+def pytest_terminal_summary(terminalreporter):
+    # after collection and every test: the runtimes the whole session mapped
+    try:
+        from taichi_sim_amd import _lib
+        terminalreporter.write_line(f"mapped HIP/HSA runtimes at the end: {_lib.mapped_runtimes()}")
+    except ImportError:
+        pass
+
+
 @pytest.fixture(autouse=True)
 def _announce(request):
     # name the case before any GPU work, so a crash can be attributed

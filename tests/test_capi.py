@@ -34,13 +34,20 @@ def test_library_runs_on_the_rocm_runtime_it_was_built_with():
 
 def test_abi_version_and_arg_validation_without_gpu():
     lib = _lib.load()
-    assert lib.vse_abi_version() == 1
+    assert lib.vse_abi_version() == _lib.ABI_VERSION == 2
     # argument validation happens before any device call
     d = _lib.VseDesc()
     d.ndim = 4
     h = ctypes.c_void_p()
     assert lib.vse_create(ctypes.byref(d), ctypes.byref(h)) == _lib.VSE_ERR_ARG
     assert not h.value
+    # grids beyond the kernels' 32-bit buffer addressing are refused, not run wrong:
+    # 2D with 2^30 cells or more, 3D with y-z planes of 2^30 cells or more
+    d = _lib.VseDesc(ndim=2, nx=32768, ny=32768, nz=1, dx=0.1, dy=0.1, dz=1.0, dt=1e-3, ord=2, variant=0, visco=0,
+                     n_sls=1, n_rows=1)
+    assert lib.vse_create(ctypes.byref(d), ctypes.byref(h)) == _lib.VSE_ERR_ARG and not h.value
+    d.ndim, d.nx, d.ny, d.nz, d.dz = 3, 8, 32768, 32768, 0.1
+    assert lib.vse_create(ctypes.byref(d), ctypes.byref(h)) == _lib.VSE_ERR_ARG and not h.value
     assert lib.vse_last_error(None) == b"null handle"
     # batches: member validation precedes any device call
     bh = ctypes.c_void_p()

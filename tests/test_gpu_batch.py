@@ -47,14 +47,7 @@ def _assert_same(a, b, what):
         assert np.array_equal(a[k], b[k]), (what, k, float(np.abs(a[k] - b[k]).max()))
 
 
-_ABORTED = ("the process aborted (SIGABRT, no runtime message) in vse_destroy of a solo simulation "
-            "after every comparison had passed; no cause found in the library's host or device code, "
-            "so Taichi-variant batches are switched off (vse_batch_create rejects them) until one is shown")
-
-
-@pytest.mark.parametrize("variant,grouping", [("wgsl", "lockstep"), ("wgsl", "mixed"),
-                                              pytest.param("taichi", "default",
-                                                           marks=pytest.mark.skip(reason=_ABORTED))])
+@pytest.mark.parametrize("variant,grouping", [("wgsl", "lockstep"), ("wgsl", "mixed"), ("taichi", "default")])
 def test_batch_uneven_specimens_match_solo_runs(variant, grouping):
     steps = 240
     print(f"batch uneven specimens {variant} {grouping}", flush=True)

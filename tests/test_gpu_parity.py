@@ -86,7 +86,7 @@ def test_2d_elastic_point_full_size(variant, channel):
 
 @pytest.mark.parametrize("channel", ["vz", "vx", "sxz"])
 def test_3d_receiver_channels(channel):
-    cfg = workloads.visco_3d_block(n=(48, 46, 50), steps=60, n_incl=2, n_rec_side=4)
+    cfg = workloads.visco_3d_block(n=(58, 56, 54), steps=60, n_incl=2, n_rec_side=4)
     _compare(Simulator3D(cfg), "wgsl", 60, att=attenuation_from_config(cfg), check_f64=False, channel=channel)
 
 

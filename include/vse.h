@@ -123,6 +123,26 @@ int vse_reset(vse_sim *s);
  * If elapsed_ms is not NULL it receives the HIP-event time of the loop. */
 int vse_run(vse_sim *s, int n_steps, int check_every, float *elapsed_ms);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: vse_run's time steps, launch for launch the same kernels
+ * (results identical to vse_run's), enqueued on `stream` (a hipStream_t; NULL =
+ * the handle's own stream) with no allocation, no graph or event creation
+ * and no host synchronisation.  The step counter, the running max of |v|^2
+ * and the 2D per-workgroup |v|^2 slots live in the library's static device
+ * scratch (2 + VSE_NOALLOC_SLOTS words per GPU, held until the process
+ * ends) instead of the handle's: the call copies the handle's counter and
+ * max in, clears the slots, and copies both back after the last step, so
+ * vse_get_step / vse_read_v2max report them.  The stability check needs the
+ * host: after synchronising the stream, read vse_read_v2max (+inf or NaN =
+ * unstable).  Every call on one GPU shares that GPU's scratch: calls that
+ * overlap on one GPU (other streams, other threads) race.  The handle must
+ * be ready (vse_reset or vse_run called once, which allocates its state);
+ * a 2D grid needing more than VSE_NOALLOC_SLOTS velocity workgroups (about
+ * 4096^2 cells) returns VSE_ERR_ARG. */
+#define VSE_NOALLOC_SLOTS 65536
+int vse_run_noalloc(vse_sim *s, int n_steps, void *stream);
+
 /* Run n_steps steps launching each kernel between its own HIP events on the
  * library's stream; ms[0..2] receive the average duration of the stress,
  * velocity and sparse (source/receiver/counter) kernels.  In 2D the sources

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -115,6 +116,49 @@ struct vse_sim {
 
 static int check_handle(vse_sim *s) { return s ? 0 : VSE_ERR_ARG; }
 
+// ---- scratch of vse_run_noalloc ----------------------------------------------
+// Word 0: running max of |v|^2 (float bits), word 1: step counter, words
+// 2 .. 2 + VSE_NOALLOC_SLOTS: the 2D velocity workgroups' |v|^2 slots.  Each
+// GPU holds its own copy until the process ends.  In host code `scratch` is
+// only a stand-in symbol: its device address, per device, is scratch_dev[]
+// (hipGetSymbolAddress at the device's first vse_create).
+#define VSE_SCRATCH_WORDS (2 + VSE_NOALLOC_SLOTS)
+static __device__ __align__(256) unsigned scratch[VSE_SCRATCH_WORDS];
+
+constexpr int kMaxDevices = 64;
+static std::mutex scratch_mu;
+static unsigned *scratch_dev[kMaxDevices] = {};
+
+// Looks up (once per device, with that device current) the device address of
+// this GPU's scratch copy; 0 on success.
+static int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return VSE_ERR_ARG;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_dev[device]) return 0;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return VSE_ERR_HIP;
+  scratch_dev[device] = (unsigned *)p;
+  return 0;
+}
+
+static unsigned *scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  return scratch_dev[device];
+}
+
+// scr[0..1] = ctrl[0..1] (the handle's running max and step), scr[2 .. 2+n) = 0
+__global__ __launch_bounds__(256) void k_scratch_begin(unsigned *scr, const unsigned *ctrl, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2) scr[i] = ctrl[i];
+  else if (i < 2 + n) scr[i] = 0u;
+}
+
+// ctrl[0..1] = scr[0..1]
+__global__ __launch_bounds__(64) void k_scratch_end(const unsigned *scr, unsigned *ctrl) {
+  if (threadIdx.x < 2) ctrl[threadIdx.x] = scr[threadIdx.x];
+}
+
 // vse_destroy(s) tells the handles built on s that it is gone (defined below
 // with their types): a batch forgets the member, a gradient handle releases
 // its device memory and reports VSE_ERR_STATE from then on.
@@ -195,6 +239,7 @@ int vse_create(const vse_desc *desc, vse_sim **out) {
   }
   s->ncell = (long)s->d.nx * s->d.ny * s->d.nz;
   int rc = s->hip(hipGetDevice(&s->device), "hipGetDevice");
+  if (!rc && scratch_setup(s->device)) rc = s->fail(VSE_ERR_HIP, "hipGetSymbolAddress of the scratch array failed");
   if (!rc) rc = s->hip(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
   if (!rc) {
     hipDeviceProp_t prop;
@@ -496,26 +541,30 @@ static void make_args(vse_sim *s, StepArgs &A, SparseArgs &S) {
 // step *counter + i and does its sources and receivers, and advance_steps()
 // moves the counter after the k steps.  3D: the sparse kernel does that work
 // and increments the counter itself.
-static hipError_t enqueue_step(vse_sim *s, const StepArgs &A, const SparseArgs &S, int i) {
+// The counter, running max and slots are the ones A and S point at (the
+// handle's own, or the no-allocation entry point's scratch); st = 0 is the
+// handle's stream.
+static hipError_t enqueue_step(vse_sim *s, const StepArgs &A, const SparseArgs &S, int i, hipStream_t st = 0) {
   const vse_desc &d = s->d;
+  if (!st) st = s->stream;
   hipError_t e;
   if (d.ndim == 2) {
-    e = vse_launch_stress2d(d.variant, d.visco, d.ord, s->math, A, s->tune[0], s->stream);
+    e = vse_launch_stress2d(d.variant, d.visco, d.ord, s->math, A, s->tune[0], st);
     if (e != hipSuccess) return e;
-    e = vse_launch_velocity2d(d.variant, d.visco, d.ord, s->math, A, s->tune[1], i, s->stream);
-    if (e == hipSuccess && A.rec_after) e = vse_launch_record2d(A, i, s->stream);
+    e = vse_launch_velocity2d(d.variant, d.visco, d.ord, s->math, A, s->tune[1], i, st);
+    if (e == hipSuccess && A.rec_after) e = vse_launch_record2d(A, i, st);
     return e;
   } else {
-    e = vse_launch_stress3d(d.visco, d.ord, s->math, A, s->tune[0], s->stream);
+    e = vse_launch_stress3d(d.visco, d.ord, s->math, A, s->tune[0], st);
     if (e != hipSuccess) return e;
-    e = vse_launch_velocity3d(d.visco, d.ord, s->math, A, s->tune[1], s->stream);
+    e = vse_launch_velocity3d(d.visco, d.ord, s->math, A, s->tune[1], st);
   }
   if (e != hipSuccess) return e;
-  return vse_launch_sparse(S, s->stream);
+  return vse_launch_sparse(S, st);
 }
 
-static hipError_t advance_steps(vse_sim *s, int k) {
-  return s->d.ndim == 2 ? vse_launch_advance((int *)(s->ctrl + 1), k, s->stream) : hipSuccess;
+static hipError_t advance_steps(vse_sim *s, const StepArgs &A, int k, hipStream_t st = 0) {
+  return s->d.ndim == 2 ? vse_launch_advance((int *)A.step, k, st ? st : s->stream) : hipSuccess;
 }
 
 static int get_graph(vse_sim *s, int steps, const StepArgs &A, const SparseArgs &S) {
@@ -526,7 +575,7 @@ static int get_graph(vse_sim *s, int steps, const StepArgs &A, const SparseArgs 
   if (rc) return rc;
   hipError_t e = hipSuccess;
   for (int i = 0; i < steps && e == hipSuccess; i++) e = enqueue_step(s, A, S, i);
-  if (e == hipSuccess) e = advance_steps(s, steps);
+  if (e == hipSuccess) e = advance_steps(s, A, steps);
   hipError_t e2 = hipStreamEndCapture(s->stream, &g);
   if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return s->hip(e, "capture launch"); }
   rc = s->hip(e2, "end capture");
@@ -584,7 +633,7 @@ int vse_run(vse_sim *s, int n_steps, int check_every, float *elapsed_ms) {
       since_check += chunk;
     } else {
       for (int i = 0; i < left && !rc; i++) rc = s->hip(enqueue_step(s, A, S, i), "step launch");
-      if (!rc) rc = s->hip(advance_steps(s, left), "step launch");
+      if (!rc) rc = s->hip(advance_steps(s, A, left), "step launch");
       done += left;
       since_check += left;
     }
@@ -602,6 +651,40 @@ int vse_run(vse_sim *s, int n_steps, int check_every, float *elapsed_ms) {
   if (e1) (void)hipEventDestroy(e1);
   if (!rc) rc = check_stable(s);
   return rc;
+}
+
+int vse_run_noalloc(vse_sim *s, int n_steps, void *stream) {
+  if (check_handle(s) || n_steps < 0) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
+  if (!s->psi_ready || !s->have_material || !s->src || !s->rec_beg)
+    return s->fail(VSE_ERR_STATE, "vse_run_noalloc: call vse_reset (or vse_run) first; that allocates the state");
+  if (s->n_slots > VSE_NOALLOC_SLOTS)
+    return s->fail(VSE_ERR_ARG, "vse_run_noalloc: the grid has " + std::to_string(s->n_slots) +
+                                    " velocity workgroups, the scratch holds VSE_NOALLOC_SLOTS = " +
+                                    std::to_string(VSE_NOALLOC_SLOTS));
+  unsigned *scr = scratch_of(s->device);
+  if (!scr) return s->fail(VSE_ERR_STATE, "vse_run_noalloc: no scratch address for this device");
+  if (n_steps == 0) return 0;
+  const hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+  StepArgs A;
+  SparseArgs S;
+  make_args(s, A, S);
+  A.v2max = scr;
+  A.step = (const int *)(scr + 1);
+  A.v2slots = s->n_slots ? scr + 2 : nullptr;
+  S.v2max = scr;
+  S.step = (int *)(scr + 1);
+  const int words = 2 + s->n_slots;
+  hipLaunchKernelGGL(k_scratch_begin, dim3((words + 255) / 256), dim3(256), 0, st, scr, s->ctrl, s->n_slots);
+  hipError_t e = hipGetLastError();
+  for (int i = 0; i < n_steps && e == hipSuccess; i++) e = enqueue_step(s, A, S, i, st);
+  if (e == hipSuccess) e = advance_steps(s, A, n_steps, st);
+  if (e == hipSuccess && s->n_slots) e = vse_launch_fold_v2(scr + 2, s->n_slots, scr, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_scratch_end, dim3(1), dim3(64), 0, st, scr, s->ctrl);
+    e = hipGetLastError();
+  }
+  return s->hip(e, "vse_run_noalloc launch");
 }
 
 int vse_profile(vse_sim *s, int n_steps, float *ms) {
@@ -625,7 +708,7 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
                                          : vse_launch_velocity3d(d.visco, d.ord, s->math, A, s->tune[1], s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
     if (e == hipSuccess && d.ndim == 2 && A.rec_after) e = vse_launch_record2d(A, 0, s->stream);
-    if (e == hipSuccess) e = d.ndim == 2 ? advance_steps(s, 1) : vse_launch_sparse(S, s->stream);
+    if (e == hipSuccess) e = d.ndim == 2 ? advance_steps(s, A, 1) : vse_launch_sparse(S, s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[3], s->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
     rc = s->hip(e, "profile step");

@@ -197,6 +197,13 @@ class VSESim:
                                       ctypes.byref(ms) if timed else None))
         return ms.value if timed else None
 
+    def run_noalloc(self, n_steps, stream=None):
+        """vse_run_noalloc: the same steps with no device allocation and no host
+        synchronisation, on ``stream`` (a raw hipStream_t of libvse's own HIP
+        runtime; None = the handle's stream).  No stability check: read
+        ``v2max()`` afterwards (it synchronises)."""
+        self._check(self._lib.vse_run_noalloc(self._h, int(n_steps), stream))
+
     def profile(self, n_steps):
         """Average (stress, velocity, sparse) kernel durations in ms over n_steps
         steps, each kernel bracketed by HIP events on the library's stream.  In

@@ -168,6 +168,23 @@ int vse_profile(vse_sim *s, int n_steps, float *ms);
  * sim_webgpu_3D_viscoelastic.py:40-42). */
 int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap);
 
+/* Schedule of the 3D step.  VSE_SCHED_TWO_PASS (default): a stress pass and
+ * a velocity pass per step, as the reference dispatches them
+ * (sim_webgpu_3D_viscoelastic.py:646-651).  VSE_SCHED_FUSED: one kernel per
+ * step does the velocity update of step n and the stress update of step n+1
+ * (velocities recomputed on a 2-cell halo of each tile; sigma, v and the
+ * velocity-pass CPML memory double-buffered), so the fields cross HBM once
+ * per step; results are bit-identical to the two-pass schedule.  It covers
+ * 3D viscoelastic simulations with 3 mechanisms and FD order 2 (acc = 2);
+ * anything else returns VSE_ERR_ARG.  Its launch shape is vse_set_tuning
+ * kernel 2 (ty in {4, 8}, nt 0, xchunk >= 1, remap); vse_profile then reports
+ * (fused kernel, 0, sparse kernel).  After a run the handle's fields may sit in
+ * the second buffer set: vse_field_device_ptr changes, vse_read_* follow.
+ * vse_run_noalloc always runs the two-pass schedule. */
+#define VSE_SCHED_TWO_PASS 0
+#define VSE_SCHED_FUSED 1
+int vse_set_schedule(vse_sim *s, int sched);
+
 /* Arithmetic mode of the forward kernels.  VSE_MATH_STRICT (default) keeps the
  * reference's FP32 operation order term for term (coef*(a-b)/dx as in
  * shader_webgpu_3D_viscoelastic.wgsl:1367-1378 and

@@ -11,7 +11,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvse.so")
-SOURCES = ["vse_k2d.hip", "vse_k3d.hip", "vse_sparse.hip", "vse_adj2d.hip", "vse_capi.hip", "vse_copy.hip"]
+SOURCES = ["vse_k2d.hip", "vse_k3d.hip", "vse_fused3d.hip", "vse_sparse.hip", "vse_adj2d.hip", "vse_capi.hip",
+           "vse_copy.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VSE_OFFLOAD_ARCH", "gfx950")
 

@@ -51,7 +51,7 @@ MEMORY = {"xx": 0, "yy": 1, "zz": 2, "xy": 3, "xz": 4, "yz": 5}
 
 # every symbol include/vse.h declares
 EXPORTS = ["vse_abi_version", "vse_create", "vse_set_material", "vse_set_cpml", "vse_set_sources",
-           "vse_set_receivers", "vse_reset", "vse_run", "vse_run_noalloc", "vse_profile", "vse_set_tuning", "vse_set_math", "vse_read_field", "vse_read_traces",
+           "vse_set_receivers", "vse_reset", "vse_run", "vse_run_noalloc", "vse_profile", "vse_set_tuning", "vse_set_math", "vse_set_schedule", "vse_read_field", "vse_read_traces",
            "vse_read_memory", "vse_read_v2max", "vse_get_step", "vse_field_device_ptr",
            "vse_device_bytes", "vse_device_name", "vse_last_error", "vse_destroy", "vse_copy_bandwidth",
            "vse_stream_bandwidth",
@@ -106,6 +106,7 @@ def load():
         "vse_profile": (I, [P, I, fp]),
         "vse_set_tuning": (I, [P, I, I, I, I, I]),
         "vse_set_math": (I, [P, I]),
+        "vse_set_schedule": (I, [P, I]),
         "vse_read_field": (I, [P, I, fp]),
         "vse_read_traces": (I, [P, fp]),
         "vse_read_memory": (I, [P, I, fp]),

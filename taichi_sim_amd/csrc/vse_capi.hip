@@ -60,11 +60,21 @@ struct vse_sim {
   int rec_channel = VSE_F_SYY;    // field the receivers sample (VSE_F_*)
   int *rec_beg = nullptr, *rec_cell = nullptr, *rec_delay = nullptr;
   float *traces = nullptr;
-  // launch shapes of the 3D stress / velocity kernels
-  KTune tune[2] = {{4, 0, 0, 0}, {4, 0, 8, 1}};
-  // cached graph
+  // 3D sources by (x, y) row (the fused step's table)
+  int *src_rbeg = nullptr, *src_rz = nullptr, *src_rcol = nullptr;
+  // launch shapes of the 3D stress / velocity kernels, and of the fused step
+  KTune tune[3] = {{4, 0, 0, 0}, {4, 0, 8, 1}, {8, 0, 32, 2}};
+  // schedule of the 3D step; the fused one needs a second set of sigma, v and
+  // velocity-pass psi buffers (set 1; set 0 is field[] / psi[])
+  int sched = VSE_SCHED_TWO_PASS;
+  float *alt[9] = {};
+  float *psi_alt[P_COUNT] = {};
+  bool alt_ready = false;
+  // cached graphs (two-pass steps; fused steps)
   hipGraphExec_t gexec = nullptr;
   int gexec_steps = 0;
+  hipGraphExec_t fgexec = nullptr;
+  int fgexec_steps = 0;
   // handles built on this simulation; vse_destroy detaches them, so any
   // destroy order is safe (a detached handle reports VSE_ERR_STATE)
   std::vector<vse_batch *> batches;
@@ -97,6 +107,9 @@ struct vse_sim {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     gexec = nullptr;
     gexec_steps = 0;
+    if (fgexec) (void)hipGraphExecDestroy(fgexec);
+    fgexec = nullptr;
+    fgexec_steps = 0;
   }
   long axis_len(int a) const { return a == 0 ? d.nx : (a == 1 ? d.ny : d.nz); }
   long psi_elems(int slot) const {
@@ -372,14 +385,16 @@ int vse_set_sources(vse_sim *s, int n_pts, const int *cells, const int *cols, in
     if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
       return s->fail(VSE_ERR_ARG, "source cell listed twice");
   }
-  std::vector<int> map, rows;
+  std::vector<int> map, rows, rbeg, rz, rcol;
   SyncOnExit sync(s->stream);     // after the vectors: waits before they are freed
   // the cached graph holds the old buffers' addresses: drop it before they go
   s->drop_graph();
   (void)hipStreamSynchronize(s->stream);
   s->release(s->src_cell); s->release(s->src_col); s->release(s->src); s->release(s->src_map);
   s->release(s->src_rowflag);
+  s->release(s->src_rbeg); s->release(s->src_rz); s->release(s->src_rcol);
   s->src_cell = s->src_col = s->src_map = s->src_rowflag = nullptr;
+  s->src_rbeg = s->src_rz = s->src_rcol = nullptr;
   s->src = nullptr;
   s->n_src_pts = 0;
   int rc = s->alloc(&s->src_cell, n_pts);
@@ -399,6 +414,30 @@ int vse_set_sources(vse_sim *s, int n_pts, const int *cells, const int *cols, in
     if (!rc) rc = s->hip(hipMemcpyAsync(s->src_map, map.data(), s->ncell * sizeof(int), hipMemcpyHostToDevice, s->stream), "src map");
     if (!rc) rc = s->alloc(&s->src_rowflag, s->d.nx);
     if (!rc) rc = s->hip(hipMemcpyAsync(s->src_rowflag, rows.data(), s->d.nx * sizeof(int), hipMemcpyHostToDevice, s->stream), "src rows");
+  }
+  if (!rc && s->d.ndim == 3) {     // the fused 3D step finds its sources by (x, y) row
+    const long nrow = (long)s->d.nx * s->d.ny;
+    rbeg.assign(nrow + 1, 0);
+    for (int i = 0; i < n_pts; i++) rbeg[cells[i] / s->d.nz + 1]++;
+    for (long r = 0; r < nrow; r++) rbeg[r + 1] += rbeg[r];
+    rz.resize(std::max(n_pts, 1));
+    rcol.resize(std::max(n_pts, 1));
+    std::vector<int> fill(rbeg.begin(), rbeg.end() - 1);
+    for (int i = 0; i < n_pts; i++) {
+      const int q = fill[cells[i] / s->d.nz]++;
+      rz[q] = cells[i] % s->d.nz;
+      rcol[q] = cols[i];
+    }
+    rows.assign(s->d.nx, 0);       // per x-plane: 1 if the plane holds a source cell
+    for (int i = 0; i < n_pts; i++) rows[cells[i] / ((long)s->d.ny * s->d.nz)] = 1;
+    rc = s->alloc(&s->src_rowflag, s->d.nx);
+    if (!rc) rc = s->hip(hipMemcpyAsync(s->src_rowflag, rows.data(), s->d.nx * sizeof(int), hipMemcpyHostToDevice, s->stream), "src planes");
+    if (!rc) rc = s->alloc(&s->src_rbeg, nrow + 1);
+    if (!rc) rc = s->alloc(&s->src_rz, n_pts);
+    if (!rc) rc = s->alloc(&s->src_rcol, n_pts);
+    if (!rc) rc = s->hip(hipMemcpyAsync(s->src_rbeg, rbeg.data(), (nrow + 1) * sizeof(int), hipMemcpyHostToDevice, s->stream), "src row table");
+    if (!rc && n_pts) rc = s->hip(hipMemcpyAsync(s->src_rz, rz.data(), n_pts * sizeof(int), hipMemcpyHostToDevice, s->stream), "src row z");
+    if (!rc && n_pts) rc = s->hip(hipMemcpyAsync(s->src_rcol, rcol.data(), n_pts * sizeof(int), hipMemcpyHostToDevice, s->stream), "src row col");
   }
   if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "set_sources");
   if (rc) return rc;
@@ -490,6 +529,10 @@ int vse_reset(vse_sim *s) {
   if (!rc) rc = s->hip(hipMemsetAsync(s->ctrl, 0, 2 * sizeof(unsigned), s->stream), "reset ctrl");
   if (!rc && s->v2slots)
     rc = s->hip(hipMemsetAsync(s->v2slots, 0, (size_t)s->n_slots * sizeof(unsigned), s->stream), "reset slots");
+  for (int f = 0; f < 9 && !rc && s->alt_ready; f++)
+    rc = s->hip(hipMemsetAsync(s->alt[f], 0, s->ncell * sizeof(float), s->stream), "reset");
+  for (int p = P_DSXX_DX; p < P_COUNT && !rc && s->alt_ready; p++)
+    rc = s->hip(hipMemsetAsync(s->psi_alt[p], 0, std::max<long>(s->psi_elems(p), 1) * sizeof(float), s->stream), "reset psi");
   if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "reset");
   return rc;
 }
@@ -523,6 +566,7 @@ static void make_args(vse_sim *s, StepArgs &A, SparseArgs &S) {
   A.rec_beg = s->rec_beg; A.rec_cell = s->rec_cell; A.rec_delay = s->rec_delay; A.traces = s->traces;
   A.rec_field = s->field[s->rec_channel];
   A.rec_after = velocity_field(s->rec_channel) ? 1 : 0;
+  A.src_rbeg = s->src_rbeg; A.src_rz = s->src_rz; A.src_rcol = s->src_rcol;
 
   memset(&S, 0, sizeof(S));
   S.ndim = d.ndim; S.nx = d.nx; S.ny = d.ny; S.nz = d.nz; S.dt = d.dt;
@@ -606,12 +650,217 @@ static int check_stable(vse_sim *s) {
   return 0;
 }
 
+// ---- fused 3D schedule --------------------------------------------------------
+// A run of n steps from state k: the stress pass of step k in place on set 0;
+// n-1 fused steps, each reading one buffer set and writing the other
+// (velocity of step m, stress of step m+1), each followed by the sparse kernel
+// of step m (receivers, counter); the velocity pass and sparse kernel of the
+// last step in place on the set in use.  The handle's field[] / psi[] then
+// point at the final state (the two sets are swapped when it ends in set 1).
+
+static bool fused_active(const vse_sim *s) {
+  return s->sched == VSE_SCHED_FUSED && s->d.ndim == 3 && s->d.variant == VSE_VARIANT_WGSL &&
+         vse_fused3d_supported(s->ncell, s->d.n_sls, s->d.visco, s->d.ord);
+}
+
+static float *set_field(vse_sim *s, int set, int f) { return set ? s->alt[f] : s->field[f]; }
+static float *set_psi(vse_sim *s, int set, int p) { return set && p >= P_DSXX_DX ? s->psi_alt[p] : s->psi[p]; }
+
+// The second buffer set (sigma, v, velocity-pass psi), zeroed, on first use.
+static int ensure_alt(vse_sim *s) {
+  if (s->alt_ready) return 0;
+  int rc = 0;
+  for (int f = 0; f < 9 && !rc; f++) rc = s->alloc(&s->alt[f], s->ncell);
+  for (int p = P_DSXX_DX; p < P_COUNT && !rc; p++) rc = s->alloc(&s->psi_alt[p], s->psi_elems(p));
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "second buffer set");
+  if (rc) {
+    for (int f = 0; f < 9; f++) { s->release(s->alt[f]); s->alt[f] = nullptr; }
+    for (int p = 0; p < P_COUNT; p++) { s->release(s->psi_alt[p]); s->psi_alt[p] = nullptr; }
+    return rc;
+  }
+  s->alt_ready = true;
+  return 0;
+}
+
+// make_args() of the state held in buffer set `set`
+static void make_args_set(vse_sim *s, StepArgs &A, SparseArgs &S, int set) {
+  make_args(s, A, S);
+  if (!set) return;
+  A.vx = s->alt[VSE_F_VX]; A.vy = s->alt[VSE_F_VY]; A.vz = s->alt[VSE_F_VZ];
+  A.sxx = s->alt[VSE_F_SXX]; A.syy = s->alt[VSE_F_SYY]; A.szz = s->alt[VSE_F_SZZ];
+  A.sxy = s->alt[VSE_F_SXY]; A.sxz = s->alt[VSE_F_SXZ]; A.syz = s->alt[VSE_F_SYZ];
+  for (int p = P_DSXX_DX; p < P_COUNT; p++) A.psi[p] = s->psi_alt[p];
+  A.rec_field = s->alt[s->rec_channel];
+  S.vx = A.vx; S.vy = A.vy; S.vz = A.vz;
+  S.rec_field = A.rec_field;
+}
+
+// One fused step reading set p and writing set 1-p, then the sparse kernel of
+// that step without sources (the fused kernel injected them): a stress channel
+// is read from set p (step m+1's stress is in set 1-p already), a velocity
+// channel from set 1-p.
+static hipError_t enqueue_fused(vse_sim *s, int p, hipStream_t st, hipEvent_t mid = nullptr) {
+  FusedArgs F;
+  SparseArgs S;
+  make_args_set(s, F.A, S, p);
+  const int o = 1 - p;
+  const int sig[6] = {VSE_F_SXX, VSE_F_SYY, VSE_F_SZZ, VSE_F_SXY, VSE_F_SXZ, VSE_F_SYZ};
+  for (int c = 0; c < 3; c++) F.vo[c] = set_field(s, o, VSE_F_VX + c);
+  for (int c = 0; c < 6; c++) F.so[c] = set_field(s, o, sig[c]);
+  for (int i = 0; i < 9; i++) F.psio[i] = set_psi(s, o, P_DSXX_DX + i);
+  hipError_t e = vse_launch_fused3d(s->math, F, s->tune[2], st);
+  if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
+  if (e != hipSuccess) return e;
+  S.n_src_pts = 0;
+  S.rec_field = set_field(s, velocity_field(s->rec_channel) ? o : p, s->rec_channel);
+  return vse_launch_sparse(S, st);
+}
+
+// graph of `steps` (even) fused steps starting from set 0
+static int get_fused_graph(vse_sim *s, int steps) {
+  if (s->fgexec && s->fgexec_steps == steps) return 0;
+  if (s->fgexec) (void)hipGraphExecDestroy(s->fgexec);
+  s->fgexec = nullptr;
+  s->fgexec_steps = 0;
+  hipGraph_t g = nullptr;
+  int rc = s->hip(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  if (rc) return rc;
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < steps && e == hipSuccess; i++) e = enqueue_fused(s, i & 1, s->stream);
+  hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+  if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return s->hip(e, "capture launch"); }
+  rc = s->hip(e2, "end capture");
+  if (rc) return rc;
+  rc = s->hip(hipGraphInstantiate(&s->fgexec, g, nullptr, nullptr, 0), "graph instantiate");
+  (void)hipGraphDestroy(g);
+  if (!rc) s->fgexec_steps = steps;
+  return rc;
+}
+
+// the final state is in set 1: make field[] / psi[] point at it
+static void swap_sets(vse_sim *s) {
+  for (int f = 0; f < 9; f++) std::swap(s->field[f], s->alt[f]);
+  for (int p = P_DSXX_DX; p < P_COUNT; p++) std::swap(s->psi[p], s->psi_alt[p]);
+  s->drop_graph();
+}
+
+// The last step's velocity pass and sparse kernel, in place on set p.
+static hipError_t enqueue_fused_tail(vse_sim *s, int p) {
+  StepArgs A;
+  SparseArgs S;
+  make_args_set(s, A, S, p);
+  hipError_t e = vse_launch_velocity3d(s->d.visco, s->d.ord, s->math, A, s->tune[1], s->stream);
+  return e == hipSuccess ? vse_launch_sparse(S, s->stream) : e;
+}
+
+static int run_fused(vse_sim *s, int n_steps, int check_every, float *elapsed_ms) {
+  int rc = ensure_alt(s);
+  if (rc) return rc;
+  StepArgs A;
+  SparseArgs S;
+  make_args(s, A, S);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (elapsed_ms) {
+    rc = s->hip(hipEventCreate(&e0), "event");
+    if (!rc) rc = s->hip(hipEventCreate(&e1), "event");
+    if (!rc) rc = s->hip(hipEventRecord(e0, s->stream), "event record");
+  }
+  if (!rc) rc = s->hip(vse_launch_stress3d(s->d.visco, s->d.ord, s->math, A, s->tune[0], s->stream), "stress launch");
+  const int nf = n_steps - 1;
+  int chunk = std::min(std::min(check_every > 0 ? check_every : 256, 256), nf) & ~1;
+  if (!rc && chunk >= 2) rc = get_fused_graph(s, chunk);
+  int p = 0, done = 0, since_check = 0;
+  while (!rc && done < nf) {
+    const int left = nf - done;
+    if (chunk >= 2 && left >= chunk) {
+      rc = s->hip(hipGraphLaunch(s->fgexec, s->stream), "graph launch");
+      done += chunk;
+      since_check += chunk;
+    } else {
+      for (int i = 0; i < left && !rc; i++, p ^= 1) rc = s->hip(enqueue_fused(s, p, s->stream), "fused launch");
+      done += left;
+      since_check += left;
+    }
+    if (!rc && check_every > 0 && since_check >= check_every && done < nf) {
+      rc = check_stable(s);
+      since_check = 0;
+    }
+  }
+  if (!rc) rc = s->hip(enqueue_fused_tail(s, p), "velocity launch");
+  if (elapsed_ms && !rc) {
+    rc = s->hip(hipEventRecord(e1, s->stream), "event record");
+    if (!rc) rc = s->hip(hipEventSynchronize(e1), "event sync");
+    if (!rc) rc = s->hip(hipEventElapsedTime(elapsed_ms, e0, e1), "elapsed");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (!rc && p == 1) {
+    (void)hipStreamSynchronize(s->stream);
+    swap_sets(s);
+  }
+  if (!rc) rc = check_stable(s);
+  return rc;
+}
+
+// vse_profile of the fused schedule: n_steps steps as run_fused does them,
+// eagerly; ms[0] = average fused kernel, ms[1] = 0, ms[2] = average sparse kernel.
+static int profile_fused(vse_sim *s, int n_steps, float *ms) {
+  if (n_steps < 2) return s->fail(VSE_ERR_ARG, "profiling the fused schedule takes 2 steps or more");
+  int rc = ensure_alt(s);
+  if (rc) return rc;
+  StepArgs A;
+  SparseArgs S;
+  make_args(s, A, S);
+  hipEvent_t ev[3] = {};
+  for (int i = 0; i < 3 && !rc; i++) rc = s->hip(hipEventCreate(&ev[i]), "event");
+  if (!rc) rc = s->hip(vse_launch_stress3d(s->d.visco, s->d.ord, s->math, A, s->tune[0], s->stream), "stress launch");
+  double acc[2] = {0, 0};
+  int p = 0;
+  for (int it = 0; it < n_steps - 1 && !rc; it++, p ^= 1) {
+    hipError_t e = hipEventRecord(ev[0], s->stream);
+    if (e == hipSuccess) e = enqueue_fused(s, p, s->stream, ev[1]);
+    if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
+    rc = s->hip(e, "profile step");
+    for (int k = 0; k < 2 && !rc; k++) {
+      float t = 0.f;
+      rc = s->hip(hipEventElapsedTime(&t, ev[k], ev[k + 1]), "elapsed");
+      acc[k] += t;
+    }
+  }
+  if (!rc) rc = s->hip(enqueue_fused_tail(s, p), "velocity launch");
+  if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "profile");
+  for (int i = 0; i < 3; i++)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
+  if (!rc && p == 1) swap_sets(s);
+  if (!rc) {
+    ms[0] = (float)(acc[0] / (n_steps - 1));
+    ms[1] = 0.f;
+    ms[2] = (float)(acc[1] / (n_steps - 1));
+    rc = check_stable(s);
+  }
+  return rc;
+}
+
+int vse_set_schedule(vse_sim *s, int sched) {
+  if (check_handle(s) || (sched != VSE_SCHED_TWO_PASS && sched != VSE_SCHED_FUSED)) return VSE_ERR_ARG;
+  OnDevice on_dev(s->device);
+  if (sched == VSE_SCHED_FUSED &&
+      !(s->d.ndim == 3 && s->d.variant == VSE_VARIANT_WGSL && vse_fused3d_supported(s->ncell, s->d.n_sls, s->d.visco, s->d.ord)))
+    return s->fail(VSE_ERR_ARG, "the fused step covers 3D viscoelastic simulations with 3 mechanisms and FD order 2 "
+                                "(acc = 2), with 3*cells*4 bytes below 2^32");
+  s->sched = sched;
+  s->drop_graph();
+  return VSE_OK;
+}
+
 int vse_run(vse_sim *s, int n_steps, int check_every, float *elapsed_ms) {
   if (check_handle(s) || n_steps < 0) return VSE_ERR_ARG;
   OnDevice on_dev(s->device);
   int rc = finalize(s);
   if (rc) return rc;
   if (n_steps == 0) { if (elapsed_ms) *elapsed_ms = 0.f; return 0; }
+  if (fused_active(s)) return run_fused(s, n_steps, check_every, elapsed_ms);
   StepArgs A;
   SparseArgs S;
   make_args(s, A, S);
@@ -692,6 +941,7 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
   OnDevice on_dev(s->device);
   int rc = finalize(s);
   if (rc) return rc;
+  if (fused_active(s)) return profile_fused(s, n_steps, ms);
   StepArgs A;
   SparseArgs S;
   make_args(s, A, S);
@@ -727,9 +977,11 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
 }
 
 int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap) {
-  if (check_handle(s) || kernel < 0 || kernel > 1) return VSE_ERR_ARG;
+  if (check_handle(s) || kernel < 0 || kernel > 2) return VSE_ERR_ARG;
   if (!(ty == 1 || ty == 4 || ty == 8) || (nt != 0 && nt != 1) || xchunk < 0 || remap < 0 || remap > 2)
     return s->fail(VSE_ERR_ARG, "tuning: ty in {1,4,8}, nt in {0,1}, xchunk >= 0, remap in {0,1,2}");
+  if (kernel == 2 && (ty == 1 || nt != 0 || xchunk < 1))
+    return s->fail(VSE_ERR_ARG, "fused-step tuning: ty in {4,8}, nt 0, xchunk >= 1");
   OnDevice on_dev(s->device);
   s->tune[kernel] = KTune{ty, nt, xchunk, remap};
   s->drop_graph();

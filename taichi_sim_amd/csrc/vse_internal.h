@@ -85,7 +85,7 @@ struct StepArgs {
   // 2D single-simulation launches: the velocity kernel also does the step's
   // sparse work -- the WGSL-variant source through src_map (rows flagged in
   // src_rowflag) and the receivers in workgroup 0
-  const VSE_G int *src_rowflag;      // per x-row: 1 if the row holds a source cell
+  const VSE_G int *src_rowflag;      // per x-row (3D: x-plane): 1 if it holds a source cell
   int n_rec, rec_gate, trace_rows;
   const VSE_G int *rec_beg, *rec_cell, *rec_delay;
   VSE_G float *traces;               // (trace_rows, n_rec)
@@ -94,6 +94,23 @@ struct StepArgs {
   // gather into a one-workgroup launch after that pass
   const VSE_G float *rec_field;
   int rec_after;
+  // 3D sources by (x, y) row, for the fused step: row r = x*ny + y holds the
+  // source points src_rbeg[r] .. src_rbeg[r+1] (their z and column)
+  const VSE_G int *src_rbeg, *src_rz, *src_rcol;
+};
+
+// Fused 3D step (vse_fused3d.hip): the velocity update of step n followed by
+// the stress update of step n+1 in one pass.  `A` holds the inputs -- sigma at
+// step n+1 (A.sxx .. A.syz), v at step n (A.vx .. A.vz) and the velocity-pass
+// psi at step n -- and the state updated in place (memory variables, the
+// stress-pass psi, the step counter, the running max); the outputs go to a
+// second set of buffers, because neighbouring workgroups recompute velocities
+// in their halos from the inputs.
+struct FusedArgs {
+  StepArgs A;
+  VSE_G float *vo[3];                // v at step n+1: x, y, z
+  VSE_G float *so[6];                // sigma at step n+2: xx, yy, zz, xy, xz, yz
+  VSE_G float *psio[9];              // velocity-pass psi at step n+1, slots P_DSXX_DX + i
 };
 
 // Sparse per-step work: source injection (WGSL variants), receiver gather,
@@ -221,6 +238,11 @@ hipError_t vse_launch_advance(int *step, int n, hipStream_t st);
 hipError_t vse_launch_record2d(const StepArgs &A, int i, hipStream_t st);
 hipError_t vse_launch_record2d_batch(const BatchArgs &B, int i, hipStream_t st);
 hipError_t vse_launch_stress3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
+// true when the fused kernel covers this problem: 3D, viscoelastic with 3
+// mechanisms, FD order 2, and every array addressable with 32-bit byte offsets
+bool vse_fused3d_supported(long ncell, int L, int visco, int ord);
+// t.ty in {4, 8}: the tile height; t.xchunk: x-planes per workgroup; t.remap: tile order
+hipError_t vse_launch_fused3d(int math, const FusedArgs &F, const KTune &t, hipStream_t st);
 hipError_t vse_launch_velocity3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
 hipError_t vse_launch_sparse(const SparseArgs &S, hipStream_t st);
 // the velocity pass (stress = false) advances every member's step *step + i

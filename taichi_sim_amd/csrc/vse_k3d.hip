@@ -24,35 +24,6 @@ namespace {
 
 constexpr int BZ = 64;
 
-// Tile assignment.  Tiles are ordered z-tile fastest, then y-tile, then
-// x-chunk.  remap = 0 dispatches them in that order (consecutive workgroups,
-// which land on XCDs round-robin, work on neighbouring tiles of one plane);
-// remap = 1 gives each XCD (dispatch bid lands on XCD bid % 8) a contiguous
-// eighth of the tiles.  remap = 2 gives XCD k the y-tiles [k, k+1) * nyb/8 over
-// every x-chunk, z fastest, then y, then x-chunk: the XCD's workgroups march
-// its whole y-band chunk after chunk, so the planes a chunk re-reads at its
-// start were read by the previous chunk of the same XCD shortly before
-// (nyb not a multiple of 8: as remap = 1).
-struct Tile { int bz, by, bxc; };
-__device__ __forceinline__ Tile tile_of_block(int nzb, int nyb, int nxc, int remap) {
-  const int bid = blockIdx.x;
-  const int total = nzb * nyb * nxc;
-  Tile r;
-  if (remap == 2 && (nyb & 7) == 0) {
-    const int k = bid & 7, j = bid >> 3, band = nyb >> 3;
-    r.bz = j % nzb;
-    r.by = k * band + (j / nzb) % band;
-    r.bxc = j / (nzb * band);
-    return r;
-  }
-  int t = bid;
-  if (remap && (total & 7) == 0) t = (bid & 7) * (total >> 3) + (bid >> 3);
-  r.bz = t % nzb;
-  r.by = (t / nzb) % nyb;
-  r.bxc = t / (nzb * nyb);
-  return r;
-}
-
 // One x-plane of a field as a raw buffer: a scalar (SGPR) resource
 // descriptor over the plane's bytes, addressed with a 32-bit per-lane offset.
 // The many loads in flight then cost one VGPR of address each (flat 64-bit

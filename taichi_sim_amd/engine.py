@@ -121,6 +121,7 @@ class VSESim:
         self.n_rec = 0
         self.channel = "syy" if ndim == 2 else "szz"
         self.math = "strict"
+        self.schedule = "two-pass"
         self._bands = {}
         self._dependents = weakref.WeakSet()   # open batches / gradient engines holding this handle
 
@@ -216,6 +217,13 @@ class VSESim:
     def set_tuning(self, kernel, ty, nt, xchunk, remap=0):
         """Launch shape of the 3D stress (kernel 0) or velocity (1) pass."""
         self._check(self._lib.vse_set_tuning(self._h, int(kernel), int(ty), int(nt), int(xchunk), int(remap)))
+
+    def set_schedule(self, sched):
+        """'two-pass' (a stress and a velocity pass per step, the default) or
+        'fused' (3D viscoelastic, 3 mechanisms, FD order 2: one kernel per step,
+        bit-identical results)."""
+        self._check(self._lib.vse_set_schedule(self._h, {"two-pass": 0, "fused": 1}[sched]))
+        self.schedule = sched
 
     def set_math(self, mode):
         """'strict' (reference operation order, default) or 'fast'.

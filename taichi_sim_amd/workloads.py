@@ -193,7 +193,7 @@ def adjoint_2d(n=1024, steps=3000, n_shots=8, n_anom=8, seed=5, amp=0.05, depth=
     return cfg, laws, true
 
 
-def visco_3d_block(n=512, steps=2000, n_incl=24, seed=3, q=50.0, n_rec_side=32, rec_pitch=1.0):
+def visco_3d_block(n=512, steps=2000, n_incl=24, seed=3, q=50.0, n_rec_side=32, rec_pitch=1.0, t0=1.0):
     roi, (L, Ly, Lz) = _roi(n, 3)
     rng = np.random.default_rng(seed)
     al_cp, al_cs, al_rho = AL
@@ -221,7 +221,7 @@ def visco_3d_block(n=512, steps=2000, n_incl=24, seed=3, q=50.0, n_rec_side=32, 
     radius = min(0.125, 0.3 * min(w, d) / 25.4)            # inches (simul_classes.py:548)
     rec_pitch = min(rec_pitch, 0.8 * min(w, d) / n_rec_side)
     probes = [{"mono": {"coord_center": [w / 2, d / 2, 0.0], "radius": radius, "freq": 5.0, "bw": 0.5,
-                        "gain": 1.0, "emitters": "True", "receivers": "False", "t0_emission": 1.0,
+                        "gain": 1.0, "emitters": "True", "receivers": "False", "t0_emission": float(t0),
                         "t0_reception": 0.0}}]
     off = (n_rec_side - 1) / 2 * rec_pitch
     for i in range(n_rec_side):

@@ -53,7 +53,7 @@ def test_implementation_2d_elastic_taichi():
 
 
 def test_implementation_3d_visco():
-    cfg = workloads.visco_3d_block(n=(52, 48, 56), steps=70, n_incl=3, n_rec_side=4)
+    cfg = workloads.visco_3d_block(n=(52, 48, 56), steps=70, n_incl=3, n_rec_side=4, t0=0.3)
     sim = SimulatorHIP3D(cfg)
     res = sim.implementation()
     o = oracle.Oracle(sim, variant="wgsl", att=attenuation_from_config(cfg)).run(70)

@@ -86,7 +86,7 @@ def test_2d_elastic_point_full_size(variant, channel):
 
 @pytest.mark.parametrize("channel", ["vz", "vx", "sxz"])
 def test_3d_receiver_channels(channel):
-    cfg = workloads.visco_3d_block(n=(58, 56, 54), steps=60, n_incl=2, n_rec_side=4)
+    cfg = workloads.visco_3d_block(n=(58, 56, 54), steps=60, n_incl=2, n_rec_side=4, t0=0.3)
     _compare(Simulator3D(cfg), "wgsl", 60, att=attenuation_from_config(cfg), check_f64=False, channel=channel)
 
 
@@ -134,14 +134,14 @@ def test_source_cell_listed_twice_is_rejected():
 
 @pytest.mark.parametrize("visco", [True, False])
 def test_3d_block(visco):
-    cfg = workloads.visco_3d_block(n=56, steps=80, n_incl=4, n_rec_side=4)
+    cfg = workloads.visco_3d_block(n=56, steps=80, n_incl=4, n_rec_side=4, t0=0.3)
     _compare(Simulator3D(cfg), "wgsl", 80, att=attenuation_from_config(cfg), visco=visco)
 
 
 @pytest.mark.parametrize("acc", [4, 5, 6])
 def test_3d_high_order(acc):
     # FD orders 4-6 on a ragged grid, x-march in tile order 2 for both passes
-    cfg = workloads.visco_3d_block(n=(50, 48, 52), steps=40, n_incl=2, n_rec_side=3)
+    cfg = workloads.visco_3d_block(n=(50, 48, 52), steps=40, n_incl=2, n_rec_side=3, t0=0.3)
     cfg["simul_params"]["acc"] = acc
     sim = Simulator3D(cfg)
     att = attenuation_from_config(cfg)
@@ -159,7 +159,7 @@ def test_3d_high_order(acc):
 
 def test_3d_launch_shapes_bit_identical():
     """Every 3D launch shape the autotuner may pick gives the same bits."""
-    cfg = workloads.visco_3d_block(n=(70, 64, 130), steps=40, n_incl=3, n_rec_side=3)     # ny/ty % 8 == 0
+    cfg = workloads.visco_3d_block(n=(70, 64, 130), steps=40, n_incl=3, n_rec_side=3, t0=0.3)     # ny/ty % 8 == 0
     sim = Simulator3D(cfg)
     att = attenuation_from_config(cfg)
     o = oracle.Oracle(sim, variant="wgsl", att=att).run(40)
@@ -181,10 +181,71 @@ def test_3d_launch_shapes_bit_identical():
     g.close()
 
 
+def _state3d(g):
+    out = {k: g.field(k) for k in ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")}
+    for c in ("xx", "yy", "zz", "xy", "xz", "yz"):
+        out["r" + c] = g.memory(c)
+    out["traces"] = g.traces()
+    out["step"] = np.array([g.step()])
+    out["v2max"] = np.array([g.v2max()], dtype=np.float32)
+    return out
+
+
+@pytest.mark.parametrize("math", ["fast", "strict"])
+@pytest.mark.parametrize("channel", [None, "vz"])
+def test_3d_fused_step_bit_identical(math, channel):
+    """The fused schedule (velocity of step n and stress of step n+1 in one
+    kernel) ends bit for bit where the two-pass schedule ends, for every launch
+    shape, for runs split into calls of 1, 2 and odd numbers of steps, with
+    CPML bands on every face and a piston source on the top face."""
+    steps = 91
+    cfg = workloads.visco_3d_block(n=(70, 66, 130), steps=steps, n_incl=3, n_rec_side=3, t0=0.3)
+    sim = Simulator3D(cfg)
+    att = attenuation_from_config(cfg)
+    ref = VSESim.from_simulator(sim, variant="wgsl", att=att, channel=channel)
+    ref.set_math(math)
+    ref.run(steps, check_every=10)
+    want = _state3d(ref)
+    ref.close()
+    assert np.abs(want["szz"]).max() > 0 and np.abs(want["traces"]).max() > 0
+    g = VSESim.from_simulator(sim, variant="wgsl", att=att, channel=channel)
+    g.set_math(math)
+    g.set_schedule("fused")
+    print(f"fused {math} channel {channel}", flush=True)
+    for ty, xc, rm, split in [(4, 16, 2, (91,)), (8, 16, 2, (1, 2, 88)), (4, 1, 0, (90, 1)), (8, 3, 1, (5, 86)),
+                              (4, 1000, 0, (91,)), (8, 7, 2, (50, 41))]:
+        assert sum(split) == steps
+        g.set_tuning(2, ty, 0, xc, rm)
+        g.reset()
+        for n in split:
+            g.run(n, check_every=8)
+        got = _state3d(g)
+        for k in want:
+            assert np.array_equal(got[k], want[k]), ((ty, xc, rm, split), k, float(np.abs(got[k] - want[k]).max()))
+    ms = g.profile(3)
+    assert ms[0] > 0 and ms[1] == 0
+    g.close()
+
+
+def test_3d_fused_schedule_refusals():
+    cfg = workloads.visco_3d_block(n=56, steps=4, n_incl=1, n_rec_side=2)
+    cfg["simul_params"]["acc"] = 4
+    sim = Simulator3D(cfg)
+    g = VSESim.from_simulator(sim, variant="wgsl", att=attenuation_from_config(cfg))
+    from taichi_sim_amd._lib import VSEError
+    with pytest.raises(VSEError, match="FD order 2"):
+        g.set_schedule("fused")
+    g.close()
+    g = VSESim.from_simulator(Simulator(workloads.elastic_2d_point(n=64, steps=4, n_rec=2)), variant="wgsl")
+    with pytest.raises(VSEError):
+        g.set_schedule("fused")
+    g.close()
+
+
 @pytest.mark.parametrize("visco", [True, False])
 def test_3d_fast_math_within_tolerance(visco):
     """FAST arithmetic (precomputed coef/dx, 1/k) stays within 1e-5 of the oracle."""
-    cfg = workloads.visco_3d_block(n=(64, 60, 70), steps=120, n_incl=4, n_rec_side=4)
+    cfg = workloads.visco_3d_block(n=(64, 60, 70), steps=120, n_incl=4, n_rec_side=4, t0=0.3)
     sim = Simulator3D(cfg)
     att = attenuation_from_config(cfg)
     o = oracle.Oracle(sim, variant="wgsl", att=att, visco=visco).run(120)
@@ -227,7 +288,7 @@ def test_2d_fast_math_within_tolerance(acc):
 
 
 def test_3d_ragged_order3():
-    cfg = workloads.visco_3d_block(n=(55, 53, 50), steps=50, n_incl=2, n_rec_side=3)
+    cfg = workloads.visco_3d_block(n=(55, 53, 50), steps=50, n_incl=2, n_rec_side=3, t0=0.3)
     cfg["simul_params"]["acc"] = 3
     _compare(Simulator3D(cfg), "wgsl", 50, att=attenuation_from_config(cfg))
 

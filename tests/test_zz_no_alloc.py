@@ -72,7 +72,7 @@ def test_noalloc_split_calls_and_velocity_channel():
 def test_noalloc_3d_matches_run():
     print("noalloc: 3D 58x56x54, then 3D 50x52x48", flush=True)
     for n, steps in (((58, 56, 54), 50), ((50, 52, 48), 30)):
-        cfg = workloads.visco_3d_block(n=n, steps=steps, n_incl=2, n_rec_side=3)
+        cfg = workloads.visco_3d_block(n=n, steps=steps, n_incl=2, n_rec_side=3, t0=0.3)
         sim = Simulator3D(cfg)
         ref, got = _pair(sim, attenuation_from_config(cfg))
         ref.run(steps)

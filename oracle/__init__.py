@@ -13,12 +13,18 @@ Pinning (each function is checked by at least one of these in tests/):
   pedro/results/result_teste_aco.npy (tests/test_oracle_golden.py);
 * all variants: zero source -> identically zero fields, linearity in the
   source, FP32 vs FP64 floor, elastic limit of the Zener path with tau_eps =
-  tau_sig, brute-force per-cell Python loops on tiny grids, and the Zener
-  recursion against its closed-form Crank-Nicolson solution
-  (tests/test_oracle_invariants.py);
-* 2D WGSL variant and 3D: no numeric output of the reference survives
-  (the WGSL backends need wgpu); beyond the invariants above they are
-  "parity unpinned" against the reference's own numbers.
+  tau_sig, mirror symmetry of a centred source (tests/test_oracle_invariants.py);
+* 2D WGSL, 2D Taichi and 3D, every update term: closed forms on synthetic
+  inputs (tests/test_oracle_closed_form.py) -- one step from cubic velocity or
+  stress fields, graded materials, random CPML profiles and random initial
+  psi, r and stresses (each staggered difference equals the derivative at the
+  half cell, each material mean the field at the centre, psi' = b psi + a D
+  with the half profile for D+ and the full one for D-), and the geometric
+  closed form of the Zener memory variables under a uniform strain rate over
+  many steps; a mistake planted in this oracle (a D+/D- swap, a coefficient
+  order, a half/full profile, an average, a Zener coefficient) fails them.
+  No numeric output of the WGSL backends survives (they need wgpu), so the
+  expected formulas are the reference kernels' text.
 """
 import ctypes
 import os

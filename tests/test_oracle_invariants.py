@@ -85,7 +85,7 @@ def test_zener_identity_relaxation_is_elastic():
 
 
 def test_zener_identity_relaxation_is_elastic_3d():
-    cfg = workloads.visco_3d_block(n=48, steps=30, n_incl=2, n_rec_side=2)
+    cfg = workloads.visco_3d_block(n=48, steps=30, n_incl=2, n_rec_side=2, t0=0.3)
     s = Simulator3D(cfg)
     tau = AttenuationCoefficients().tau_table()
     ident = AttenuationCoefficients(tau=(tau[1], tau[1], tau[3], tau[3]))
@@ -103,7 +103,7 @@ def test_fp32_vs_fp64_floor(kind):
         cfg = workloads.visco_2d_array(n=96, steps=200, n_elem=8, n_voids=2)
         s, att, steps = Simulator(cfg), attenuation_from_config(cfg), 200
     else:
-        cfg = workloads.visco_3d_block(n=48, steps=60, n_incl=2, n_rec_side=3)
+        cfg = workloads.visco_3d_block(n=48, steps=60, n_incl=2, n_rec_side=3, t0=0.3)
         s, att, steps = Simulator3D(cfg), attenuation_from_config(cfg), 60
     a = oracle.Oracle(s, precision="f32", att=att).run(steps)
     b = oracle.Oracle(s, precision="f64", att=att).run(steps)

@@ -216,6 +216,10 @@ def main():
     ap.add_argument("--shots", type=int, default=16, help="array elements = FMC shots per specimen (--workload shots)")
     ap.add_argument("--group-sweep", action="store_true",
                     help="also time the batch at cache-group budgets 0..320 MB (--workload shots)")
+    ap.add_argument("--extras", type=int, default=1,
+                    help="3D line: also measure batched FMC shots/s and the adjoint/forward ratio (0: skip)")
+    ap.add_argument("--shots-window", type=int, default=240,
+                    help="steps of each shot timed for the 3D line's shots/s (batch and per-shot loop)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 5 launches): rocprof counter calibration")
     ap.add_argument("--bandwidth-probe", action="store_true",
@@ -348,9 +352,25 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else round(traffic),
+                     "traffic_source": None if traffic is None else
+                     "rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch shape, profiles/visco3d_512cube_kernels.md",
                      "kernel": "stress3d", "bytes_per_cell": round(b_stress, 2)},
         "cpu_baseline": cpu,
     }
+    if args.extras:
+        # the rest of the metric: batched shots/s and the gradient's cost, each on its
+        # own configuration, measured after the 3D handle is released
+        s = measure_shots((512, 1024, 1536, 2048), 16, args.shots_window, profile_steps=0)
+        line["shots_per_s"] = {"batched": s["value"], "per_shot_loop": s["per_shot_loop_shots_per_s"],
+                               "speedup": s["batch_speedup"], "shots": s["config"]["shots"],
+                               "window_steps": s["config"]["window_steps"], "shot_steps": 3000,
+                               "cache_groups": s["config"]["cache_groups"], "workload": s["config"]["workload"]}
+        a = measure_adjoint(1)
+        line["adjoint_over_forward"] = {"value": a["value"], "forward_ms": a["forward_ms_per_shot"],
+                                        "checkpointed_forward_ms": a["checkpointed_forward_ms_per_shot"],
+                                        "recompute_adjoint_ms": a["recompute_adjoint_ms_per_shot"],
+                                        "segment": a["config"]["segment"], "shots": 1, "steps": a["steps"],
+                                        "workload": a["config"]["workload"]}
     emit(line)
 
 
@@ -381,22 +401,28 @@ def bandwidth_probe():
 
 def bench_shots(args):
     """Batched multi-shot 2D throughput vs the per-shot loop (see module doc)."""
+    sizes = tuple(int(v) for v in args.sizes.split(","))
+    emit(measure_shots(sizes, args.shots, args.steps, args.warmup, args.profile_steps, args.group_sweep))
+
+
+def measure_shots(sizes, n_shots, steps, warmup=0, profile_steps=10, group_sweep=False):
+    """FMC on plates of the given sizes x n_shots: every shot as one batch
+    against the same shots one after another, over a window of `steps` steps of
+    each shot; shots/s is for 3000-step shots (window time x 3000 / steps)."""
     from taichi_sim_amd import workloads
     from taichi_sim_amd.backend import attenuation_from_config
     from taichi_sim_amd.engine import VSEBatch, VSESim
     from taichi_sim_amd.sim_support.simulator import Simulator
 
     shot_len = 3000                                   # steps of one shot
-    sizes = tuple(int(v) for v in args.sizes.split(","))
-    steps = args.steps
     t_setup = time.perf_counter()
     sims, cells = [], 0
-    for cfg, laws in workloads.fmc_2d(sizes=sizes, n_elem=args.shots, steps=shot_len,
+    for cfg, laws in workloads.fmc_2d(sizes=sizes, n_elem=n_shots, steps=shot_len,
                                       seeds=tuple(40 + i for i in range(len(sizes)))):
         sim = Simulator(cfg)
         sim._emission_laws = laws
         att = attenuation_from_config(cfg)
-        for law in range(args.shots):
+        for law in range(n_shots):
             sim.select_law(law)
             sims.append(VSESim.from_simulator(sim, variant="wgsl", att=att))
             cells += int(np.prod(sims[-1].shape))
@@ -406,11 +432,13 @@ def bench_shots(args):
 
     batch = VSEBatch(sims)
     groups = batch.groups()
-    k_batch = batch.profile(max(1, args.profile_steps))        # all members in one launch per pass
-    k_solo = [0.0, 0.0, 0.0]
-    for g in sims:
-        t = g.profile(max(1, args.profile_steps))
-        k_solo = [a + b for a, b in zip(k_solo, t)]
+    k_batch = k_solo = None
+    if profile_steps > 0:
+        k_batch = batch.profile(profile_steps)        # all members in one launch per pass
+        k_solo = [0.0, 0.0, 0.0]
+        for g in sims:
+            t = g.profile(profile_steps)
+            k_solo = [a + b for a, b in zip(k_solo, t)]
 
     def timed(fn):
         # both paths run hipGraphs of min(steps, 256) steps; an untimed call of
@@ -431,7 +459,7 @@ def bench_shots(args):
     wall_l, _ = timed(lambda n: [g.run(n, check_every=check) for g in sims])
     print(f"# per-shot loop: {wall_l:.3f} s", file=sys.stderr, flush=True)
     sweep = None
-    if args.group_sweep:
+    if group_sweep:
         # batch Gcell/s at each cache-group budget (MB of member device memory per group)
         sweep, n_sw = {}, min(steps, 200)
         for mb in (0, 32, 64, 96, 128, 160, 192, 224, 256, 320):
@@ -451,37 +479,45 @@ def bench_shots(args):
         g.close()
 
     shots = len(sims) * steps / shot_len
-    line = {
+    ms = lambda k: None if k is None else {n: round(v, 4) for n, v in zip(("stress", "velocity", "sparse"), k)}
+    return {
         "metric": "shots/s (2D viscoelastic Zener L=3 FP32 FMC, 3000-step shots, batched)",
         "value": round(shots / wall_b, 4), "unit": "shots/s", "n_gpus": 1,
-        "steps": steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall_b / steps, 4),
+        "steps": steps, "warmup": warmup, "ms_per_step": round(1e3 * wall_b / steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"FMC: {len(sizes)} steel plates with voids {'/'.join(f'{n}^2' for n in sizes)} "
-                               f"(incl. CPML) x {args.shots} shots ({args.shots}-element array, all receiving)",
-                   "shots": len(sims), "cells_per_step": cells, "shot_steps": shot_len,
+                               f"(incl. CPML) x {n_shots} shots ({n_shots}-element array, all receiving)",
+                   "shots": len(sims), "cells_per_step": cells, "shot_steps": shot_len, "window_steps": steps,
                    "parallelism": "single GPU, one batch", "cache_groups": len(groups)},
         "per_shot_loop_shots_per_s": round(shots / wall_l, 4),
         "group_sweep_gcell_per_s": sweep,
         "batch_speedup": round(wall_l / wall_b, 3),
         "gcell_per_s_batched": round(cells * steps / wall_b / 1e9, 3),
         "gcell_per_s_loop": round(cells * steps / wall_l / 1e9, 3),
-        "kernel_ms_all_in_one_launch": {k: round(v, 4) for k, v in zip(("stress", "velocity", "sparse"), k_batch)},
-        "kernel_ms_loop_sum": {k: round(v, 4) for k, v in zip(("stress", "velocity", "sparse"), k_solo)},
+        "kernel_ms_all_in_one_launch": ms(k_batch),
+        "kernel_ms_loop_sum": ms(k_solo),
         "device": name,
     }
-    emit(line)
 
 
 def bench_adjoint(args):
     """Adjoint/forward time ratio of the C-ABI gradient (see module doc)."""
+    emit(measure_adjoint(args.adj_shots, args.fd_dirs, args.store_all))
+
+
+def measure_adjoint(n_shots, fd_dirs=0, store_all=False):
+    """Gradient of the trace misfit on the 1024^2 inversion problem: device
+    time of the checkpointed forward + recompute + adjoint over the plain
+    forward, per shot; optional finite-difference directions and the
+    every-level-stored variant."""
     from taichi_sim_amd import workloads
     from taichi_sim_amd.adjoint import VSEGrad, chain_to_cells
     from taichi_sim_amd.backend import attenuation_from_config
     from taichi_sim_amd.engine import VSESim
     from taichi_sim_amd.sim_support.simulator import Simulator
 
-    n, steps, n_shots = 1024, 3000, args.adj_shots
+    n, steps = 1024, 3000
     cfg, laws, true = workloads.adjoint_2d(n=n, steps=steps, n_shots=n_shots)
     sim = Simulator(cfg)
     sim._emission_laws = laws
@@ -489,7 +525,8 @@ def bench_adjoint(args):
     g = VSESim.from_simulator(sim, variant="wgsl", visco=True, att=att)
     start = [np.asarray(a, np.float32).copy() for a in (sim._rho_grid_vx, sim._cp_grid_vx, sim._cs_grid_vx)]
     eng = VSEGrad(g)
-    print(f"# setup: grid {g.shape}, segment {eng.seg_len} steps, {eng.device_bytes() / 2**30:.1f} GiB "
+    seg = eng.seg_len
+    print(f"# setup: grid {g.shape}, segment {seg} steps, {eng.device_bytes() / 2**30:.1f} GiB "
           f"of checkpoints/adjoint state", file=sys.stderr, flush=True)
     ncell = int(np.prod(g.shape))
 
@@ -539,13 +576,15 @@ def bench_adjoint(args):
     # FP32 forward resolves.  The difference quotient uses forward runs only;
     # random directions at 128^2 are checked in the test suite.
     zero = np.zeros_like(obs[0])
-    sim.select_law(0)
-    g.load_sources(sim)
-    eng.set_model(*start)
-    eng.forward(steps, check_every=500)
-    energy = eng.misfit(zero)
-    e_coef = eng.backward()
-    e_rho, e_cp, e_cs = (t.numpy() for t in chain_to_cells(e_coef, *start))
+    energy = None
+    if fd_dirs > 0:
+        sim.select_law(0)
+        g.load_sources(sim)
+        eng.set_model(*start)
+        eng.forward(steps, check_every=500)
+        energy = eng.misfit(zero)
+        e_coef = eng.backward()
+        e_rho, e_cp, e_cs = (t.numpy() for t in chain_to_cells(e_coef, *start))
 
     def misfit_all(model):
         eng.set_model(*model)
@@ -553,11 +592,11 @@ def bench_adjoint(args):
         return eng.misfit(zero)
 
     fd_rel = []
-    if args.fd_dirs > 0:
+    if fd_dirs > 0:
         rng = np.random.default_rng(5)
         y_top = int(np.min(np.nonzero(np.asarray(sim._pos_sources) != -1)[1]))
         X, Y = np.meshgrid(np.arange(g.shape[0]), np.arange(g.shape[1]), indexing="ij")
-        for d in range(args.fd_dirs):
+        for d in range(fd_dirs):
             w = np.zeros(g.shape)
             for _ in range(4):
                 cx, cy = rng.uniform(0.2, 0.8) * g.shape[0], rng.uniform(y_top + 2, y_top + 40)
@@ -577,8 +616,8 @@ def bench_adjoint(args):
 
     # the same gradient with every state kept (segment = N levels, no
     # recompute): 3000 levels of 1024^2 take ~180 GB, which one MI355X holds
-    store_all = None
-    if args.store_all:
+    stored = None
+    if store_all:
         sim.select_law(0)
         g.load_sources(sim)
         ea = VSEGrad(g, seg_len=steps)
@@ -589,34 +628,33 @@ def bench_adjoint(args):
             ea.misfit(obs[0])
             c_all = ea.backward()
             ta.append((ea.ms["forward"], ea.ms["backward"]))
-        store_all = {"segment": ea.seg_len, "device_gib": round(ea.device_bytes() / 2**30, 1),
-                     "forward_ms": round(ta[-1][0], 2), "adjoint_ms": round(ta[-1][1], 2),
-                     "same_gradient_bits": all(np.array_equal(c_all[key], coef0[key]) for key in coef0)}
+        stored = {"segment": ea.seg_len, "device_gib": round(ea.device_bytes() / 2**30, 1),
+                  "forward_ms": round(ta[-1][0], 2), "adjoint_ms": round(ta[-1][1], 2),
+                  "same_gradient_bits": all(np.array_equal(c_all[key], coef0[key]) for key in coef0)}
         ea.close()
-        print(f"# every state stored: {store_all}", file=sys.stderr, flush=True)
+        print(f"# every state stored: {stored}", file=sys.stderr, flush=True)
     g.close()
 
     fwd, gf, bw = (float(np.mean(t)) for t in (t_fwd, t_gfwd, t_bwd))
     ratio = (gf + bw) / fwd
-    if store_all is not None:
-        store_all["ratio"] = round((store_all["forward_ms"] + store_all["adjoint_ms"]) / fwd, 3)
-    line = {
+    if stored is not None:
+        stored["ratio"] = round((stored["forward_ms"] + stored["adjoint_ms"]) / fwd, 3)
+    return {
         "metric": "adjoint/forward time (2D viscoelastic 1024^2 L=3, 3000 steps, sqrt(N) checkpoints)",
         "value": round(ratio, 3), "unit": "x", "n_gpus": 1, "steps": steps, "warmup": 0,
         "ms_per_step": round((gf + bw) / steps, 4), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 (FP64 misfit and gradient accumulators)", "data": "synthetic",
         "config": {"workload": f"gradient of the L2 trace misfit, {n_shots} FMC shots, true model with 8 Gaussian "
                                f"velocity anomalies (+-5%), homogeneous start", "grid": [n, n], "n_sls": 3,
-                   "segment": eng.seg_len, "parallelism": "single GPU"},
+                   "segment": seg, "parallelism": "single GPU"},
         "forward_ms_per_shot": round(fwd, 2), "checkpointed_forward_ms_per_shot": round(gf, 2),
         "recompute_adjoint_ms_per_shot": round(bw, 2), "misfit": J, "trace_energy_shot0": energy,
         "gradient_norms": {"rho": float(np.linalg.norm(g_rho)), "cp": float(np.linalg.norm(g_cp)),
                            "cs": float(np.linalg.norm(g_cs)), "tau": float(np.linalg.norm(grad["tau"]))},
         "gcell_per_s_forward": round(ncell * steps / fwd / 1e6, 3),
         "fd_rel_err_trace_energy": [float(f"{e:.2e}") for e in fd_rel],
-        "every_state_stored": store_all,
+        "every_state_stored": stored,
     }
-    emit(line)
 
 
 if __name__ == "__main__":

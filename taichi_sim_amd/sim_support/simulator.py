@@ -281,7 +281,8 @@ class Simulator:
         ``implementation_many()`` (a backend may simulate them together); the
         results, their order and the saved files are the same as with 1."""
         now = datetime.now()
-        sim_times, total_times, results = [], [], []
+        sim_times, total_times, results, mse_values = [], [], [], []
+        n = 0
         for n in range(self._n_iter):
             print(f'Iteracao {n}')
             n_laws = self._emission_laws.shape[0] if hasattr(self, "_emission_laws") else 1
@@ -295,15 +296,61 @@ class Simulator:
                     print(f'\tLaw {law} of {n_laws}')
                     sim_times.append(res["sim_time"])
                     total_times.append(sim_total)
-                    self._report(res, n, law, now, sim_times[-1], total_times[-1])
+                    mse = self._report(res, n, law, now, sim_times[-1], total_times[-1])
+                    if mse is not None:
+                        mse_values.append(mse)
                     results.append(res)
-        self._sim_times = np.array(sim_times)
-        self._sim_total_times = np.array(total_times)
+        sim_times, total_times = np.array(sim_times), np.array(total_times)
+        mse_values = np.array(mse_values)
+        self._sim_times, self._sim_total_times, self._mse_values = sim_times, total_times, mse_values
         print(f'TEMPO - {self._n_steps} pontos de tempo')
+        if self._n_iter > 5:
+            print(f'Tempo medio de execucao: {sim_times[5:].mean():.3}s (std = {sim_times[5:].std():.4})')
+            print(f'Tempo medio total (inclui transferencia de dados): {total_times[5:].mean():.3}s '
+                  f'(std = {total_times[5:].std():.4})')
+        if self._n_iter > 5 and mse_values.shape[0] > 5:
+            print(f'MSE medio do campo de pressao: {mse_values[5:, 0].mean():.4} (std = {mse_values[5:, 0].std():.4})')
+            print(f'MSE medio dos sensores de pressao: {mse_values[5:, 1].mean():.4} '
+                  f'(std = {mse_values[5:, 1].std():.4})')
+        result_dir = getattr(self, "_results_dir", ".")
         if self._save_sources:
-            np.save(os.path.join(getattr(self, "_results_dir", "."),
-                                 f'sources_{self._name}_{now.strftime("%Y%m%d-%H%M%S")}'), self._source_term)
+            np.save(os.path.join(result_dir, f'sources_{self._name}_{now.strftime("%Y%m%d-%H%M%S")}'),
+                    self._source_term)
+        if self._save_results and results:
+            self._write_run_files(result_dir, now, n, sim_times, total_times, mse_values, results[-1])
         return results
+
+    def _write_run_files(self, result_dir, now, n, sim_times, total_times, mse_values, last):
+        """The per-run outputs of simulator.py:563-599: the loop times as
+        ``..._GPU_.csv`` and the run description ``..._desc.txt`` (the last
+        law's GPU string, energies and implementation message)."""
+        name = os.path.join(result_dir, f'result_{self._name}_{now.strftime("%Y%m%d-%H%M%S")}_'
+                                        f'{self._nx}x{self._ny}_{self._n_steps}_iter_{n}_')
+        np.savetxt(name + 'GPU_.csv', sim_times, '%10.3f', delimiter=',')
+        lines = ['Parametros do ensaio', '--------------------', '',
+                 f'Simulador: {self._name}', f'Modelo do simulador: {self._sim_model}',
+                 f'Quantidade de iteracoes no tempo: {self._n_steps}', f'Tamanho da ROI: {self._nx}x{self._ny}',
+                 f'GPU: {last["gpu_str"]}', f'Numero de simulacoes: {self._n_iter}',
+                 f"Energia refletida (sensor): {self._reflected_energy:.2}",
+                 f"Energia ultimo frame: {self._field_energy:.2}"]
+        text = "\n".join(lines) + "\n" + last.get("msg_impl", "")
+        if self._n_iter > 5:
+            text += (f'Tempo medio de execucao: {sim_times[5:].mean():.3}s\nDesvio padrao: {sim_times[5:].std():.4}\n'
+                     f'Tempo medio total (inclui transferencia de dados): {total_times[5:].mean():.3}s\n'
+                     f'Desvio padrao: {total_times[5:].std():.4}\n')
+            if mse_values.shape[0] > 5:
+                text += (f'MSE medio do campo de pressao: {mse_values[5:, 0].mean():.4}\n'
+                         f'Desvio padrao: {mse_values[5:, 0].std():.4}\n'
+                         f'MSE medio dos sensores de pressao: {mse_values[5:, 1].mean():.4}\n'
+                         f'Desvio padrao: {mse_values[5:, 1].std():.4}\n')
+        else:
+            text += (f'Tempo execucao: {sim_times[-1]:.3}s\n'
+                     f'Tempo total (inclui transferencia de dados): {total_times[-1]:.3}s\n')
+            if mse_values.shape[0] > 0:
+                text += (f'MSE do campo de pressao: {mse_values[-1, 0]:.4}\n'
+                         f'MSE medio dos sensores de pressao: {mse_values[-1, 1]:.4}\n')
+        with open(name + '_desc.txt', 'w') as f:
+            f.write(text)
 
     def select_law(self, law):
         """Apply emission law ``law`` to the probes and rebuild the source term
@@ -337,11 +384,33 @@ class Simulator:
         print(f"Energia refletida (sensor): {self._reflected_energy:.2}")
         self._field_energy = (field ** 2).sum() * self._dx * self._dy
         print(f"Energia ultimo frame: {self._field_energy:.2}")
+        mse = self._compare_with_reference(result_dir, field, traces)
         tag = "pressure" if self._sim_type == "acoustic" else "stress"
         if self._save_results and self._save_field:
             np.save(name + ('_field_pressure' if tag == "pressure" else '_field_stress_yy'), field)
         if self._save_bscan:
             np.save(name + f'_bscan_{tag}', traces)
+        return mse
+
+    def _compare_with_reference(self, result_dir, field, traces):
+        """MSE of the ROI window of the last field and of the B-scan against
+        ``result_ref_{sim_model}_field_pressure.npy`` / ``..._bscan_pressure.npy``
+        in the results folder (simulator.py:358-393): inf for a shape
+        mismatch, None (and a message) when a reference file is missing."""
+        roi = self._roi
+        window = (slice(roi.get_ix_min(), roi.get_ix_max()), slice(roi.get_iz_min(), roi.get_iz_max()))
+        try:
+            ref = np.load(os.path.join(result_dir, f"result_ref_{self._sim_model}_field_pressure.npy"))[window]
+            mine = field[window]
+            mse_field = np.mean((ref - mine) ** 2) if ref.shape == mine.shape else np.inf
+            ref_traces = np.load(os.path.join(result_dir, f"result_ref_{self._sim_model}_bscan_pressure.npy"))
+            mse_traces = np.mean((ref_traces - traces) ** 2) if ref_traces.shape == traces.shape else np.inf
+        except FileNotFoundError as err:
+            print(f"Arquivo {err} nao encontrado. Nao pode ser feita a comparacao com a referencia.")
+            return None
+        print(f"MSE do campo de pressao em relacao a referencia: {mse_field:.4}")
+        print(f"MSE dos sensores de pressao em relacao a referencia: {mse_traces:.4}")
+        return [mse_field, mse_traces]
 
     def implementation(self):
         """Backend hook: returns {"stress"/"pressure", "sens_stress"/"sens_pressure",

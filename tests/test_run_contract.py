@@ -6,7 +6,9 @@ implementation() and reports/saves its result.  A stub backend whose result
 is a function of the current source term checks the order of calls, that
 each law reaches the source term, the result-dict contract, the saved file
 names, and that handing laws over in groups (implementation_many) changes
-nothing.
+nothing.  The end-of-run outputs follow simulator.py:358-599: the MSE pair
+against ``result_ref_*`` files (ROI window, inf on a shape mismatch, a
+message when a file is missing), the timing CSV and the description text.
 """
 import os
 import zlib
@@ -84,3 +86,102 @@ def test_without_laws_one_call_per_iteration(tmp_path):
     sim = StubBackend(cfg)
     res = sim.run(laws_per_call=4)
     assert len(res) == 2 and len(sim.calls) == 2
+
+
+def _ref_files(tmp_path, sim, field, traces):
+    np.save(os.path.join(tmp_path, f"result_ref_{sim._sim_model}_field_pressure.npy"), field)
+    np.save(os.path.join(tmp_path, f"result_ref_{sim._sim_model}_bscan_pressure.npy"), traces)
+
+
+def test_mse_against_reference_files(tmp_path, capsys):
+    cfg, laws = _cfg(tmp_path, False)
+    sim = StubBackend(cfg)
+    sim._emission_laws = laws
+    # reference files: the law-0 result, with one ROI cell and one trace sample changed
+    sim.select_law(0)
+    r0 = sim.implementation()
+    field, traces = r0["stress"].copy(), r0["sens_stress"].copy()
+    roi = sim._roi
+    ix, iz = roi.get_ix_min(), roi.get_iz_min()
+    field[ix + 3, iz + 2] += 2.0
+    field[0, 0] += 1e6               # outside the ROI window: not compared
+    traces[7, 1] -= 3.0
+    _ref_files(tmp_path, sim, field, traces)
+    sim.calls.clear()
+    sim.run()
+    mse = sim._mse_values
+    assert mse.shape == (10, 2)
+    n_win = (roi.get_ix_max() - ix) * (roi.get_iz_max() - iz)
+    assert ix > 0 and mse[0, 0] == pytest.approx(4.0 / n_win, rel=1e-6)
+    assert mse[0, 1] == pytest.approx(9.0 / traces.size, rel=1e-6)
+    assert mse[5, 0] == mse[0, 0]                   # law 0 of iteration 1
+    assert mse[5, 1] == mse[0, 1]
+    assert np.all(mse[1:5, 1] > mse[0, 1])          # other laws' traces differ from law 0's
+    out = capsys.readouterr().out
+    assert "MSE do campo de pressao em relacao a referencia" in out
+
+
+def test_mse_shape_mismatch_is_inf_and_missing_file_is_reported(tmp_path, capsys):
+    cfg, laws = _cfg(tmp_path, False)
+    sim = StubBackend(cfg)
+    _ref_files(tmp_path, sim, np.zeros((8, 8), np.float32), np.zeros((3, 3), np.float32))
+    sim.run()
+    assert sim._mse_values.shape == (2, 2) and np.all(np.isinf(sim._mse_values))
+    os.remove(os.path.join(tmp_path, f"result_ref_{sim._sim_model}_bscan_pressure.npy"))
+    capsys.readouterr()
+    sim.run()
+    assert sim._mse_values.shape == (0,)
+    out = capsys.readouterr().out
+    assert "nao encontrado. Nao pode ser feita a comparacao com a referencia." in out
+
+
+@pytest.mark.parametrize("n_iter", [2, 7])
+def test_timing_csv_and_description(tmp_path, capsys, n_iter):
+    cfg, laws = _cfg(tmp_path, True)
+    cfg["simul_configs"]["n_iter"] = n_iter
+    sim = StubBackend(cfg)
+    sim._emission_laws = laws
+
+    times = iter(np.arange(1, 100) * 0.125)
+
+    impl = sim.implementation
+
+    def timed():
+        r = impl()
+        r["sim_time"] = float(next(times))
+        r["msg_impl"] = "Implementacao: stub\n"
+        return r
+    sim.implementation = timed
+    sim.select_law(0)
+    r0 = impl()
+    _ref_files(tmp_path, sim, r0["stress"], r0["sens_stress"])
+    sim.run(laws_per_call=2)
+    names = os.listdir(tmp_path)
+    csv = [n for n in names if n.endswith("GPU_.csv")]
+    desc = [n for n in names if n.endswith("__desc.txt")]
+    assert len(csv) == 1 and len(desc) == 1
+    assert f"_96x96_60_iter_{n_iter - 1}_GPU_.csv" in csv[0]
+    got = np.loadtxt(os.path.join(tmp_path, csv[0]), delimiter=",")
+    np.testing.assert_allclose(got, np.arange(1, 5 * n_iter + 1) * 0.125, atol=5e-4)
+    text = open(os.path.join(tmp_path, desc[0])).read().splitlines()
+    assert text[:4] == ["Parametros do ensaio", "--------------------", "", "Simulador: stub"]
+    assert text[4] == f"Modelo do simulador: {sim._sim_model}"
+    assert text[5:9] == ["Quantidade de iteracoes no tempo: 60", "Tamanho da ROI: 96x96", "GPU: stub",
+                         f"Numero de simulacoes: {n_iter}"]
+    assert text[9].startswith("Energia refletida (sensor): ") and text[10].startswith("Energia ultimo frame: ")
+    assert text[11] == "Implementacao: stub"
+    st = sim._sim_times
+    out = capsys.readouterr().out
+    if n_iter > 5:
+        assert text[12] == f"Tempo medio de execucao: {st[5:].mean():.3}s"
+        assert text[13] == f"Desvio padrao: {st[5:].std():.4}"
+        assert text[16].startswith("MSE medio do campo de pressao: ")
+        assert len(text) == 20
+        assert f"Tempo medio de execucao: {st[5:].mean():.3}s (std = {st[5:].std():.4})" in out
+        assert "MSE medio dos sensores de pressao" in out
+    else:
+        assert text[12] == f"Tempo execucao: {st[-1]:.3}s"
+        assert text[13].startswith("Tempo total (inclui transferencia de dados): ")
+        mse = sim._mse_values[-1]
+        assert text[14:] == [f"MSE do campo de pressao: {mse[0]:.4}", f"MSE medio dos sensores de pressao: {mse[1]:.4}"]
+        assert "Tempo medio de execucao" not in out

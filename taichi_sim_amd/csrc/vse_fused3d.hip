@@ -352,7 +352,7 @@ __global__ __launch_bounds__(BZ * TY) void k_fused3d(FusedArgs F, int xchunk, in
           float oxx[LC], oyy[LC], ozz[LC];
 #pragma unroll
           for (int l = 0; l < LC; l++) {
-            const unsigned li = (unsigned)l * (unsigned)A.ncell + i0;
+            const unsigned li = (unsigned)(l * A.rs) + i0;
             oxx[l] = R0[li]; oyy[l] = R1[li]; ozz[l] = R2[li];
           }
           float Sxx = 0.f, Syy = 0.f, Szz = 0.f;
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(BZ * TY) void k_fused3d(FusedArgs F, int xchunk, in
             const float nyy = ((oyy[l] + (dvx - div3) * A.zk.phi_s[l] - oyy[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
             const float nzz = ((ozz[l] + (dvy - div3) * A.zk.phi_s[l] - ozz[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
             Sxx += oxx[l] + nxx; Syy += oyy[l] + nyy; Szz += ozz[l] + nzz;
-            const unsigned li = (unsigned)l * (unsigned)A.ncell + i0;
+            const unsigned li = (unsigned)(l * A.rs) + i0;
             R0[li] = nxx; R1[li] = nyy; R2[li] = nzz;
           }
           const float l23 = lambda + 2.0f * mu / 3.0f;
@@ -445,14 +445,14 @@ __global__ __launch_bounds__(BZ * TY) void k_fused3d(FusedArgs F, int xchunk, in
         float *const R = (float *)rr;
         float o[LC];
 #pragma unroll
-        for (int l = 0; l < LC; l++) o[l] = R[(unsigned)l * (unsigned)A.ncell + i0];
+        for (int l = 0; l < LC; l++) o[l] = R[(unsigned)(l * A.rs) + i0];
         float S = 0.f;
         const float X = d1 + d2;
 #pragma unroll
         for (int l = 0; l < LC; l++) {
           const float n = (o[l] + X * A.zk.phi_s[l] - o[l] * A.zk.h_s[l]) * A.zk.m_s[l];
           S += o[l] + n;
-          R[(unsigned)l * (unsigned)A.ncell + i0] = n;
+          R[(unsigned)(l * A.rs) + i0] = n;
         }
         Osig[i0] = sig0 + (mu * (d1 + d2) + mu * 0.5f * S) * dt;
       }
@@ -463,13 +463,13 @@ __global__ __launch_bounds__(BZ * TY) void k_fused3d(FusedArgs F, int xchunk, in
 
 }  // namespace
 
-bool vse_fused3d_supported(long ncell, int L, int visco, int ord) {
-  return visco && L == LC && ord == ORD && ncell > 0 && ncell * 4L * LC < (1L << 32);
+bool vse_fused3d_supported(long ncell, long rs, int L, int visco, int ord) {
+  return visco && L == LC && ord == ORD && ncell > 0 && rs >= ncell && rs * 4L * LC < (1L << 32);
 }
 
 hipError_t vse_launch_fused3d(int math, const FusedArgs &F, const KTune &t, hipStream_t st) {
   const StepArgs &A = F.A;
-  if (!vse_fused3d_supported(A.ncell, A.L, 1, ORD)) return hipErrorInvalidValue;
+  if (!vse_fused3d_supported(A.ncell, A.rs, A.L, 1, ORD)) return hipErrorInvalidValue;
   const int ty = t.ty == 8 ? 8 : 4;
   const int nzb = (A.nz + BZ - 1) / BZ, nyb = (A.ny + ty - 1) / ty;
   const int xchunk = std::max(1, std::min(t.xchunk > 0 ? t.xchunk : 16, A.nx));

@@ -3,9 +3,10 @@
 //
 // Grid layout is the reference's: C order with the last axis fastest
 // (2D index y + x*ny, wgsl2d :18-22; 3D index z + y*nz + x*ny*nz, wgsl3d
-// :28-32).  Zener memory variables are stored as L contiguous planes per
-// component (plane l of component c at r[c] + l*ncell) instead of the
-// reference's (cells, L) interleave, so every load is a coalesced stream.
+// :28-32).  Zener memory variables are stored as L planes per component
+// (plane l of component c at r[c] + l*rs, rs = ncell plus a small skew)
+// instead of the reference's (cells, L) interleave, so every load is a
+// coalesced stream.
 // CPML memory variables (psi) are stored only over each axis's active band.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -69,6 +70,7 @@ struct StepArgs {
   VSE_G float *sxx, *syy, *szz, *sxy, *sxz, *syz;
   VSE_G float *psi[P_COUNT];
   VSE_G float *r[6];                 // r_xx, r_yy, r_zz, r_xy, r_xz, r_yz (3D); 2D: xx, yy, -, xy
+  long rs;                           // mechanism l of a component at r[c] + l * rs (rs >= ncell)
   const VSE_G float *rho, *cp, *cs;
   ZenerK zk;
   int L;
@@ -240,7 +242,7 @@ hipError_t vse_launch_record2d_batch(const BatchArgs &B, int i, hipStream_t st);
 hipError_t vse_launch_stress3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
 // true when the fused kernel covers this problem: 3D, viscoelastic with 3
 // mechanisms, FD order 2, and every array addressable with 32-bit byte offsets
-bool vse_fused3d_supported(long ncell, int L, int visco, int ord);
+bool vse_fused3d_supported(long ncell, long rs, int L, int visco, int ord);
 // t.ty in {4, 8}: the tile height; t.xchunk: x-planes per workgroup; t.remap: tile order
 hipError_t vse_launch_fused3d(int math, const FusedArgs &F, const KTune &t, hipStream_t st);
 hipError_t vse_launch_velocity3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);

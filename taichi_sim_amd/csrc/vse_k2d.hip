@@ -74,8 +74,8 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
         float oxx[LC], oyy[LC];
 #pragma unroll
         for (int l = 0; l < LC; l++) {
-          oxx[l] = A.r[0][l * A.ncell + id];
-          oyy[l] = A.r[1][l * A.ncell + id];
+          oxx[l] = A.r[0][l * A.rs + id];
+          oyy[l] = A.r[1][l * A.rs + id];
         }
         const float sxx0 = A.sxx[id], syy0 = A.syy[id];
         float sum_xx = 0.f, sum_yy = 0.f;
@@ -85,8 +85,8 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
           const float nyy = ((oyy[l] + 0.5f * (dvx - dvy) * A.zk.phi_s[l] - oyy[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
           sum_xx += oxx[l] + nxx;
           sum_yy += oyy[l] + nyy;
-          A.r[0][l * A.ncell + id] = nxx;
-          A.r[1][l * A.ncell + id] = nyy;
+          A.r[0][l * A.rs + id] = nxx;
+          A.r[1][l * A.rs + id] = nyy;
         }
         A.sxx[id] = sxx0 + (lp2m * dvx + lambda * dvy + 0.5f * lpm * sum_xx + mu * sum_yy) * dt;
         A.syy[id] = syy0 + (lambda * dvx + lp2m * dvy + 0.5f * lpm * sum_xx - mu * sum_yy) * dt;
@@ -94,7 +94,7 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
         const float lpm = lambda + mu;
         float sum_xx = 0.f, sum_yy = 0.f;
         for (int l = 0; l < A.L; l++) {
-          float *pxx = A.r[0] + l * A.ncell + id, *pyy = A.r[1] + l * A.ncell + id;
+          float *pxx = A.r[0] + l * A.rs + id, *pyy = A.r[1] + l * A.rs + id;
           const float oxx = *pxx, oyy = *pyy;
           const float nxx = ((oxx + (dvx + dvy) * A.zk.phi_p[l] - oxx * A.zk.h_p[l]) * A.zk.m_p[l]);
           const float nyy = ((oyy + 0.5f * (dvx - dvy) * A.zk.phi_s[l] - oyy * A.zk.h_s[l]) * A.zk.m_s[l]);
@@ -134,20 +134,20 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
     if constexpr (VISCO && VAR == VSE_VAR_WGSL && LC > 0) {
       float o[LC];
 #pragma unroll
-      for (int l = 0; l < LC; l++) o[l] = A.r[3][l * A.ncell + id];
+      for (int l = 0; l < LC; l++) o[l] = A.r[3][l * A.rs + id];
       const float sxy0 = A.sxy[id];
       float sum_xy = 0.f;
 #pragma unroll
       for (int l = 0; l < LC; l++) {
         const float n = ((o[l] + (dvydx + dvxdy) * A.zk.phi_s[l] - o[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
         sum_xy += o[l] + n;
-        A.r[3][l * A.ncell + id] = n;
+        A.r[3][l * A.rs + id] = n;
       }
       A.sxy[id] = sxy0 + (mu * (dvydx + dvxdy) + 0.5f * mu * sum_xy) * dt;
     } else if constexpr (VISCO && VAR == VSE_VAR_WGSL) {
       float sum_xy = 0.f;
       for (int l = 0; l < A.L; l++) {
-        float *pxy = A.r[3] + l * A.ncell + id;
+        float *pxy = A.r[3] + l * A.rs + id;
         const float o = *pxy;
         const float n = ((o + (dvydx + dvxdy) * A.zk.phi_s[l] - o * A.zk.h_s[l]) * A.zk.m_s[l]);
         sum_xy += o + n;

@@ -197,7 +197,7 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
         float oxx[LC], oyy[LC], ozz[LC];
 #pragma unroll
         for (int l = 0; l < LC; l++) {
-          const long lo = (long)l * A.ncell + rofs;
+          const long lo = (long)l * A.rs + rofs;
           oxx[l] = PB(A.r[0] + lo, pbytes).ld<CP>(col);
           oyy[l] = PB(A.r[1] + lo, pbytes).ld<CP>(col);
           ozz[l] = PB(A.r[2] + lo, pbytes).ld<CP>(col);
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
           const float nyy = ((oyy[l] + (dvx - div3) * A.zk.phi_s[l] - oyy[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
           const float nzz = ((ozz[l] + (dvy - div3) * A.zk.phi_s[l] - ozz[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
           Sxx += oxx[l] + nxx; Syy += oyy[l] + nyy; Szz += ozz[l] + nzz;
-          const long lo = (long)l * A.ncell + rofs;
+          const long lo = (long)l * A.rs + rofs;
           PB(A.r[0] + lo, pbytes).st<CP>(col, nxx);
           PB(A.r[1] + lo, pbytes).st<CP>(col, nyy);
           PB(A.r[2] + lo, pbytes).st<CP>(col, nzz);
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
         const float div = dvx + dvy + dvz;
         const float div3 = FAST ? div * (1.0f / 3.0f) : div / 3.0f;
         for (int l = 0; l < A.L; l++) {
-          const long lo = (long)l * A.ncell + rofs;
+          const long lo = (long)l * A.rs + rofs;
           const PB pxx(A.r[0] + lo, pbytes), pyy(A.r[1] + lo, pbytes), pzz(A.r[2] + lo, pbytes);
           const float oxx = pxx.ld<CP>(col), oyy = pyy.ld<CP>(col), ozz = pzz.ld<CP>(col);
           const float nxx = ((oxx + div * A.zk.phi_p[l] - oxx * A.zk.h_p[l]) * A.zk.m_p[l]);
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
       if constexpr (VISCO && LC > 0) {
         float o[LC];
 #pragma unroll
-        for (int l = 0; l < LC; l++) o[l] = PB(rr + (long)l * A.ncell + rofs, pbytes).ld<CP>(col);
+        for (int l = 0; l < LC; l++) o[l] = PB(rr + (long)l * A.rs + rofs, pbytes).ld<CP>(col);
         const float sig0 = sigP.ld<CP>(col);
         float S = 0.f;
         const float X = d1 + d2;
@@ -323,14 +323,14 @@ __global__ __launch_bounds__(BZ * TY) void k_stress3d(StepArgs A, int xchunk, in
         for (int l = 0; l < LC; l++) {
           const float n = (o[l] + X * A.zk.phi_s[l] - o[l] * A.zk.h_s[l]) * A.zk.m_s[l];
           S += o[l] + n;
-          PB(rr + (long)l * A.ncell + rofs, pbytes).st<CP>(col, n);
+          PB(rr + (long)l * A.rs + rofs, pbytes).st<CP>(col, n);
         }
         sigP.st<CP>(col, sig0 + (mu * (d1 + d2) + mu * 0.5f * S) * dt);
       } else if constexpr (VISCO) {
         float S = 0.f;
         const float X = d1 + d2;
         for (int l = 0; l < A.L; l++) {
-          const PB pr(rr + (long)l * A.ncell + rofs, pbytes);
+          const PB pr(rr + (long)l * A.rs + rofs, pbytes);
           const float o = pr.ld<CP>(col);
           const float n = (o + X * A.zk.phi_s[l] - o * A.zk.h_s[l]) * A.zk.m_s[l];
           S += o + n;
@@ -529,9 +529,9 @@ __global__ __launch_bounds__(BZ * BYY) void k_stress3d_plain(StepArgs A) {
       float oxx[LC], oyy[LC], ozz[LC];
 #pragma unroll
       for (int l = 0; l < LC; l++) {
-        oxx[l] = A.r[0][l * A.ncell + id];
-        oyy[l] = A.r[1][l * A.ncell + id];
-        ozz[l] = A.r[2][l * A.ncell + id];
+        oxx[l] = A.r[0][l * A.rs + id];
+        oyy[l] = A.r[1][l * A.rs + id];
+        ozz[l] = A.r[2][l * A.rs + id];
       }
       const float sxx0 = A.sxx[id], syy0 = A.syy[id], szz0 = A.szz[id];
       float Sxx = 0.f, Syy = 0.f, Szz = 0.f;
@@ -543,9 +543,9 @@ __global__ __launch_bounds__(BZ * BYY) void k_stress3d_plain(StepArgs A) {
         const float nyy = ((oyy[l] + (dvx - div3) * A.zk.phi_s[l] - oyy[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
         const float nzz = ((ozz[l] + (dvy - div3) * A.zk.phi_s[l] - ozz[l] * A.zk.h_s[l]) * A.zk.m_s[l]);
         Sxx += oxx[l] + nxx; Syy += oyy[l] + nyy; Szz += ozz[l] + nzz;
-        A.r[0][l * A.ncell + id] = nxx;
-        A.r[1][l * A.ncell + id] = nyy;
-        A.r[2][l * A.ncell + id] = nzz;
+        A.r[0][l * A.rs + id] = nxx;
+        A.r[1][l * A.rs + id] = nyy;
+        A.r[2][l * A.rs + id] = nzz;
       }
       const float l23 = lambda + 2.0f * mu / 3.0f;
       A.sxx[id] = sxx0 + (lp2m * dvx + lambda * (dvy + dvz) + l23 * 0.5f * Sxx + mu * Syy) * dt;
@@ -556,7 +556,7 @@ __global__ __launch_bounds__(BZ * BYY) void k_stress3d_plain(StepArgs A) {
       const float div = dvx + dvy + dvz;
       const float div3 = FAST ? div * (1.0f / 3.0f) : div / 3.0f;
       for (int l = 0; l < A.L; l++) {
-        float *pxx = A.r[0] + l * A.ncell + id, *pyy = A.r[1] + l * A.ncell + id, *pzz = A.r[2] + l * A.ncell + id;
+        float *pxx = A.r[0] + l * A.rs + id, *pyy = A.r[1] + l * A.rs + id, *pzz = A.r[2] + l * A.rs + id;
         const float oxx = *pxx, oyy = *pyy, ozz = *pzz;
         const float nxx = ((oxx + div * A.zk.phi_p[l] - oxx * A.zk.h_p[l]) * A.zk.m_p[l]);
         const float nyy = ((oyy + (dvx - div3) * A.zk.phi_s[l] - oyy * A.zk.h_s[l]) * A.zk.m_s[l]);
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(BZ * BYY) void k_stress3d_plain(StepArgs A) {
     if constexpr (VISCO && LC > 0) {
       float o[LC];
 #pragma unroll
-      for (int l = 0; l < LC; l++) o[l] = rr[l * A.ncell + id];
+      for (int l = 0; l < LC; l++) o[l] = rr[l * A.rs + id];
       const float sig0 = sig[id];
       float S = 0.f;
       const float X = d1 + d2;
@@ -632,14 +632,14 @@ __global__ __launch_bounds__(BZ * BYY) void k_stress3d_plain(StepArgs A) {
       for (int l = 0; l < LC; l++) {
         const float n = (o[l] + X * A.zk.phi_s[l] - o[l] * A.zk.h_s[l]) * A.zk.m_s[l];
         S += o[l] + n;
-        rr[l * A.ncell + id] = n;
+        rr[l * A.rs + id] = n;
       }
       sig[id] = sig0 + (mu * (d1 + d2) + mu * 0.5f * S) * dt;
     } else if constexpr (VISCO) {
       float S = 0.f;
       const float X = d1 + d2;
       for (int l = 0; l < A.L; l++) {
-        float *pr = rr + l * A.ncell + id;
+        float *pr = rr + l * A.rs + id;
         const float o = *pr;
         const float n = (o + X * A.zk.phi_s[l] - o * A.zk.h_s[l]) * A.zk.m_s[l];
         S += o + n;

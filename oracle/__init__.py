@@ -22,7 +22,12 @@ Pinning (each function is checked by at least one of these in tests/):
   with the half profile for D+ and the full one for D-), and the geometric
   closed form of the Zener memory variables under a uniform strain rate over
   many steps; a mistake planted in this oracle (a D+/D- swap, a coefficient
-  order, a half/full profile, an average, a Zener coefficient) fails them.
+  order, a half/full profile, an average, a Zener coefficient) fails them;
+* 2D WGSL, 2D Taichi and 3D, many steps: plane waves along each axis, every
+  FD order, against the closed form of the staggered leapfrog per Fourier
+  mode with the dispersion relation sin(w dt/2) = c dt K(k)/2
+  (tests/test_oracle_dispersion.py); a D+/D- swap or reversed coefficients
+  planted in this oracle fail them.
   No numeric output of the WGSL backends survives (they need wgpu), so the
   expected formulas are the reference kernels' text.
 """

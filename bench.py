@@ -370,6 +370,7 @@ def main():
                                         "checkpointed_forward_ms": a["checkpointed_forward_ms_per_shot"],
                                         "recompute_adjoint_ms": a["recompute_adjoint_ms_per_shot"],
                                         "segment": a["config"]["segment"], "shots": 1, "steps": a["steps"],
+                                        "gradient_accumulation": a["gradient_accumulation"],
                                         "workload": a["config"]["workload"]}
     emit(line)
 
@@ -563,6 +564,22 @@ def measure_adjoint(n_shots, fd_dirs=0, store_all=False):
               f"recompute+adjoint {t_bwd[-1]:.1f} ms", file=sys.stderr, flush=True)
     g_rho, g_cp, g_cs = (t.numpy() for t in chain_to_cells(grad, *start))
 
+    # shot 0 again with the FP32 partials flushed into FP64 every step: how far
+    # the default 32-step grouping is from a per-step FP64 time accumulation
+    sim.select_law(0)
+    g.load_sources(sim)
+    eng.set_flush(1)
+    eng.set_model(*start)
+    eng.forward(steps, check_every=500)
+    eng.misfit(obs[0])
+    c_every = eng.backward()
+    eng.set_flush(32)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    accumulation = {"flush_steps": 32, "every_step_recompute_adjoint_ms": round(eng.ms["backward"], 2),
+                    "rel_l2_vs_every_step_fp64": {key: float(f"{rel(coef0[key], c_every[key]):.2e}")
+                                                  for key in coef0}}
+    print(f"# FP32 partials vs per-step FP64 accumulation: {accumulation}", file=sys.stderr, flush=True)
+
     # finite-difference spot check at full size on shot 0.  The misfit of the
     # inversion is tiny at the starting model (smooth anomalies reflect
     # weakly), where a central difference measures its cubic term; the check
@@ -653,6 +670,7 @@ def measure_adjoint(n_shots, fd_dirs=0, store_all=False):
                            "cs": float(np.linalg.norm(g_cs)), "tau": float(np.linalg.norm(grad["tau"]))},
         "gcell_per_s_forward": round(ncell * steps / fwd / 1e6, 3),
         "fd_rel_err_trace_energy": [float(f"{e:.2e}") for e in fd_rel],
+        "gradient_accumulation": accumulation,
         "every_state_stored": stored,
     }
 

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define VSE_ABI_VERSION 2
+#define VSE_ABI_VERSION 3
 
 #define VSE_OK 0
 #define VSE_ERR_ARG (-1)        /* invalid argument / shape */
@@ -319,7 +319,10 @@ void vse_batch_destroy(vse_batch *b);
  * the hand-written adjoint kernels -- accumulating the gradient in FP64.
  * grad_traces non-NULL supplies the adjoint source (d misfit / d traces)
  * directly.  Wavefields are FP32; only the misfit and the gradient
- * accumulators are FP64. */
+ * accumulators are FP64.  Each step's gradient terms are formed in FP32 and
+ * added into FP32 per-cell partials, which go into the FP64 maps after every
+ * step n with n % flush_every == 0 (vse_grad_set_flush: default 32, 1 = an
+ * FP64 time accumulation of every step's terms, at one more kernel per step). */
 #define VSE_GRAD_LAMBDA 0
 #define VSE_GRAD_MU 1
 #define VSE_GRAD_MU_XY 2
@@ -332,6 +335,8 @@ int vse_grad_set_tau_scale(vse_grad *g, const float *scale);   /* (n cells), NUL
 int vse_grad_forward(vse_grad *g, int n_steps, int check_every, float *traces, float *elapsed_ms);
 int vse_grad_misfit(vse_grad *g, const float *observed, double *misfit);
 int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms);
+/* Steps per FP64 flush of the FP32 gradient partials, 1 .. 4096 (default 32). */
+int vse_grad_set_flush(vse_grad *g, int flush_every);
 /* FP64 gradient map VSE_GRAD_* (n cells, grid layout) of the last backward. */
 int vse_grad_read(vse_grad *g, int which, double *out);
 /* Field 0..4 = vx, vy, sxx, syy, sxy of the last forward's final state. */

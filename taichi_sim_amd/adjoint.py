@@ -93,6 +93,11 @@ class VSEGrad:
     def seg_len(self):
         return int(self._lib.vse_grad_segment(self._h))
 
+    def set_flush(self, every):
+        """Steps per FP64 flush of the FP32 per-step gradient partials (default
+        32; 1 accumulates every step's terms in FP64)."""
+        self._check(self._lib.vse_grad_set_flush(self._h, int(every)))
+
     def device_bytes(self):
         return int(self._lib.vse_grad_device_bytes(self._h))
 

@@ -34,8 +34,8 @@
 // and n+1 in FP32 (the differences with f32(C/d), see sumT).  Each step's
 // gradient terms are formed in FP32 and added to FP32 per-cell partials; k_grad_flush
 // adds the partials into the FP64 gradients after every step n divisible by
-// VSE_GRAD_FLUSH, so the time accumulation is FP64 at that granularity while
-// every step moves 4-byte partials instead of 8-byte totals.
+// the flush interval (32 by default), so the time accumulation is FP64 at that
+// granularity while every step moves 4-byte partials instead of 8-byte totals.
 #include "vse_device.h"
 
 namespace {

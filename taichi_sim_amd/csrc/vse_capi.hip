@@ -1490,6 +1490,7 @@ const char *vse_batch_last_error(vse_batch *b) { return b ? b->err.c_str() : "nu
 struct vse_grad {
   vse_sim *s = nullptr;
   int K = 0, n_ckpt = 0, n_fwd = 0;
+  int flush = VSE_GRAD_FLUSH;     // steps per FP64 flush of the FP32 partials
   int n_rec = 0;                  // receivers of s when the trace buffers were sized
   bool last_seg_stored = false;   // the forward left the last segment's levels in seg[]
   std::vector<DLevel> ckpt, seg;
@@ -1816,7 +1817,7 @@ int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms) 
       A.n = n;
       hipError_t e = vse_launch_adj_velocity(A, s->stream);
       if (e == hipSuccess) e = vse_launch_adj_stress(A, s->stream);
-      if (e == hipSuccess && n % VSE_GRAD_FLUSH == 0) e = vse_launch_grad_flush(A, s->stream);
+      if (e == hipSuccess && n % g->flush == 0) e = vse_launch_grad_flush(A, s->stream);
       rc = g->hip(e, "adjoint step");
     }
   }
@@ -1854,6 +1855,14 @@ int vse_grad_read_state(vse_grad *g, int field, float *out) {
   int rc = g->hip(hipMemcpyAsync(out, src[field], s->ncell * sizeof(float), hipMemcpyDeviceToHost, s->stream), "read state");
   if (!rc) rc = g->hip(hipStreamSynchronize(s->stream), "read state");
   return rc;
+}
+
+int vse_grad_set_flush(vse_grad *g, int flush_every) {
+  if (int rc = grad_usable(g)) return rc;
+  if (flush_every < 1 || flush_every > VSE_GRAD_FLUSH_MAX)
+    return g->fail(VSE_ERR_ARG, "flush interval must be 1 .. " + std::to_string(VSE_GRAD_FLUSH_MAX));
+  g->flush = flush_every;
+  return 0;
 }
 
 int vse_grad_segment(vse_grad *g) { return g ? g->K : 0; }

@@ -179,9 +179,10 @@ struct DLevel {
 enum { VSE_GR_LAMBDA = 0, VSE_GR_MU = 1, VSE_GR_MU_XY = 2, VSE_GR_RHO_VX = 3, VSE_GR_RHO_VY = 4,
        VSE_GR_TAU = 5, VSE_GR_COUNT = 6 };
 // The reverse sweep adds the FP32 partials into the FP64 gradients after every
-// step n with n % VSE_GRAD_FLUSH == 0, so the grouping depends on n only (not
-// on the checkpoint spacing).
+// step n with n % flush == 0, so the grouping depends on n only (not on the
+// checkpoint spacing).  VSE_GRAD_FLUSH is the default flush interval.
 constexpr int VSE_GRAD_FLUSH = 32;
+constexpr int VSE_GRAD_FLUSH_MAX = 4096;
 
 struct DiffArgs {
   int nx, ny, ord;

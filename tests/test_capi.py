@@ -34,7 +34,7 @@ def test_library_runs_on_the_rocm_runtime_it_was_built_with():
 
 def test_abi_version_and_arg_validation_without_gpu():
     lib = _lib.load()
-    assert lib.vse_abi_version() == _lib.ABI_VERSION == 2
+    assert lib.vse_abi_version() == _lib.ABI_VERSION == 3
     # argument validation happens before any device call
     d = _lib.VseDesc()
     d.ndim = 4

@@ -8,6 +8,8 @@
   directions (all four maps perturbed together) agree with the adjoint's
   directional derivative to 1e-3.
 * The checkpoint spacing changes nothing (bit-identical gradients).
+* The FP32 partials flushed into FP64 every 32 steps stay within 1e-6 of a
+  per-step FP64 time accumulation.
 """
 import numpy as np
 import pytest
@@ -16,6 +18,7 @@ import torch
 import oracle
 from oracle.visco2d_torch import Visco2D
 from taichi_sim_amd import workloads
+from taichi_sim_amd._lib import VSEError
 from taichi_sim_amd.adjoint import GRAD_NAMES, VSEGrad, ViscoTraces, chain_to_cells, misfit_and_gradient
 from taichi_sim_amd.backend import attenuation_from_config
 from taichi_sim_amd.engine import VSESim
@@ -150,6 +153,48 @@ def test_gradient_vs_fp64_autograd_64():
         print(f"d/d{name}: rel-L2 vs FP64 autograd {rel:.2e}")
         assert np.abs(ref).max() > 0
         assert rel < 1e-3, name
+
+
+def test_fp32_partials_against_per_step_fp64_accumulation():
+    # The gradient maps take each step's FP32 terms into FP32 partials and add
+    # those into FP64 every 32 steps; with a flush every step the time
+    # accumulation is FP64 throughout.  The default grouping must stay within
+    # 1e-6 rel-L2 of it on every map (measured 6e-8: the FP32 rounding of
+    # 32-term partial sums), and both as close to FP64 autograd as the FP32
+    # wavefields allow (the same 7e-6 .. 1.3e-5 for both).
+    steps = 250
+    sim, att = _setup(64, steps)
+    true, start = _perturbed(sim)
+    eng = VSEGrad(VSESim.from_simulator(sim, variant="wgsl", att=att), seg_len=16)
+    eng.set_model(*true)
+    obs = eng.forward(steps)
+    res = {}
+    for every in (32, 1, 7):
+        eng.set_flush(every)
+        eng.set_model(*start)
+        eng.forward(steps)
+        eng.misfit(obs)
+        res[every] = eng.backward()
+    for bad in (0, 4097):
+        with pytest.raises(VSEError):
+            eng.set_flush(bad)
+    eng.close()
+
+    m = Visco2D(sim, att, n_steps=steps)
+    params = [torch.tensor(a.astype(np.float64), requires_grad=True) for a in start]
+    tr = m.forward(*params)
+    (0.5 * ((tr - torch.tensor(obs.astype(np.float64))) ** 2).sum()).backward()
+    cells = {every: [t.numpy() for t in chain_to_cells(r, *start[:3])] + [r["tau"]] for every, r in res.items()}
+    for name in GRAD_NAMES:
+        d32, d7 = oracle.rel_l2(res[32][name], res[1][name]), oracle.rel_l2(res[7][name], res[1][name])
+        print(f"{name}: flush 32 vs every step {d32:.2e}, flush 7 vs every step {d7:.2e}", flush=True)
+        assert np.abs(res[1][name]).max() > 0
+        assert d32 < 1e-6 and d7 < 1e-6, name
+    for i, (name, p) in enumerate(zip(("rho", "cp", "cs", "tau"), params)):
+        ref = p.grad.numpy()
+        e32, e1 = oracle.rel_l2(cells[32][i], ref), oracle.rel_l2(cells[1][i], ref)
+        print(f"d/d{name} vs FP64 autograd: flush 32 {e32:.2e}, every step {e1:.2e}", flush=True)
+        assert e32 < 1e-3 and e1 < 1e-3, name
 
 
 def test_gradient_central_differences_10_directions():

@@ -95,7 +95,7 @@ def bytes_per_cell_2d(g, L):
 
 def bench_2d(args):
     """2D viscoelastic time loop at 2048^2 (see module doc)."""
-    from taichi_sim_amd import workloads
+    from taichi_sim_amd import autotune, workloads
     from taichi_sim_amd.backend import attenuation_from_config
     from taichi_sim_amd.engine import VSESim
     from taichi_sim_amd.sim_support.simulator import Simulator
@@ -109,7 +109,11 @@ def bench_2d(args):
     ncell = int(np.prod(g.shape))
     print(f"# setup done: grid {g.shape}, {g.device_bytes() / 2**30:.2f} GiB on {g.device_name()}",
           file=sys.stderr, flush=True)
+    best = autotune.tune(g, steps=3, cache=None)           # before the timed region; it advances the state
+    shapes = {k: list(best[k]) for k in ("stress", "velocity")}
+    print(f"# launch shapes (ty, nt, rows, remap, step ms): {shapes}", file=sys.stderr, flush=True)
     if args.warmup:
+        g.reset()
         g.run(args.warmup, check_every=args.warmup)
     g.reset()
     g.run(min(args.steps, 256), check_every=256)           # capture the graph outside the timed region
@@ -147,6 +151,7 @@ def bench_2d(args):
                                f"16 seeded voids, 64-element array with a focusing law, 64 receivers",
                    "grid": [n, n], "n_sls": 3, "fd_order": 2, "parallelism": "single GPU",
                    "math": args.math2d},
+        "launch_shapes": shapes,
         "hbm_frac_step": round(gcell * 1e9 * b_step / (HBM_PEAK_GBS * 1e9), 4),
         "bytes_per_cell_step": round(b_step, 2),
         "kernel_ms": {"stress": round(k_ms[0], 4), "velocity": round(k_ms[1], 4),
@@ -364,7 +369,8 @@ def main():
         line["shots_per_s"] = {"batched": s["value"], "per_shot_loop": s["per_shot_loop_shots_per_s"],
                                "speedup": s["batch_speedup"], "shots": s["config"]["shots"],
                                "window_steps": s["config"]["window_steps"], "shot_steps": 3000,
-                               "cache_groups": s["config"]["cache_groups"], "workload": s["config"]["workload"]}
+                               "cache_groups": s["config"]["cache_groups"], "tuning": s["tuning"],
+                               "workload": s["config"]["workload"]}
         a = measure_adjoint(1)
         line["adjoint_over_forward"] = {"value": a["value"], "forward_ms": a["forward_ms_per_shot"],
                                         "checkpointed_forward_ms": a["checkpointed_forward_ms_per_shot"],
@@ -410,7 +416,7 @@ def measure_shots(sizes, n_shots, steps, warmup=0, profile_steps=10, group_sweep
     """FMC on plates of the given sizes x n_shots: every shot as one batch
     against the same shots one after another, over a window of `steps` steps of
     each shot; shots/s is for 3000-step shots (window time x 3000 / steps)."""
-    from taichi_sim_amd import workloads
+    from taichi_sim_amd import autotune, workloads
     from taichi_sim_amd.backend import attenuation_from_config
     from taichi_sim_amd.engine import VSEBatch, VSESim
     from taichi_sim_amd.sim_support.simulator import Simulator
@@ -431,7 +437,22 @@ def measure_shots(sizes, n_shots, steps, warmup=0, profile_steps=10, group_sweep
     print(f"# setup: {len(sims)} simulations, {cells / 1e6:.0f} Mcell, {gib:.1f} GiB, "
           f"{time.perf_counter() - t_setup:.0f} s", file=sys.stderr, flush=True)
 
+    # launch shapes: the 2D autotuner once per member shape (the per-shot loop
+    # and the batch's groups of one run with them), then the batch's row groups
+    # and cache-group budget
+    t_tune = time.perf_counter()
+    tuned2d = {}
+    for g in sims:
+        if g.shape not in tuned2d:
+            best = autotune.tune(g, steps=3, cache=None)
+            tuned2d[g.shape] = {k: best[k] for k in ("stress", "velocity")}
+        for k, name in enumerate(("stress", "velocity")):
+            g.set_tuning(k, *tuned2d[g.shape][name][:4])
     batch = VSEBatch(sims)
+    bt = autotune.tune_batch(batch, cache=None)
+    tuning = {"solo": {"x".join(map(str, shp)): {k: list(v[:4]) for k, v in t.items()} for shp, t in tuned2d.items()},
+              "batch": {"rows": bt["rows"], "group_mb": bt["group_mb"], "groups": bt["groups"]}}
+    print(f"# tuning ({time.perf_counter() - t_tune:.0f} s): {tuning}", file=sys.stderr, flush=True)
     groups = batch.groups()
     k_batch = k_solo = None
     if profile_steps > 0:
@@ -498,6 +519,7 @@ def measure_shots(sizes, n_shots, steps, warmup=0, profile_steps=10, group_sweep
         "gcell_per_s_loop": round(cells * steps / wall_l / 1e9, 3),
         "kernel_ms_all_in_one_launch": ms(k_batch),
         "kernel_ms_loop_sum": ms(k_solo),
+        "tuning": tuning,
         "device": name,
     }
 

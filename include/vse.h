@@ -152,9 +152,11 @@ int vse_run_noalloc(vse_sim *s, int n_steps, void *stream);
  * loop, sim_webgpu_viscoelastic.py:470,537). */
 int vse_profile(vse_sim *s, int n_steps, float *ms);
 
-/* Launch shape of a kernel (kernel 0 = stress, 1 = velocity).  2D: only remap
- * applies (1, the default: each XCD takes a contiguous band of 64x4 tiles;
- * 0: round-robin order).  3D: workgroup
+/* Launch shape of a kernel (kernel 0 = stress, 1 = velocity).  2D: xchunk is
+ * the number of 4-row groups each 64x4 workgroup walks in x (0 or 1: one,
+ * at most 16) and remap the tile order (1, the default: each XCD takes a
+ * contiguous band of tiles; 0: round-robin order); ty and nt do not apply.
+ * 3D: workgroup
  * tile of 64 (z) x ty (y) cells, ty in {1,4,8}; nt = 1 streams the fields
  * touched only at the own cell with non-temporal loads/stores; each
  * workgroup marches through xchunk x-planes (0 = the plain kernel, one cell

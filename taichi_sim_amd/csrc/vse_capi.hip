@@ -1000,6 +1000,8 @@ int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap
     return s->fail(VSE_ERR_ARG, "tuning: ty in {1,4,8}, nt in {0,1}, xchunk >= 0, remap in {0,1,2}");
   if (kernel == 2 && (ty == 1 || nt != 0 || xchunk < 1))
     return s->fail(VSE_ERR_ARG, "fused-step tuning: ty in {4,8}, nt 0, xchunk >= 1");
+  if (s->d.ndim == 2 && (kernel == 2 || xchunk > 16))
+    return s->fail(VSE_ERR_ARG, "2D tuning: kernel 0 or 1, xchunk (row groups per workgroup) 0 .. 16");
   OnDevice on_dev(s->device);
   s->tune[kernel] = KTune{ty, nt, xchunk, remap};
   s->drop_graph();

@@ -295,10 +295,13 @@ __device__ __forceinline__ Tile2 tile2d(int nty, int remap) {
   return Tile2{t % nty, t / nty};
 }
 
+// A tile is BY (y) x BX*rows (x) cells: the workgroup walks its `rows`
+// row-groups of BX rows one after another (the autotuner picks rows).
 template <int VAR, bool VISCO, int ORD, int LC = 0, bool FAST = false>
-__global__ __launch_bounds__(BX * BY) void k_stress2d(StepArgs A, int nty, int remap) {
+__global__ __launch_bounds__(BX * BY) void k_stress2d(StepArgs A, int nty, int remap, int rows) {
   const Tile2 t = tile2d(nty, remap);
-  stress2d_cell<VAR, VISCO, ORD, LC, FAST>(A, t.tx * BX + threadIdx.y, t.ty * BY + threadIdx.x);
+  const int x0 = t.tx * BX * rows + threadIdx.y, y = t.ty * BY + threadIdx.x;
+  for (int k = 0; k < rows; k++) stress2d_cell<VAR, VISCO, ORD, LC, FAST>(A, x0 + k * BX, y);
 }
 
 // The step's sparse work, done inside the velocity pass (vse_sparse.hip does
@@ -353,12 +356,16 @@ __device__ __forceinline__ void record_receivers(const StepArgs &A, int s) {
 
 // Velocity pass of one simulation, with its step's sparse work.
 template <int VAR, bool VISCO, int ORD, bool FAST = false>
-__global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A, int nty, int remap, int i) {
+__global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A, int nty, int remap, int rows, int i) {
   const Tile2 t = tile2d(nty, remap);
-  const int x = t.tx * BX + threadIdx.y, y = t.ty * BY + threadIdx.x;
+  const int x0 = t.tx * BX * rows + threadIdx.y, y = t.ty * BY + threadIdx.x;
   const int s = *A.step + i;
-  unsigned v2 = velocity2d_cell<VAR, VISCO, ORD, FAST>(A, x, y, s);
-  source_cell<VAR>(A, x, y, s, v2);
+  unsigned v2 = 0;
+  for (int k = 0; k < rows; k++) {
+    unsigned u = velocity2d_cell<VAR, VISCO, ORD, FAST>(A, x0 + k * BX, y, s);
+    source_cell<VAR>(A, x0 + k * BX, y, s, u);
+    v2 = v2 > u ? v2 : u;
+  }
   if (blockIdx.x == 0 && !A.rec_after) record_receivers(A, s);
   wave_max_to_slot(v2, A.v2slots + blockIdx.x);
 }
@@ -436,15 +443,16 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B, int i
 template <int VAR, bool VISCO, bool FAST>
 hipError_t launch_ord(bool stress, int ord, const StepArgs &A, const KTune &t, int i, hipStream_t st) {
   const dim3 block(BY, BX);
-  const int nty = (A.ny + BY - 1) / BY, ntx = (A.nx + BX - 1) / BX;
+  const int rows = t.xchunk > 1 ? t.xchunk : 1;
+  const int nty = (A.ny + BY - 1) / BY, ntx = ((A.nx + BX - 1) / BX + rows - 1) / rows;
   const dim3 grid((unsigned)(nty * ntx));
   const int rm = t.remap;
   const bool l3 = VISCO && VAR == VSE_VAR_WGSL && A.L == 3;
-#define VSE_CASE(O)                                                                                        \
-  case O:                                                                                                  \
-    if (stress && l3) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O, 3, FAST>), grid, block, 0, st, A, nty, rm); \
-    else if (stress) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O, 0, FAST>), grid, block, 0, st, A, nty, rm); \
-    else hipLaunchKernelGGL((k_velocity2d<VAR, VISCO, O, FAST>), grid, block, 0, st, A, nty, rm, i);          \
+#define VSE_CASE(O)                                                                                                \
+  case O:                                                                                                          \
+    if (stress && l3) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O, 3, FAST>), grid, block, 0, st, A, nty, rm, rows); \
+    else if (stress) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O, 0, FAST>), grid, block, 0, st, A, nty, rm, rows); \
+    else hipLaunchKernelGGL((k_velocity2d<VAR, VISCO, O, FAST>), grid, block, 0, st, A, nty, rm, rows, i);          \
     break;
   switch (ord) {
     VSE_CASE(2) VSE_CASE(3) VSE_CASE(4) VSE_CASE(5) VSE_CASE(6)

@@ -10,6 +10,7 @@ import pytest
 
 import oracle
 from taichi_sim_amd import workloads
+from taichi_sim_amd._lib import VSEError
 from taichi_sim_amd.backend import attenuation_from_config
 from taichi_sim_amd.engine import VSESim
 from taichi_sim_amd.sim_support import StabilityError
@@ -178,6 +179,35 @@ def test_3d_launch_shapes_bit_identical():
         else:
             for a, b in zip(ref, out):
                 np.testing.assert_array_equal(a, b)
+    g.close()
+
+
+@pytest.mark.parametrize("variant,channel", [("wgsl", None), ("wgsl", "vy"), ("taichi", None)])
+def test_2d_launch_shapes_bit_identical(variant, channel):
+    """Every 2D launch shape the autotuner may pick (row groups per workgroup,
+    tile order) gives the same bits: fields, memory variables, traces, |v|^2
+    max; an nx that no row count divides exercises the ragged last tile."""
+    cfg = workloads.visco_2d_array(n=(203, 160), steps=60, n_elem=8, n_voids=2)
+    sim = Simulator(cfg)
+    att = attenuation_from_config(cfg)
+    g = VSESim.from_simulator(sim, variant=variant, att=att, channel=channel)
+    ref = None
+    for rows, rm in [(1, 1), (1, 0), (2, 1), (3, 0), (4, 1), (8, 0), (16, 1)]:
+        g.set_tuning(0, 4, 0, rows, rm)
+        g.set_tuning(1, 4, 0, 17 - rows, 1 - rm)
+        g.reset()
+        g.run(60, check_every=20)
+        out = [g.field(k) for k in ("vx", "vy", "sxx", "syy", "sxy")] + [g.traces(), np.array([g.v2max()])]
+        if variant == "wgsl":
+            out += [g.memory(c) for c in ("xx", "yy", "xy")]
+        if ref is None:
+            ref = out
+            assert np.abs(out[3]).max() > 0 and np.abs(out[5]).max() > 0
+        else:
+            for a, b in zip(ref, out):
+                np.testing.assert_array_equal(a, b)
+    with pytest.raises(VSEError):
+        g.set_tuning(0, 4, 0, 17, 1)
     g.close()
 
 

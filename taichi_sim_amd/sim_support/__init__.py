@@ -30,6 +30,38 @@ _LIU_NUM_DEN = [
 coefs_Lui = [[n / d for (n, d) in row] for row in _LIU_NUM_DEN]
 
 
+def _forward_stencil(k):
+    """First-derivative weights on the offsets -(k+1) .. k+2 (exact rationals):
+    the unique c with sum_j c_j o_j^m = [m == 1] for m = 0 .. 2k+3."""
+    from fractions import Fraction
+    offs = list(range(-(k + 1), k + 3))
+    n = len(offs)
+    rows = [[Fraction(o) ** m for o in offs] + [Fraction(int(m == 1))] for m in range(n)]
+    for i in range(n):                                   # Gauss-Jordan; the Vandermonde matrix is regular
+        p = next(r for r in range(i, n) if rows[r][i] != 0)
+        rows[i], rows[p] = rows[p], rows[i]
+        rows[i] = [v / rows[i][i] for v in rows[i]]
+        for r in range(n):
+            if r != i and rows[r][i] != 0:
+                f = rows[r][i]
+                rows[r] = [a - f * b for a, b in zip(rows[r], rows[i])]
+    return [rows[i][n] for i in range(n)]
+
+
+# Forward-difference ("unsplit") first-derivative coefficients, one row per
+# order: row k has 2k + 4 weights on the offsets -(k+1) .. k+2 (accuracy
+# 2k + 3).  The backward derivative uses the mirrored row with the sign
+# flipped (sim_support/__init__.py:32-42).
+coefs_forward = [[float(c) for c in _forward_stencil(k)] for k in range(5)]
+
+
+def unsplit_coefficients(row):
+    """(forward, backward) float32 weights of ``coefs_forward[row]``; the
+    backward weights apply to the mirrored offsets -(k+2) .. k+1."""
+    fwd = np.array(coefs_forward[row], dtype=flt32)
+    return fwd, -fwd[::-1]
+
+
 class CourantError(ValueError):
     """Raised at setup when the Courant number exceeds 1 (sim_support/__init__.py:48-62)."""
 
@@ -57,4 +89,4 @@ class StabilityError(ValueError):
 
 
 __all__ = ['flt32', 'int32', 'PI', 'DEGREES_TO_RADIANS', 'ZERO', 'STABILITY_THRESHOLD',
-           'HUGEVAL', 'coefs_Lui', 'CourantError', 'StabilityError']
+           'HUGEVAL', 'coefs_Lui', 'coefs_forward', 'unsplit_coefficients', 'CourantError', 'StabilityError']

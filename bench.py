@@ -59,6 +59,8 @@ def emit(obj):
 # stress-kernel HBM traffic measured below.
 SHAPES_3D = [(4, 0, 4, 2), (4, 0, 8, 2), (4, 0, 16, 2), (1, 0, 4, 2), (1, 0, 8, 2), (1, 0, 16, 2),
              (4, 0, 4, 1), (4, 0, 8, 1), (1, 0, 4, 1), (1, 0, 8, 1)]
+# the velocity pass also tries its LDS-staged y-neighbour shapes (nt bit 1)
+VELOCITY_SHAPES_3D = SHAPES_3D + [(4, 2, 8, 2), (4, 2, 16, 2)]
 # HBM bytes per launch of the 512^3 viscoelastic FAST stress kernel by launch
 # shape, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over
 # `bench.py --shape-sweep` (scaled by 2.0 and 1.0 from the copy-kernel
@@ -170,8 +172,8 @@ def shape_sweep(g, k):
     at each shape of SHAPES_3D in turn, the velocity pass fixed: the run that
     rocprofv3 --pmc counts per dispatch (after 2 warm-up dispatches at the
     first shape, dispatch 2 + i of the stress kernel has shape SHAPES_3D[i // k]);
-    then the same for the velocity pass (its dispatch 2 + k * len(SHAPES_3D) + i
-    has shape SHAPES_3D[i // k])."""
+    then the same for the velocity pass over VELOCITY_SHAPES_3D (its dispatch
+    2 + k * len(SHAPES_3D) + i has shape VELOCITY_SHAPES_3D[i // k])."""
     g.set_math("fast")
     g.set_tuning(0, *SHAPES_3D[0])
     g.set_tuning(1, *SHAPES_3D[0])
@@ -185,7 +187,7 @@ def shape_sweep(g, k):
     # then the velocity pass at each shape, the stress pass fixed at the first
     g.set_tuning(0, *SHAPES_3D[0])
     vel = []
-    for shape in SHAPES_3D:
+    for shape in VELOCITY_SHAPES_3D:
         g.set_tuning(1, *shape)
         ms = g.profile(k)
         vel.append({"shape": list(shape), "velocity_ms": round(ms[1], 4)})
@@ -279,7 +281,8 @@ def main():
     else:
         for math in ("fast", "strict"):
             g.set_math(math)
-            best = autotune.tune(g, steps=2, candidates=SHAPES_3D, cache=None)
+            best = autotune.tune(g, steps=2, candidates={"stress": SHAPES_3D, "velocity": VELOCITY_SHAPES_3D},
+                                 cache=None)
             tuned[math] = {k: best[k] for k in ("stress", "velocity")}
     print(f"# launch shapes (ty, nt, xchunk, remap, step ms): {tuned}", file=sys.stderr, flush=True)
 

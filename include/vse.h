@@ -157,8 +157,10 @@ int vse_profile(vse_sim *s, int n_steps, float *ms);
  * at most 16) and remap the tile order (1, the default: each XCD takes a
  * contiguous band of tiles; 0: round-robin order); ty and nt do not apply.
  * 3D: workgroup
- * tile of 64 (z) x ty (y) cells, ty in {1,4,8}; nt = 1 streams the fields
- * touched only at the own cell with non-temporal loads/stores; each
+ * tile of 64 (z) x ty (y) cells, ty in {1,4,8}; nt bit 0 streams the fields
+ * touched only at the own cell with non-temporal loads/stores; nt bit 1
+ * (velocity pass only, ty >= 2 * FD order, xchunk >= 1) stages the y-stencil
+ * operands of sigma_xy, sigma_yy and sigma_yz in LDS per x-plane; each
  * workgroup marches through xchunk x-planes (0 = the plain kernel, one cell
  * per thread on a (z, y, x) grid, where ty and nt do not apply; the velocity
  * pass of a grid of 2^30 cells or more always runs the plain kernel);

@@ -72,11 +72,16 @@ def _store(path, key, value):
         pass
 
 
-def candidates_for(g):
-    """Every launch shape (ty, nt, xchunk, remap) ``tune`` tries on ``g``."""
+def candidates_for(g, name="stress"):
+    """Every launch shape (ty, nt, xchunk, remap) ``tune`` tries on pass ``name``
+    of ``g``; the 3D velocity pass adds the LDS-staged shapes (nt bit 1, ty >=
+    2 * FD order)."""
     if g.ndim == 2:
         return [(4, 0, rows, rm) for rows in ROWS_2D for rm in REMAP_2D]
-    return [PLAIN] + [(ty, nt, xc, rm) for ty in TY for nt in NT for xc in XCHUNK for rm in REMAP]
+    out = [PLAIN] + [(ty, nt, xc, rm) for ty in TY for nt in NT for xc in XCHUNK for rm in REMAP]
+    if name == "velocity":
+        out += [(ty, 2 | nt, xc, rm) for ty in TY if ty >= 2 * g.ord for nt in NT for xc in XCHUNK for rm in REMAP]
+    return out
 
 
 def tune(g, steps=3, candidates=None, cache=DEFAULT_CACHE, verbose=False, use_cache=True, finalists=3,
@@ -88,6 +93,7 @@ def tune(g, steps=3, candidates=None, cache=DEFAULT_CACHE, verbose=False, use_ca
     of each other are not told apart reliably by a short timing), and the fastest
     of that second timing pass is kept.
 
+    ``candidates`` is one list for both passes or {"stress": [...], "velocity": [...]}.
     Returns {"stress": (ty, nt, xchunk, remap, step_ms), "velocity": (...), "table": [...]}.
     """
     key = problem_key(g)
@@ -98,10 +104,13 @@ def tune(g, steps=3, candidates=None, cache=DEFAULT_CACHE, verbose=False, use_ca
             ty, nt, xc, rm, _ = best[name]
             g.set_tuning(k, ty, nt, xc, rm)
         return best
-    cands = candidates or candidates_for(g)
     table = []
     best = {}
     for k, name in enumerate(("stress", "velocity")):
+        if isinstance(candidates, dict):
+            cands = candidates[name]
+        else:
+            cands = candidates or candidates_for(g, name)
         results = []
         for ty, nt, xc, rm in cands:
             g.set_tuning(k, ty, nt, xc, rm)

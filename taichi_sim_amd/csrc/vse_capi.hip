@@ -996,8 +996,11 @@ int vse_profile(vse_sim *s, int n_steps, float *ms) {
 
 int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap) {
   if (check_handle(s) || kernel < 0 || kernel > 2) return VSE_ERR_ARG;
-  if (!(ty == 1 || ty == 4 || ty == 8) || (nt != 0 && nt != 1) || xchunk < 0 || remap < 0 || remap > 2)
-    return s->fail(VSE_ERR_ARG, "tuning: ty in {1,4,8}, nt in {0,1}, xchunk >= 0, remap in {0,1,2}");
+  if (!(ty == 1 || ty == 4 || ty == 8) || nt < 0 || nt > 3 || xchunk < 0 || remap < 0 || remap > 2)
+    return s->fail(VSE_ERR_ARG, "tuning: ty in {1,4,8}, nt in {0..3}, xchunk >= 0, remap in {0,1,2}");
+  if ((nt & 2) && (s->d.ndim != 3 || kernel != 1 || xchunk < 1 || ty < 2 * s->d.ord))
+    return s->fail(VSE_ERR_ARG, "tuning: nt bit 1 (LDS-staged y-neighbours) needs the 3D velocity pass (kernel 1), "
+                                "xchunk >= 1 and ty >= 2 * FD order");
   if (kernel == 2 && (ty == 1 || nt != 0 || xchunk < 1))
     return s->fail(VSE_ERR_ARG, "fused-step tuning: ty in {4,8}, nt 0, xchunk >= 1");
   if (s->d.ndim == 2 && (kernel == 2 || xchunk > 16))

@@ -474,6 +474,164 @@ __global__ __launch_bounds__(BZ * TY) void k_velocity3d(StepArgs A, int xchunk, 
   block_max_u32(v2bits, A.v2max);
 }
 
+// The same velocity pass with the y-neighbours of sigma_xy, sigma_yy and
+// sigma_yz staged in LDS: per x-plane, each thread puts its own cell of the
+// three fields into the tile's rows, the first 2*ORD wavefronts load the ORD
+// halo rows above and below the tile, and after one barrier every y-stencil
+// operand comes from LDS.  A tile then loads TY + 2*ORD rows of those fields
+// per plane instead of TY * (2*ORD + ...) separate row loads that mostly hit
+// other tiles' rows.  The LDS rows are double-buffered by plane parity, so a
+// plane needs one barrier: a wavefront writes buffer p only after every
+// wavefront has passed the barrier of the plane that read p before.  Same
+// operands, same arithmetic: bit-identical to k_velocity3d.  Needs TY >= 2*ORD.
+template <int ORD, int TY, bool NT, bool FAST>
+__global__ __launch_bounds__(BZ * TY) void k_velocity3d_lds(StepArgs A, int xchunk, int nzb, int nyb, int nxc, int remap) {
+  static_assert(TY >= 2 * ORD, "the halo rows are loaded by the first 2*ORD wavefronts");
+  constexpr int CP = NT ? 2 : 0;
+  constexpr int ROWS = TY + 2 * ORD;
+  __shared__ float s_xy[2][ROWS][BZ], s_yy[2][ROWS][BZ], s_yz[2][ROWS][BZ];
+  const Tile T = tile_of_block(nzb, nyb, nxc, remap);
+  const int ty = threadIdx.y;
+  const int z = T.bz * BZ + threadIdx.x;
+  const int y0 = T.by * TY;
+  const int y = y0 + ty;
+  const int nx = A.nx, ny = A.ny, nz = A.nz;
+  const bool valid = z < nz && y < ny;
+  constexpr int last = ORD - 1, off = ORD - 1;
+  const unsigned sy = nz;
+  const long sx = (long)ny * nz;
+  const unsigned pbytes = (unsigned)(sx * 4);
+  const float dt = A.dt;
+  const int x0 = T.bxc * xchunk, x1 = min(nx, x0 + xchunk);
+  const FBuf Fsxx(A.sxx, A.ncell), Fsxy(A.sxy, A.ncell), Fsxz(A.sxz, A.ncell), Fsyy(A.syy, A.ncell);
+  const FBuf Fsyz(A.syz, A.ncell), Fszz(A.szz, A.ncell), Frho(A.rho, A.ncell);
+  const FBuf Fvx(A.vx, A.ncell), Fvy(A.vy, A.ncell), Fvz(A.vz, A.ncell);
+  auto pl = [&](int x) { return (unsigned)((x < 0 ? 0 : (x >= nx ? nx - 1 : x)) * sx); };
+  // column of this thread (z clamped for the lanes past nz, which compute nothing)
+  const unsigned col = (unsigned)(min(y, ny - 1)) * sy + (unsigned)min(z, nz - 1);
+  // halo row loaded by this wavefront: ORD rows above the tile, then ORD below
+  const bool halo = ty < 2 * ORD;
+  const int hrow = ty < ORD ? ty : TY + ty;                    // LDS row
+  const int hy = y0 - ORD + hrow;                              // grid row (may lie outside [0, ny))
+  const unsigned hcol = (unsigned)((long)hy * sy + min(z, nz - 1));
+  const int jy = y - off, jz = z - off;
+  const int by = band_index(A.ax[1], jy), bz = band_index(A.ax[2], jz);
+  const int nby = A.ax[1].nb, nbz = A.ax[2].nb;
+  const unsigned py_col = by * nz + z, pz_col = y * nbz + bz;
+  const unsigned pyb = (unsigned)nby * nz * 4u, pzb = (unsigned)ny * nbz * 4u;
+  const bool yi_vx = y >= last + 1 && y < ny - last, zi_vx = z >= last + 1 && z < nz - last;
+  const bool yi_vy = y >= last && y < ny - last - 1, zi_vy = z >= last + 1 && z < nz - last;
+  const bool yi_vz = y >= last + 1 && y < ny - last, zi_vz = z >= last && z < nz - last - 1;
+  float wxx[2 * ORD], wxy[2 * ORD], wxz[2 * ORD];
+#pragma unroll
+  for (int i = 0; i < 2 * ORD - 1; i++) {
+    wxx[i + 1] = Fsxx.ld(pl(x0 - ORD + i) + col);
+    wxy[i + 1] = Fsxy.ld(pl(x0 - (ORD - 1) + i) + col);
+    wxz[i + 1] = Fsxz.ld(pl(x0 - (ORD - 1) + i) + col);
+  }
+  float rho_n = Frho.ld(pl(x0) + col);
+  unsigned v2bits = 0;
+#pragma unroll 1
+  for (int x = x0; x < x1; x++) {
+    const int p = x & 1;
+#pragma unroll
+    for (int i = 0; i < 2 * ORD - 1; i++) { wxx[i] = wxx[i + 1]; wxy[i] = wxy[i + 1]; wxz[i] = wxz[i + 1]; }
+    wxx[2 * ORD - 1] = Fsxx.ld(pl(x + ORD - 1) + col);
+    wxy[2 * ORD - 1] = Fsxy.ld(pl(x + ORD) + col);
+    wxz[2 * ORD - 1] = Fsxz.ld(pl(x + ORD) + col);
+    const float rho0 = rho_n;
+    rho_n = Frho.ld(pl(x + 1) + col);
+    const long rofs = (long)x * sx;
+    const unsigned i0 = (unsigned)rofs + col;
+    // own cells of the three y-stencil fields into the tile rows, halo rows
+    // from the neighbouring tiles (a row outside the grid reads the adjacent
+    // plane or 0, used only outside the interiors, as in k_velocity3d)
+    s_xy[p][ty + ORD][threadIdx.x] = wxy[ORD - 1];
+    s_yy[p][ty + ORD][threadIdx.x] = Fsyy.ld(i0);
+    s_yz[p][ty + ORD][threadIdx.x] = Fsyz.ld(i0);
+    if (halo) {
+      const unsigned h0 = (unsigned)rofs + hcol;
+      s_xy[p][hrow][threadIdx.x] = Fsxy.ld(h0);
+      s_yy[p][hrow][threadIdx.x] = Fsyy.ld(h0);
+      s_yz[p][hrow][threadIdx.x] = Fsyz.ld(h0);
+    }
+    float vxn = Fvx.ld<CP>(i0), vyn = Fvy.ld<CP>(i0), vzn = Fvz.ld<CP>(i0);
+    float xz_p[ORD], xz_m[ORD], yz_p[ORD], yz_m[ORD], zz_p[ORD], zz_m[ORD];
+#pragma unroll
+    for (int c = 0; c < ORD; c++) {
+      xz_p[c] = c == 0 ? wxz[ORD - 1] : Fsxz.ld(i0 + c); xz_m[c] = Fsxz.ld(i0 - c - 1);
+      yz_p[c] = Fsyz.ld(i0 + c); yz_m[c] = Fsyz.ld(i0 - c - 1);
+      zz_p[c] = Fszz.ld(i0 + c + 1); zz_m[c] = Fszz.ld(i0 - c);
+    }
+    const float rho_y1 = Frho.ld(i0 + sy);
+    __syncthreads();
+    float xy_p[ORD], xy_m[ORD], yy_p[ORD], yy_m[ORD], zy_p[ORD], zy_m[ORD];
+#pragma unroll
+    for (int c = 0; c < ORD; c++) {
+      xy_p[c] = s_xy[p][ty + ORD + c][threadIdx.x]; xy_m[c] = s_xy[p][ty + ORD - c - 1][threadIdx.x];
+      yy_p[c] = s_yy[p][ty + ORD + c + 1][threadIdx.x]; yy_m[c] = s_yy[p][ty + ORD - c][threadIdx.x];
+      zy_p[c] = s_yz[p][ty + ORD + c][threadIdx.x]; zy_m[c] = s_yz[p][ty + ORD - c - 1][threadIdx.x];
+    }
+    if (!valid) continue;
+    const int jx = x - off;
+    const int bx = band_index(A.ax[0], jx);
+    // vx (wgsl3d :1647-1681)
+    if (x >= last + 1 && x < nx - last && yi_vx && zi_vx) {
+      float a = 0.f, b = 0.f, c_ = 0.f;
+#pragma unroll
+      for (int c = 0; c < ORD; c++) {
+        a += dterm<FAST, 0>(A, c, wxx[ORD + c], wxx[ORD - 1 - c]);
+        b += dterm<FAST, 1>(A, c, xy_p[c], xy_m[c]);
+        c_ += dterm<FAST, 2>(A, c, xz_p[c], xz_m[c]);
+      }
+      if (bx >= 0) a = stretch<CP, FAST>(A.ax[0].a, A.ax[0].b, A.ax[0].k, A.ax[0].ki, PB(A.psi[P_DSXX_DX] + (long)bx * sx, pbytes), col, jx, a);
+      if (by >= 0) b = stretch<CP, FAST>(A.ax[1].a, A.ax[1].b, A.ax[1].k, A.ax[1].ki, PB(A.psi[P_DSXY_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
+      if (bz >= 0) c_ = stretch<CP, FAST>(A.ax[2].a, A.ax[2].b, A.ax[2].k, A.ax[2].ki, PB(A.psi[P_DSXZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
+      const float rho = 0.5f * (rho_n + rho0);
+      if (rho > 0.f) vxn = (a + b + c_) * dt / rho + vxn;
+    }
+    // vy (wgsl3d :1683-1717)
+    const bool xi_vyz = x >= last && x < nx - last - 1;
+    if (xi_vyz && yi_vy && zi_vy) {
+      float a = 0.f, b = 0.f, c_ = 0.f;
+#pragma unroll
+      for (int c = 0; c < ORD; c++) {
+        a += dterm<FAST, 0>(A, c, wxy[ORD + c], wxy[ORD - 1 - c]);
+        b += dterm<FAST, 1>(A, c, yy_p[c], yy_m[c]);
+        c_ += dterm<FAST, 2>(A, c, yz_p[c], yz_m[c]);
+      }
+      if (bx >= 0) a = stretch<CP, FAST>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, A.ax[0].khi, PB(A.psi[P_DSXY_DX] + (long)bx * sx, pbytes), col, jx, a);
+      if (by >= 0) b = stretch<CP, FAST>(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, A.ax[1].khi, PB(A.psi[P_DSYY_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
+      if (bz >= 0) c_ = stretch<CP, FAST>(A.ax[2].a, A.ax[2].b, A.ax[2].k, A.ax[2].ki, PB(A.psi[P_DSYZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
+      const float rho = 0.5f * (rho_y1 + rho0);
+      if (rho > 0.f) vyn = (a + b + c_) * dt / rho + vyn;
+    }
+    // vz, with the vy density (wgsl3d :1719-1758)
+    if (xi_vyz && yi_vz && zi_vz) {
+      float a = 0.f, b = 0.f, c_ = 0.f;
+#pragma unroll
+      for (int c = 0; c < ORD; c++) {
+        a += dterm<FAST, 0>(A, c, wxz[ORD + c], wxz[ORD - 1 - c]);
+        b += dterm<FAST, 1>(A, c, zy_p[c], zy_m[c]);
+        c_ += dterm<FAST, 2>(A, c, zz_p[c], zz_m[c]);
+      }
+      if (bx >= 0) a = stretch<CP, FAST>(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, A.ax[0].khi, PB(A.psi[P_DSXZ_DX] + (long)bx * sx, pbytes), col, jx, a);
+      if (by >= 0) b = stretch<CP, FAST>(A.ax[1].a, A.ax[1].b, A.ax[1].k, A.ax[1].ki, PB(A.psi[P_DSYZ_DY] + (long)x * nby * nz, pyb), py_col, jy, b);
+      if (bz >= 0) c_ = stretch<CP, FAST>(A.ax[2].ah, A.ax[2].bh, A.ax[2].kh, A.ax[2].khi, PB(A.psi[P_DSZZ_DZ] + (long)x * ny * nbz, pzb), pz_col, jz, c_);
+      const float rho = 0.5f * (rho_y1 + rho0);
+      if (rho > 0.f) vzn = (a + b + c_) * dt / rho + vzn;
+    } else {
+      vxn = 0.f; vyn = 0.f; vzn = 0.f;
+    }
+    Fvx.st<CP>(i0, vxn);
+    Fvy.st<CP>(i0, vyn);
+    Fvz.st<CP>(i0, vzn);
+    const unsigned u = __float_as_uint(vxn * vxn + vyn * vyn + vzn * vzn);
+    v2bits = v2bits > u ? v2bits : u;
+  }
+  block_max_u32(v2bits, A.v2max);
+}
+
 // ---------------------------------------------------------------------------
 // Plain kernels: one cell per thread, grid (z-tiles, y-tiles, x), flat global
 // loads; neighbours in x come from L2/MALL.  The autotuner's xchunk = 0.
@@ -743,7 +901,7 @@ Launch launch_shape(const StepArgs &A, int ty, int xchunk) {
 }
 
 template <int ORD, int TY, bool NT, bool FAST>
-hipError_t launch_one(bool stress, int visco, const StepArgs &A, int xchunk, int remap, hipStream_t st) {
+hipError_t launch_one(bool stress, int visco, const StepArgs &A, int xchunk, int remap, bool lds, hipStream_t st) {
   const Launch L = launch_shape(A, TY, xchunk);
   const dim3 grid((unsigned)(L.nzb * L.nyb * L.nxc)), block(BZ, TY);
   if (stress) {
@@ -756,6 +914,11 @@ hipError_t launch_one(bool stress, int visco, const StepArgs &A, int xchunk, int
     if (done) {
     } else if (visco) hipLaunchKernelGGL((k_stress3d<true, ORD, TY, NT, FAST>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
     else hipLaunchKernelGGL((k_stress3d<false, ORD, TY, NT, FAST>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
+  } else if (lds) {
+    if constexpr (TY >= 2 * ORD)
+      hipLaunchKernelGGL((k_velocity3d_lds<ORD, TY, NT, FAST>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
+    else
+      return hipErrorInvalidValue;
   } else {
     hipLaunchKernelGGL((k_velocity3d<ORD, TY, NT, FAST>), grid, block, 0, st, A, L.xchunk, L.nzb, L.nyb, L.nxc, remap);
   }
@@ -783,13 +946,15 @@ hipError_t launch_tuned(bool stress, int visco, const StepArgs &A, const KTune &
     }
     return hipGetLastError();
   }
-  // compiled shapes: TY in {1, 4, 8} x non-temporal on/off
-  if (t.ty == 1) return t.nt ? launch_one<ORD, 1, true, FAST>(stress, visco, A, t.xchunk, t.remap, st)
-                             : launch_one<ORD, 1, false, FAST>(stress, visco, A, t.xchunk, t.remap, st);
-  if (t.ty == 4) return t.nt ? launch_one<ORD, 4, true, FAST>(stress, visco, A, t.xchunk, t.remap, st)
-                             : launch_one<ORD, 4, false, FAST>(stress, visco, A, t.xchunk, t.remap, st);
-  if (t.ty == 8) return t.nt ? launch_one<ORD, 8, true, FAST>(stress, visco, A, t.xchunk, t.remap, st)
-                             : launch_one<ORD, 8, false, FAST>(stress, visco, A, t.xchunk, t.remap, st);
+  // compiled shapes: TY in {1, 4, 8} x non-temporal on/off (nt bit 0); nt bit 1
+  // stages the velocity pass's y-neighbours in LDS (TY >= 2*ORD)
+  const bool nt = (t.nt & 1) != 0, lds = !stress && (t.nt & 2) != 0;
+  if (t.ty == 1) return nt ? launch_one<ORD, 1, true, FAST>(stress, visco, A, t.xchunk, t.remap, lds, st)
+                           : launch_one<ORD, 1, false, FAST>(stress, visco, A, t.xchunk, t.remap, lds, st);
+  if (t.ty == 4) return nt ? launch_one<ORD, 4, true, FAST>(stress, visco, A, t.xchunk, t.remap, lds, st)
+                           : launch_one<ORD, 4, false, FAST>(stress, visco, A, t.xchunk, t.remap, lds, st);
+  if (t.ty == 8) return nt ? launch_one<ORD, 8, true, FAST>(stress, visco, A, t.xchunk, t.remap, lds, st)
+                           : launch_one<ORD, 8, false, FAST>(stress, visco, A, t.xchunk, t.remap, lds, st);
   return hipErrorInvalidValue;
 }
 

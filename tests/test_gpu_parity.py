@@ -211,6 +211,42 @@ def test_2d_launch_shapes_bit_identical(variant, channel):
     g.close()
 
 
+@pytest.mark.parametrize("acc", [2, 4])
+def test_3d_lds_velocity_shapes_bit_identical(acc):
+    """The velocity pass with LDS-staged y-neighbours (nt bit 1) gives the same
+    bits as the register x-march, on a grid whose ny and nz no tile divides."""
+    cfg = workloads.visco_3d_block(n=(50, 61, 130), steps=30, n_incl=3, n_rec_side=3, t0=0.3)
+    cfg["simul_params"]["acc"] = acc
+    sim = Simulator3D(cfg)
+    att = attenuation_from_config(cfg)
+    g = VSESim.from_simulator(sim, variant="wgsl", att=att)
+    assert g.ord == acc
+    g.set_tuning(0, 1, 0, 4, 2)
+    shapes = [(1, 0, 8, 2)] + [(ty, nt, xc, rm) for ty, nt, xc, rm in
+                               [(4, 2, 8, 2), (8, 2, 1, 0), (8, 3, 3, 1), (4, 3, 64, 0), (8, 2, 16, 2), (4, 2, 5, 1)]
+                               if ty >= 2 * acc]
+    ref = None
+    for shape in shapes:
+        g.set_tuning(1, *shape)
+        g.reset()
+        g.run(30, check_every=10)
+        out = _state3d(g)
+        if ref is None:
+            ref = out
+            assert np.abs(out["vz"]).max() > 0
+        else:
+            for k in ref:
+                np.testing.assert_array_equal(out[k], ref[k], err_msg=f"{shape} {k}")
+    with pytest.raises(VSEError):
+        g.set_tuning(0, 8, 2, 4, 2)          # LDS staging is for the velocity pass only
+    with pytest.raises(VSEError):
+        g.set_tuning(1, 8, 2, 0, 2)          # not with the plain kernel
+    if acc == 4:
+        with pytest.raises(VSEError):
+            g.set_tuning(1, 4, 2, 8, 2)      # ty < 2 * FD order
+    g.close()
+
+
 def _state3d(g):
     out = {k: g.field(k) for k in ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")}
     for c in ("xx", "yy", "zz", "xy", "xz", "yz"):

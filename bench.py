@@ -65,12 +65,13 @@ VELOCITY_SHAPES_3D = SHAPES_3D + [(4, 2, 8, 2), (4, 2, 16, 2)]
 # shape, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over
 # `bench.py --shape-sweep` (scaled by 2.0 and 1.0 from the copy-kernel
 # calibration); the table and commands are in profiles/visco3d_512cube_kernels.md.
-STRESS_TRAFFIC_512 = {(4, 0, 4, 2): 32.05e9, (4, 0, 8, 2): 32.38e9, (4, 0, 16, 2): 33.20e9,
-                      (1, 0, 4, 2): 31.97e9, (1, 0, 8, 2): 32.03e9, (1, 0, 16, 2): 32.80e9,
-                      (4, 0, 4, 1): 32.29e9, (4, 0, 8, 1): 32.54e9, (1, 0, 4, 1): 32.22e9,
-                      (1, 0, 8, 1): 32.14e9}
-# The same for the 2048^2 2D viscoelastic stress kernel (profiles/visco2d_2048_kernels.md).
-STRESS2D_TRAFFIC_2048 = 489.5e6
+STRESS_TRAFFIC_512 = {(4, 0, 4, 2): 31.86e9, (4, 0, 8, 2): 32.02e9, (4, 0, 16, 2): 32.79e9,
+                      (1, 0, 4, 2): 31.78e9, (1, 0, 8, 2): 31.64e9, (1, 0, 16, 2): 32.25e9,
+                      (4, 0, 4, 1): 32.10e9, (4, 0, 8, 1): 32.13e9, (1, 0, 4, 1): 32.05e9,
+                      (1, 0, 8, 1): 31.75e9}
+# The same for the 2048^2 2D viscoelastic stress kernel at its tuned shape
+# (profiles/visco2d_2048_kernels.md).
+STRESS2D_TRAFFIC_2048 = 494.8e6
 
 
 def bytes_per_cell(g, L):
@@ -161,6 +162,8 @@ def bench_2d(args):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": STRESS2D_TRAFFIC_2048 if n == 2048 else None,
+                     "traffic_source": None if n != 2048 else
+                     "rocprofv3 FETCH_SIZE + WRITE_SIZE of the tuned shape, profiles/visco2d_2048_kernels.md",
                      "kernel": "stress2d", "bytes_per_cell": round(b_stress, 2)},
         "cpu_baseline": cpu,
     }

@@ -155,7 +155,9 @@ int vse_profile(vse_sim *s, int n_steps, float *ms);
 /* Launch shape of a kernel (kernel 0 = stress, 1 = velocity).  2D: xchunk is
  * the number of 4-row groups each 64x4 workgroup walks in x (0 or 1: one,
  * at most 16) and remap the tile order (1, the default: each XCD takes a
- * contiguous band of tiles; 0: round-robin order); ty and nt do not apply.
+ * contiguous band of tiles; 0: round-robin order); nt = 1 (velocity pass
+ * only) makes each wavefront walk xchunk consecutive rows with its
+ * x-stencil operands in register windows instead; ty does not apply.
  * 3D: workgroup
  * tile of 64 (z) x ty (y) cells, ty in {1,4,8}; nt bit 0 streams the fields
  * touched only at the own cell with non-temporal loads/stores; nt bit 1

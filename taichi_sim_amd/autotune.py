@@ -29,6 +29,7 @@ XCHUNK = (1, 2, 4, 8, 16, 32)
 PLAIN = (4, 0, 0, 0)      # xchunk 0: one cell per thread
 REMAP = (0, 1, 2)
 ROWS_2D = (1, 2, 4, 8)
+MARCH_ROWS_2D = (2, 4, 8, 16)
 REMAP_2D = (0, 1)
 BATCH_ROWS = (1, 2, 4, 8)
 BATCH_GROUP_MB = (0, 64, 128, 224, 320)
@@ -77,7 +78,10 @@ def candidates_for(g, name="stress"):
     of ``g``; the 3D velocity pass adds the LDS-staged shapes (nt bit 1, ty >=
     2 * FD order)."""
     if g.ndim == 2:
-        return [(4, 0, rows, rm) for rows in ROWS_2D for rm in REMAP_2D]
+        out = [(4, 0, rows, rm) for rows in ROWS_2D for rm in REMAP_2D]
+        if name == "velocity":      # the x-row march with register windows (nt = 1)
+            out += [(4, 1, rows, rm) for rows in MARCH_ROWS_2D for rm in REMAP_2D]
+        return out
     out = [PLAIN] + [(ty, nt, xc, rm) for ty in TY for nt in NT for xc in XCHUNK for rm in REMAP]
     if name == "velocity":
         out += [(ty, 2 | nt, xc, rm) for ty in TY if ty >= 2 * g.ord for nt in NT for xc in XCHUNK for rm in REMAP]

@@ -1003,8 +1003,9 @@ int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap
                                 "xchunk >= 1 and ty >= 2 * FD order");
   if (kernel == 2 && (ty == 1 || nt != 0 || xchunk < 1))
     return s->fail(VSE_ERR_ARG, "fused-step tuning: ty in {4,8}, nt 0, xchunk >= 1");
-  if (s->d.ndim == 2 && (kernel == 2 || xchunk > 16))
-    return s->fail(VSE_ERR_ARG, "2D tuning: kernel 0 or 1, xchunk (row groups per workgroup) 0 .. 16");
+  if (s->d.ndim == 2 && (kernel == 2 || xchunk > 16 || nt > 1 || (nt == 1 && kernel != 1)))
+    return s->fail(VSE_ERR_ARG, "2D tuning: kernel 0 or 1, xchunk (row groups per workgroup) 0 .. 16, "
+                                "nt 1 (x-row march) for the velocity pass only");
   OnDevice on_dev(s->device);
   s->tune[kernel] = KTune{ty, nt, xchunk, remap};
   s->drop_graph();

@@ -160,122 +160,138 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
   }
 }
 
-// All loads of a cell are issued before its arithmetic: raw stencil operands
-// go into registers and the sums run afterwards, in the same order, so the
-// results are the same; a division right after a load would otherwise make
-// the wavefront wait on it before issuing the loads that follow.
-// s: the step being advanced (the Taichi-variant source row).
+// The stencil operands of one velocity cell: vx, vy at the cell, sigma_xx
+// along x and sigma_xy along y for vx (D-), sigma_xy along x and sigma_yy
+// along y for vy (D+), and the four densities of the vy point.
+template <int ORD>
+struct VelOps2 {
+  float vxn, vyn;
+  float xx_p[ORD], xx_m[ORD], xy_p[ORD], xy_m[ORD];
+  float yx_p[ORD], yx_m[ORD], yy_p[ORD], yy_m[ORD];
+  float r00, r10, r11, r01;
+};
+
+// The velocity update of one cell (x, y) < (nx, ny) from its operands: the
+// CPML band loads, the two updates, the Dirichlet rules and the stores;
+// returns |v|^2 as float bits.  s: the step being advanced (the
+// Taichi-variant source row).
 template <int VAR, bool VISCO, int ORD, bool FAST = false>
-__device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, int y, int s) {
+__device__ __forceinline__ unsigned velocity2d_apply(const StepArgs &A, int x, int y, int s, VelOps2<ORD> &o) {
   const int nx = A.nx, ny = A.ny;
-  const bool valid = x < nx && y < ny;
   constexpr int last = ORD - 1, off = ORD - 1;
   const long id = (long)x * ny + y;
   const float dt = A.dt;
-  unsigned v2bits = 0;
-  if (valid) {
-    const bool in_vx = x >= last + 1 && x < nx - last && y >= last + 1 && y < ny - last;
-    const bool in_vy = x >= last && x < nx - last - 1 && y >= last && y < ny - last - 1;
-    const int jx = x - off, jy = y - off;
-    const int bx = band_index(A.ax[0], jx), by = band_index(A.ax[1], jy);
-    const long pxi = (long)bx * ny + y, pyi = (long)x * A.ax[1].nb + by;
-    // ---- loads: every stencil operand unconditionally through the field
-    // buffers (reads outside the grid give 0; operands outside an update's
-    // interior are loaded but not used), the CPML band and source map only
-    // where they apply
-    const FieldBuf Bvx(A.vx, A.ncell), Bvy(A.vy, A.ncell), Bsxx(A.sxx, A.ncell), Bsxy(A.sxy, A.ncell),
-        Bsyy(A.syy, A.ncell), Brho(A.rho, A.ncell);
-    const int i = (int)id;
-    float vxn = Bvx[i], vyn = Bvy[i];
-    float xx_p[ORD], xx_m[ORD], xy_p[ORD], xy_m[ORD];     // vx: sxx along x, sxy along y (D-)
-    float yx_p[ORD], yx_m[ORD], yy_p[ORD], yy_m[ORD];     // vy: sxy along x, syy along y (D+)
+  const bool in_vx = x >= last + 1 && x < nx - last && y >= last + 1 && y < ny - last;
+  const bool in_vy = x >= last && x < nx - last - 1 && y >= last && y < ny - last - 1;
+  const int jx = x - off, jy = y - off;
+  const int bx = band_index(A.ax[0], jx), by = band_index(A.ax[1], jy);
+  const long pxi = (long)bx * ny + y, pyi = (long)x * A.ax[1].nb + by;
+  float vxn = o.vxn, vyn = o.vyn;
+  float q1 = 0.f, q2 = 0.f, q3 = 0.f, q4 = 0.f;
+  int is = -1;
+  if (in_vx) {
+    if (bx >= 0) q1 = A.psi[P_DSXX_DX][pxi];
+    if (by >= 0) q2 = A.psi[P_DSXY_DY][pyi];
+  }
+  if (in_vy) {
+    if (bx >= 0) q3 = A.psi[P_DSXY_DX][pxi];
+    if (by >= 0) q4 = A.psi[P_DSYY_DY][pyi];
+    if constexpr (VAR == VSE_VAR_TAICHI) is = A.src_map[id];
+  }
+  // ---- vx (wgsl2d :998-1025)
+  if (in_vx) {
+    float d1 = 0.f, d2 = 0.f;
 #pragma unroll
     for (int c = 0; c < ORD; c++) {
-      xx_p[c] = Bsxx[i + c * ny];
-      xx_m[c] = Bsxx[i - (c + 1) * ny];
-      xy_p[c] = Bsxy[i + c];
-      xy_m[c] = Bsxy[i - c - 1];
-      yx_p[c] = Bsxy[i + (c + 1) * ny];
-      yx_m[c] = Bsxy[i - c * ny];
-      yy_p[c] = Bsyy[i + c + 1];
-      yy_m[c] = Bsyy[i - c];
+      d1 += dterm2<VAR, FAST, 0>(A, c, o.xx_p[c], o.xx_m[c]);
+      d2 += dterm2<VAR, FAST, 1>(A, c, o.xy_p[c], o.xy_m[c]);
     }
-    // in the vy range x+1 and y+1 are inside the grid
-    const float r00 = Brho[i], r10 = Brho[i + ny], r11 = Brho[i + ny + 1], r01 = Brho[i + 1];
-    float q1 = 0.f, q2 = 0.f, q3 = 0.f, q4 = 0.f;
-    int is = -1;
-    if (in_vx) {
-      if (bx >= 0) q1 = A.psi[P_DSXX_DX][pxi];
-      if (by >= 0) q2 = A.psi[P_DSXY_DY][pyi];
+    if (bx >= 0) {
+      const float pn = A.ax[0].b[jx] * q1 + A.ax[0].a[jx] * d1;
+      A.psi[P_DSXX_DX][pxi] = pn;
+      d1 = kdiv<FAST>(d1, A.ax[0].k, A.ax[0].ki, jx) + pn;
     }
-    if (in_vy) {
-      if (bx >= 0) q3 = A.psi[P_DSXY_DX][pxi];
-      if (by >= 0) q4 = A.psi[P_DSYY_DY][pyi];
-      if constexpr (VAR == VSE_VAR_TAICHI) is = A.src_map[id];
+    if (by >= 0) {
+      const float pn = A.ax[1].b[jy] * q2 + A.ax[1].a[jy] * d2;
+      A.psi[P_DSXY_DY][pyi] = pn;
+      d2 = kdiv<FAST>(d2, A.ax[1].k, A.ax[1].ki, jy) + pn;
     }
-    // ---- vx (wgsl2d :998-1025)
-    if (in_vx) {
-      float d1 = 0.f, d2 = 0.f;
+    if (o.r00 > 0.f) {
+      if constexpr (VAR == VSE_VAR_TAICHI) vxn += (d1 + d2) * dt / o.r00;
+      else vxn = (d1 + d2) * dt / o.r00 + vxn;
+    }
+  } else if constexpr (VAR == VSE_VAR_TAICHI) {
+    vxn = 0.f;     // per-component Dirichlet (sim_taichi_elastic.py:266-268)
+  }
+  // ---- vy (wgsl2d :1027-1057; sim_taichi_elastic.py:270-304)
+  if (in_vy) {
+    float d1 = 0.f, d2 = 0.f;
 #pragma unroll
-      for (int c = 0; c < ORD; c++) {
-        d1 += dterm2<VAR, FAST, 0>(A, c, xx_p[c], xx_m[c]);
-        d2 += dterm2<VAR, FAST, 1>(A, c, xy_p[c], xy_m[c]);
-      }
-      if (bx >= 0) {
-        const float pn = A.ax[0].b[jx] * q1 + A.ax[0].a[jx] * d1;
-        A.psi[P_DSXX_DX][pxi] = pn;
-        d1 = kdiv<FAST>(d1, A.ax[0].k, A.ax[0].ki, jx) + pn;
-      }
-      if (by >= 0) {
-        const float pn = A.ax[1].b[jy] * q2 + A.ax[1].a[jy] * d2;
-        A.psi[P_DSXY_DY][pyi] = pn;
-        d2 = kdiv<FAST>(d2, A.ax[1].k, A.ax[1].ki, jy) + pn;
-      }
-      if (r00 > 0.f) {
-        if constexpr (VAR == VSE_VAR_TAICHI) vxn += (d1 + d2) * dt / r00;
-        else vxn = (d1 + d2) * dt / r00 + vxn;
-      }
-    } else if constexpr (VAR == VSE_VAR_TAICHI) {
-      vxn = 0.f;     // per-component Dirichlet (sim_taichi_elastic.py:266-268)
+    for (int c = 0; c < ORD; c++) {
+      d1 += dterm2<VAR, FAST, 0>(A, c, o.yx_p[c], o.yx_m[c]);
+      d2 += dterm2<VAR, FAST, 1>(A, c, o.yy_p[c], o.yy_m[c]);
     }
-    // ---- vy (wgsl2d :1027-1057; sim_taichi_elastic.py:270-304)
-    if (in_vy) {
-      float d1 = 0.f, d2 = 0.f;
-#pragma unroll
-      for (int c = 0; c < ORD; c++) {
-        d1 += dterm2<VAR, FAST, 0>(A, c, yx_p[c], yx_m[c]);
-        d2 += dterm2<VAR, FAST, 1>(A, c, yy_p[c], yy_m[c]);
-      }
-      if (bx >= 0) {
-        const float pn = A.ax[0].bh[jx] * q3 + A.ax[0].ah[jx] * d1;
-        A.psi[P_DSXY_DX][pxi] = pn;
-        d1 = kdiv<FAST>(d1, A.ax[0].kh, A.ax[0].khi, jx) + pn;
-      }
-      if (by >= 0) {
-        const float pn = A.ax[1].bh[jy] * q4 + A.ax[1].ah[jy] * d2;
-        A.psi[P_DSYY_DY][pyi] = pn;
-        d2 = kdiv<FAST>(d2, A.ax[1].kh, A.ax[1].khi, jy) + pn;
-      }
-      if constexpr (VAR == VSE_VAR_TAICHI) {
-        const float rho = 0.25f * (r01 + r00 + r11 + r10);
-        if (rho > 0.f) {
-          float vs = 0.f;
-          if (is != -1 && s < A.src_rows) vs = A.src[(long)s * A.src_cols + is] * dt / rho;
-          vyn += vs + (d1 + d2) * dt / rho;
-        }
-      } else {
-        const float rho = 0.25f * (r00 + r10 + r11 + r01);
-        if (rho > 0.f) vyn = (d1 + d2) * dt / rho + vyn;
+    if (bx >= 0) {
+      const float pn = A.ax[0].bh[jx] * q3 + A.ax[0].ah[jx] * d1;
+      A.psi[P_DSXY_DX][pxi] = pn;
+      d1 = kdiv<FAST>(d1, A.ax[0].kh, A.ax[0].khi, jx) + pn;
+    }
+    if (by >= 0) {
+      const float pn = A.ax[1].bh[jy] * q4 + A.ax[1].ah[jy] * d2;
+      A.psi[P_DSYY_DY][pyi] = pn;
+      d2 = kdiv<FAST>(d2, A.ax[1].kh, A.ax[1].khi, jy) + pn;
+    }
+    if constexpr (VAR == VSE_VAR_TAICHI) {
+      const float rho = 0.25f * (o.r01 + o.r00 + o.r11 + o.r10);
+      if (rho > 0.f) {
+        float vs = 0.f;
+        if (is != -1 && s < A.src_rows) vs = A.src[(long)s * A.src_cols + is] * dt / rho;
+        vyn += vs + (d1 + d2) * dt / rho;
       }
     } else {
-      if constexpr (VAR == VSE_VAR_WGSL) vxn = 0.f;   // the vy else zeroes both (wgsl2d :1054-1057)
-      vyn = 0.f;
+      const float rho = 0.25f * (o.r00 + o.r10 + o.r11 + o.r01);
+      if (rho > 0.f) vyn = (d1 + d2) * dt / rho + vyn;
     }
-    A.vx[id] = vxn;
-    A.vy[id] = vyn;
-    v2bits = __float_as_uint(vxn * vxn + vyn * vyn);
+  } else {
+    if constexpr (VAR == VSE_VAR_WGSL) vxn = 0.f;   // the vy else zeroes both (wgsl2d :1054-1057)
+    vyn = 0.f;
   }
-  return v2bits;
+  A.vx[id] = vxn;
+  A.vy[id] = vyn;
+  return __float_as_uint(vxn * vxn + vyn * vyn);
+}
+
+// All loads of a cell are issued before its arithmetic: raw stencil operands
+// go into registers and the sums run afterwards, in the same order, so the
+// results are the same; a division right after a load would otherwise make
+// the wavefront wait on it before issuing the loads that follow.  Every
+// stencil operand is loaded unconditionally through the field buffers (reads
+// outside the grid give 0; operands outside an update's interior are loaded
+// but not used); the CPML band and source map only where they apply.
+template <int VAR, bool VISCO, int ORD, bool FAST = false>
+__device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, int y, int s) {
+  const int nx = A.nx, ny = A.ny;
+  if (x >= nx || y >= ny) return 0u;
+  const FieldBuf Bvx(A.vx, A.ncell), Bvy(A.vy, A.ncell), Bsxx(A.sxx, A.ncell), Bsxy(A.sxy, A.ncell),
+      Bsyy(A.syy, A.ncell), Brho(A.rho, A.ncell);
+  const int i = (int)((long)x * ny + y);
+  VelOps2<ORD> o;
+  o.vxn = Bvx[i];
+  o.vyn = Bvy[i];
+#pragma unroll
+  for (int c = 0; c < ORD; c++) {
+    o.xx_p[c] = Bsxx[i + c * ny];
+    o.xx_m[c] = Bsxx[i - (c + 1) * ny];
+    o.xy_p[c] = Bsxy[i + c];
+    o.xy_m[c] = Bsxy[i - c - 1];
+    o.yx_p[c] = Bsxy[i + (c + 1) * ny];
+    o.yx_m[c] = Bsxy[i - c * ny];
+    o.yy_p[c] = Bsyy[i + c + 1];
+    o.yy_m[c] = Bsyy[i - c];
+  }
+  // in the vy range x+1 and y+1 are inside the grid
+  o.r00 = Brho[i]; o.r10 = Brho[i + ny]; o.r11 = Brho[i + ny + 1]; o.r01 = Brho[i + 1];
+  return velocity2d_apply<VAR, VISCO, ORD, FAST>(A, x, y, s, o);
 }
 
 // Tile of a 1D launch over nty (y) x ntx (x) tiles of BY x BX cells, y-tile
@@ -370,6 +386,71 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A, int nty, int
   wave_max_to_slot(v2, A.v2slots + blockIdx.x);
 }
 
+// The velocity pass with each wavefront walking `rows` consecutive x-rows of
+// its 64-cell y column (a workgroup: BX wavefronts, BX*rows rows).  The
+// x-stencil operands (sigma_xx for vx, sigma_xy for vy) and the x+1 densities
+// move through register windows, so a row step loads one new row of each
+// instead of 2*ORD; the y-stencil operands are loaded as in velocity2d_cell.
+// Same operands (a window row outside the grid reads 0 through the field
+// buffer, as the per-cell loads do), same arithmetic (velocity2d_apply):
+// bit-identical to k_velocity2d.
+template <int VAR, bool VISCO, int ORD, bool FAST = false>
+__global__ __launch_bounds__(BX * BY) void k_velocity2d_march(StepArgs A, int nty, int remap, int rows, int i) {
+  const Tile2 t = tile2d(nty, remap);
+  const int nx = A.nx, ny = A.ny;
+  const int y = t.ty * BY + threadIdx.x;
+  const int x0 = (t.tx * BX + threadIdx.y) * rows;
+  const int s = *A.step + i;
+  unsigned v2 = 0;
+  if (y < ny && x0 < nx) {
+    const int x1 = min(nx, x0 + rows);
+    const FieldBuf Bvx(A.vx, A.ncell), Bvy(A.vy, A.ncell), Bsxx(A.sxx, A.ncell), Bsxy(A.sxy, A.ncell),
+        Bsyy(A.syy, A.ncell), Brho(A.rho, A.ncell);
+    // windows over x: wxx[j] = sigma_xx(x - ORD + j), wxy[j] = sigma_xy(x - ORD + 1 + j)
+    float wxx[2 * ORD], wxy[2 * ORD];
+    const int i0 = (int)((long)x0 * ny + y);
+#pragma unroll
+    for (int j = 0; j < 2 * ORD - 1; j++) {
+      wxx[j + 1] = Bsxx[i0 + (j - ORD) * ny];
+      wxy[j + 1] = Bsxy[i0 + (j - ORD + 1) * ny];
+    }
+    float rn0 = Brho[i0], rn1 = Brho[i0 + 1];          // rho(x, y), rho(x, y+1) of the next row
+#pragma unroll 1
+    for (int x = x0; x < x1; x++) {
+      const int ic = (int)((long)x * ny + y);
+#pragma unroll
+      for (int j = 0; j < 2 * ORD - 1; j++) { wxx[j] = wxx[j + 1]; wxy[j] = wxy[j + 1]; }
+      wxx[2 * ORD - 1] = Bsxx[ic + (ORD - 1) * ny];
+      wxy[2 * ORD - 1] = Bsxy[ic + ORD * ny];
+      VelOps2<ORD> o;
+      o.vxn = Bvx[ic];
+      o.vyn = Bvy[ic];
+#pragma unroll
+      for (int c = 0; c < ORD; c++) {
+        o.xx_p[c] = wxx[ORD + c];
+        o.xx_m[c] = wxx[ORD - 1 - c];
+        o.xy_p[c] = c == 0 ? wxy[ORD - 1] : Bsxy[ic + c];
+        o.xy_m[c] = Bsxy[ic - c - 1];
+        o.yx_p[c] = wxy[ORD + c];
+        o.yx_m[c] = wxy[ORD - 1 - c];
+        o.yy_p[c] = Bsyy[ic + c + 1];
+        o.yy_m[c] = Bsyy[ic - c];
+      }
+      o.r00 = rn0;
+      o.r01 = rn1;
+      rn0 = Brho[ic + ny];
+      rn1 = Brho[ic + ny + 1];
+      o.r10 = rn0;
+      o.r11 = rn1;
+      unsigned u = velocity2d_apply<VAR, VISCO, ORD, FAST>(A, x, y, s, o);
+      source_cell<VAR>(A, x, y, s, u);
+      v2 = v2 > u ? v2 : u;
+    }
+  }
+  if (blockIdx.x == 0 && !A.rec_after) record_receivers(A, s);
+  wave_max_to_slot(v2, A.v2slots + blockIdx.x);
+}
+
 // Receivers of a velocity channel, after the velocity pass that wrote it
 // (one workgroup; step *A.step + i as in k_velocity2d).
 __global__ __launch_bounds__(256) void k_record2d(StepArgs A, int i) { record_receivers(A, *A.step + i); }
@@ -447,11 +528,13 @@ hipError_t launch_ord(bool stress, int ord, const StepArgs &A, const KTune &t, i
   const int nty = (A.ny + BY - 1) / BY, ntx = ((A.nx + BX - 1) / BX + rows - 1) / rows;
   const dim3 grid((unsigned)(nty * ntx));
   const int rm = t.remap;
+  const bool march = t.nt == 1;      // velocity pass: x-row march with register windows
   const bool l3 = VISCO && VAR == VSE_VAR_WGSL && A.L == 3;
 #define VSE_CASE(O)                                                                                                \
   case O:                                                                                                          \
     if (stress && l3) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O, 3, FAST>), grid, block, 0, st, A, nty, rm, rows); \
     else if (stress) hipLaunchKernelGGL((k_stress2d<VAR, VISCO, O, 0, FAST>), grid, block, 0, st, A, nty, rm, rows); \
+    else if (march) hipLaunchKernelGGL((k_velocity2d_march<VAR, VISCO, O, FAST>), grid, block, 0, st, A, nty, rm, rows, i); \
     else hipLaunchKernelGGL((k_velocity2d<VAR, VISCO, O, FAST>), grid, block, 0, st, A, nty, rm, rows, i);          \
     break;
   switch (ord) {

@@ -192,9 +192,10 @@ def test_2d_launch_shapes_bit_identical(variant, channel):
     att = attenuation_from_config(cfg)
     g = VSESim.from_simulator(sim, variant=variant, att=att, channel=channel)
     ref = None
-    for rows, rm in [(1, 1), (1, 0), (2, 1), (3, 0), (4, 1), (8, 0), (16, 1)]:
+    for rows, rm, march in [(1, 1, 0), (1, 0, 0), (2, 1, 0), (3, 0, 0), (4, 1, 0), (8, 0, 0), (16, 1, 0),
+                            (1, 1, 1), (2, 0, 1), (5, 1, 1), (8, 1, 1), (13, 0, 1), (16, 1, 1)]:
         g.set_tuning(0, 4, 0, rows, rm)
-        g.set_tuning(1, 4, 0, 17 - rows, 1 - rm)
+        g.set_tuning(1, 4, march, 17 - rows, 1 - rm)
         g.reset()
         g.run(60, check_every=20)
         out = [g.field(k) for k in ("vx", "vy", "sxx", "syy", "sxy")] + [g.traces(), np.array([g.v2max()])]
@@ -208,6 +209,8 @@ def test_2d_launch_shapes_bit_identical(variant, channel):
                 np.testing.assert_array_equal(a, b)
     with pytest.raises(VSEError):
         g.set_tuning(0, 4, 0, 17, 1)
+    with pytest.raises(VSEError):
+        g.set_tuning(0, 4, 1, 4, 1)          # the x-row march is a velocity-pass shape
     g.close()
 
 

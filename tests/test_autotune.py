@@ -58,7 +58,11 @@ def test_two_dimensional_handles_tune_rows_and_tile_order(tmp_path):
     g.ndim, g.shape = 2, (2048, 2048)
     best = autotune.tune(g, steps=2, cache=str(tmp_path / "t.json"))
     assert tuple(best["stress"][:4]) == best_shape and g.tuning[0] == best_shape
-    assert {c[:4] for c in (e[1:5] for e in best["table"])} == set(cands)
+    vel = autotune.candidates_for(g, "velocity")
+    assert {e[1:5] for e in best["table"] if e[0] == "stress"} == set(cands)
+    assert {e[1:5] for e in best["table"] if e[0] == "velocity"} == set(vel)
+    # the velocity pass also tries the x-row march (nt = 1); the stress pass does not
+    assert {c[1] for c in vel} == {0, 1} and {c[1] for c in cands} == {0}
 
 
 class FakeBatch:

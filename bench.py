@@ -540,7 +540,7 @@ def measure_adjoint(n_shots, fd_dirs=0, store_all=False):
     time of the checkpointed forward + recompute + adjoint over the plain
     forward, per shot; optional finite-difference directions and the
     every-level-stored variant."""
-    from taichi_sim_amd import workloads
+    from taichi_sim_amd import autotune, workloads
     from taichi_sim_amd.adjoint import VSEGrad, chain_to_cells
     from taichi_sim_amd.backend import attenuation_from_config
     from taichi_sim_amd.engine import VSESim
@@ -552,6 +552,8 @@ def measure_adjoint(n_shots, fd_dirs=0, store_all=False):
     sim._emission_laws = laws
     att = attenuation_from_config(cfg)
     g = VSESim.from_simulator(sim, variant="wgsl", visco=True, att=att)
+    # the plain forward, the ratio's denominator, runs at its tuned launch shapes
+    fwd_shapes = {k: list(v[:4]) for k, v in autotune.tune(g, steps=3, cache=None).items() if k != "table"}
     start = [np.asarray(a, np.float32).copy() for a in (sim._rho_grid_vx, sim._cp_grid_vx, sim._cs_grid_vx)]
     eng = VSEGrad(g)
     seg = eng.seg_len
@@ -700,6 +702,7 @@ def measure_adjoint(n_shots, fd_dirs=0, store_all=False):
         "fd_rel_err_trace_energy": [float(f"{e:.2e}") for e in fd_rel],
         "gradient_accumulation": accumulation,
         "every_state_stored": stored,
+        "forward_launch_shapes": fwd_shapes,
     }
 
 

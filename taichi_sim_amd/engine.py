@@ -215,7 +215,9 @@ class VSESim:
         return tuple(float(v) for v in ms)
 
     def set_tuning(self, kernel, ty, nt, xchunk, remap=0):
-        """Launch shape of the 3D stress (kernel 0) or velocity (1) pass."""
+        """Launch shape (ty, nt, xchunk, remap) of the stress (kernel 0) or
+        velocity (1) pass, or of the fused 3D step (2); include/vse.h gives each
+        field's meaning in 2D and 3D.  Results are identical for every shape."""
         self._check(self._lib.vse_set_tuning(self._h, int(kernel), int(ty), int(nt), int(xchunk), int(remap)))
 
     def set_schedule(self, sched):

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <mutex>
 #include <string>
@@ -33,7 +34,7 @@ struct vse_sim {
   size_t bytes = 0;
   // device allocations as (address handed out, address hipMalloc returned)
   std::vector<std::pair<void *, void *>> allocs;
-  unsigned n_allocs = 0;
+  static std::atomic<unsigned> g_allocs;    // device arrays allocated so far (the base skew)
   // mechanism stride of the memory variables in floats: mechanism l of
   // component c is r[c] + l * rs (ncell plus a skew, see alloc())
   long rs = 0;
@@ -95,12 +96,14 @@ struct vse_sim {
   // modulo 4 KiB: the passes stream 10-30 arrays at the same cell index, and
   // with every base on the same 4 KiB alignment those streams meet in the
   // same HBM channels (measured on a 30-array in-place mix at 512^3: 5.0 TB/s
-  // with aligned bases, 5.2 TB/s skewed).
+  // with aligned bases, 5.2 TB/s skewed).  The count runs over every
+  // simulation of the process, so the same field of batch members that run
+  // together does not share an alignment either.
   template <typename T>
   int alloc(T **p, size_t n) {
     constexpr size_t kSkewStep = 256, kSkewSpan = 16;
     const size_t nb = std::max<size_t>(n, 1) * sizeof(T);
-    const size_t skew = kSkewStep * (n_allocs++ % kSkewSpan);
+    const size_t skew = kSkewStep * (g_allocs.fetch_add(1, std::memory_order_relaxed) % kSkewSpan);
     void *q = nullptr;
     int rc = hip(hipMalloc(&q, nb + kSkewStep * kSkewSpan), "hipMalloc");
     if (rc) return rc;
@@ -140,6 +143,8 @@ struct vse_sim {
     return false;
   }
 };
+
+std::atomic<unsigned> vse_sim::g_allocs{0};
 
 static int check_handle(vse_sim *s) { return s ? 0 : VSE_ERR_ARG; }
 

@@ -27,6 +27,9 @@ model with sqrt(N) checkpointing.  value = time(checkpointed forward +
 recompute + adjoint) / time(plain forward), per shot, device time.
 
 Prints one JSON line.  value = Gcell-updates/s of the whole job (one GPU).
+The default line also carries the rest of the metric and the 2D line as
+extra keys: shots_per_s (batched FMC against the per-shot loop),
+adjoint_over_forward (one shot, sqrt(N) checkpoints) and visco2d.
 The headline runs the FAST arithmetic (FP32 with the FD coefficients
 pre-divided by the spacing, within 1e-5 rel-L2 of the reference's operation
 order); STRICT, the reference's order term for term, is reported beside it.
@@ -98,44 +101,49 @@ def bytes_per_cell_2d(g, L):
 
 def bench_2d(args):
     """2D viscoelastic time loop at 2048^2 (see module doc)."""
+    emit(measure_2d(args.n2d, args.steps, args.warmup, args.math2d, args.profile_steps, args.cpu_steps))
+
+
+def measure_2d(n, steps, warmup, math, profile_steps, cpu_steps):
+    """The 2D line: Gcell-updates/s of the n^2 viscoelastic loop, its stress
+    pass against the roofline and the CPU oracle on a bounded sample."""
     from taichi_sim_amd import autotune, workloads
     from taichi_sim_amd.backend import attenuation_from_config
     from taichi_sim_amd.engine import VSESim
     from taichi_sim_amd.sim_support.simulator import Simulator
 
-    n = args.n2d
-    cfg = workloads.visco_2d_array(n=n, steps=args.warmup + args.steps)
+    cfg = workloads.visco_2d_array(n=n, steps=warmup + steps)
     sim = Simulator(cfg)
     att = attenuation_from_config(cfg)
     g = VSESim.from_simulator(sim, variant="wgsl", att=att)
-    g.set_math(args.math2d)
+    g.set_math(math)
     ncell = int(np.prod(g.shape))
     print(f"# setup done: grid {g.shape}, {g.device_bytes() / 2**30:.2f} GiB on {g.device_name()}",
           file=sys.stderr, flush=True)
     best = autotune.tune(g, steps=3, cache=None)           # before the timed region; it advances the state
     shapes = {k: list(best[k]) for k in ("stress", "velocity")}
     print(f"# launch shapes (ty, nt, rows, remap, step ms): {shapes}", file=sys.stderr, flush=True)
-    if args.warmup:
+    if warmup:
         g.reset()
-        g.run(args.warmup, check_every=args.warmup)
+        g.run(warmup, check_every=warmup)
     g.reset()
-    g.run(min(args.steps, 256), check_every=256)           # capture the graph outside the timed region
+    g.run(min(steps, 256), check_every=256)           # capture the graph outside the timed region
     g.reset()
     t0 = time.perf_counter()
-    dev_ms = g.run(args.steps, check_every=500, timed=True)
+    dev_ms = g.run(steps, check_every=500, timed=True)
     wall = time.perf_counter() - t0
-    gcell = ncell * args.steps / wall / 1e9
-    k_ms = g.profile(args.profile_steps)
+    gcell = ncell * steps / wall / 1e9
+    k_ms = g.profile(profile_steps)
     b_step, b_stress, b_vel = bytes_per_cell_2d(g, g.n_sls)
     achieved = b_stress * ncell / (k_ms[0] * 1e-3) / 1e9
     g.close()
-    print(f"# timed {args.steps} steps: wall {wall:.3f} s, device {dev_ms / 1e3:.3f} s", file=sys.stderr, flush=True)
+    print(f"# timed {steps} steps: wall {wall:.3f} s, device {dev_ms / 1e3:.3f} s", file=sys.stderr, flush=True)
 
     cpu = None
-    if args.cpu_steps > 0:
+    if cpu_steps > 0:
         import oracle
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        steps_cpu = 10 * args.cpu_steps
+        steps_cpu = 10 * cpu_steps
         o = oracle.Oracle(sim, variant="wgsl", att=att, n_threads=threads)
         t = time.perf_counter()
         o.run(steps_cpu)
@@ -148,12 +156,12 @@ def bench_2d(args):
     line = {
         "metric": "Gcell-upd/s (2D viscoelastic Zener L=3 FP32 forward)",
         "value": round(gcell, 4), "unit": "Gcell-upd/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 4),
+        "steps": steps, "warmup": warmup, "ms_per_step": round(1e3 * wall / steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"2D viscoelastic Zener FD {n}^2 (incl. 20-cell CPML), L=3 Q=50, steel plate with "
                                f"16 seeded voids, 64-element array with a focusing law, 64 receivers",
                    "grid": [n, n], "n_sls": 3, "fd_order": 2, "parallelism": "single GPU",
-                   "math": args.math2d},
+                   "math": math},
         "launch_shapes": shapes,
         "hbm_frac_step": round(gcell * 1e9 * b_step / (HBM_PEAK_GBS * 1e9), 4),
         "bytes_per_cell_step": round(b_step, 2),
@@ -167,7 +175,7 @@ def bench_2d(args):
                      "kernel": "stress2d", "bytes_per_cell": round(b_stress, 2)},
         "cpu_baseline": cpu,
     }
-    emit(line)
+    return line
 
 
 def shape_sweep(g, k):
@@ -221,13 +229,18 @@ def main():
     ap.add_argument("--store-all", type=int, default=1,
                     help="also time the gradient with every state stored, no recompute (--workload adjoint)")
     ap.add_argument("--adj-shots", type=int, default=8, help="shots (--workload adjoint)")
+    ap.add_argument("--adj-schedules", type=int, default=0,
+                    help="--workload adjoint: also time shot 0's reverse sweep under each schedule "
+                         "(two kernels per step, fused tile shapes 0..2), with sqrt(N) checkpoints and "
+                         "with every state stored, and check that the gradients have the same bits")
     ap.add_argument("--sizes", default="512,1024,1536,2048",
                     help="specimen grid sizes (--workload shots), comma separated")
     ap.add_argument("--shots", type=int, default=16, help="array elements = FMC shots per specimen (--workload shots)")
     ap.add_argument("--group-sweep", action="store_true",
                     help="also time the batch at cache-group budgets 0..320 MB (--workload shots)")
     ap.add_argument("--extras", type=int, default=1,
-                    help="3D line: also measure batched FMC shots/s and the adjoint/forward ratio (0: skip)")
+                    help="3D line: also measure batched FMC shots/s, the adjoint/forward ratio and the 2D "
+                         "2048^2 line (key visco2d, reference operation order) (0: skip)")
     ap.add_argument("--shots-window", type=int, default=240,
                     help="steps of each shot timed for the 3D line's shots/s (batch and per-shot loop)")
     ap.add_argument("--copy-only", action="store_true",
@@ -377,6 +390,11 @@ def main():
                                "window_steps": s["config"]["window_steps"], "shot_steps": 3000,
                                "cache_groups": s["config"]["cache_groups"], "tuning": s["tuning"],
                                "workload": s["config"]["workload"]}
+        c2 = measure_2d(2048, 2000, 10, "strict", 10, 0)
+        line["visco2d"] = {"value": c2["value"], "unit": c2["unit"], "ms_per_step": c2["ms_per_step"],
+                           "steps": c2["steps"], "math": "strict", "kernel_ms": c2["kernel_ms"],
+                           "roofline": c2["roofline"], "hbm_frac_step": c2["hbm_frac_step"],
+                           "launch_shapes": c2["launch_shapes"], "workload": c2["config"]["workload"]}
         a = measure_adjoint(1)
         line["adjoint_over_forward"] = {"value": a["value"], "forward_ms": a["forward_ms_per_shot"],
                                         "checkpointed_forward_ms": a["checkpointed_forward_ms_per_shot"],
@@ -532,10 +550,10 @@ def measure_shots(sizes, n_shots, steps, warmup=0, profile_steps=10, group_sweep
 
 def bench_adjoint(args):
     """Adjoint/forward time ratio of the C-ABI gradient (see module doc)."""
-    emit(measure_adjoint(args.adj_shots, args.fd_dirs, args.store_all))
+    emit(measure_adjoint(args.adj_shots, args.fd_dirs, args.store_all, args.adj_schedules))
 
 
-def measure_adjoint(n_shots, fd_dirs=0, store_all=False):
+def measure_adjoint(n_shots, fd_dirs=0, store_all=False, schedules=False):
     """Gradient of the trace misfit on the 1024^2 inversion problem: device
     time of the checkpointed forward + recompute + adjoint over the plain
     forward, per shot; optional finite-difference directions and the
@@ -680,6 +698,29 @@ def measure_adjoint(n_shots, fd_dirs=0, store_all=False):
                   "same_gradient_bits": all(np.array_equal(c_all[key], coef0[key]) for key in coef0)}
         ea.close()
         print(f"# every state stored: {stored}", file=sys.stderr, flush=True)
+    # the reverse sweep of shot 0 under each schedule: two kernels per step (the
+    # default) and the fused step at each tile shape, bit-identical gradients
+    sched = None
+    if schedules:
+        sim.select_law(0)
+        g.load_sources(sim)
+        sched = {}
+        for seg in (0, steps):
+            es = VSEGrad(g, seg_len=seg)
+            es.set_model(*start)
+            ref = None
+            for fused, shape in ((False, 0), (True, 0), (True, 1), (True, 2)):
+                es.set_schedule(fused, shape)
+                es.forward(steps, check_every=500)
+                es.misfit(obs[0])
+                c = es.backward()
+                ref = c if ref is None else ref
+                key = f"{'sqrtN' if seg == 0 else 'every_state'}_{'fused' + str(shape) if fused else 'two_kernels'}"
+                sched[key] = {"backward_ms": round(es.ms["backward"], 2),
+                              "same_gradient_bits": all(np.array_equal(c[k], ref[k]) for k in ref)}
+                print(f"# schedule {key}: {sched[key]}", file=sys.stderr, flush=True)
+            es.close()
+
     g.close()
 
     fwd, gf, bw = (float(np.mean(t)) for t in (t_fwd, t_gfwd, t_bwd))
@@ -702,6 +743,7 @@ def measure_adjoint(n_shots, fd_dirs=0, store_all=False):
         "fd_rel_err_trace_energy": [float(f"{e:.2e}") for e in fd_rel],
         "gradient_accumulation": accumulation,
         "every_state_stored": stored,
+        "reverse_schedules": sched,
         "forward_launch_shapes": fwd_shapes,
     }
 

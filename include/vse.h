@@ -343,6 +343,18 @@ int vse_grad_misfit(vse_grad *g, const float *observed, double *misfit);
 int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms);
 /* Steps per FP64 flush of the FP32 gradient partials, 1 .. 4096 (default 32). */
 int vse_grad_set_flush(vse_grad *g, int flush_every);
+/* Schedule of the reverse sweep.  VSE_ADJ_TWO_PASS (default): two launches
+ * per step, abar5..8 through device memory.  VSE_ADJ_FUSED: one launch per
+ * step, the velocity part recomputed over halo rows and columns of each tile
+ * so that abar5..8 stay in LDS; shape 0..2 picks the tile (0: 8 wavefronts x
+ * 32 rows, 1: 4 x 16, 2: 8 x 24).  Both give the same gradient bits unless a
+ * cell holds two receiver points (then the residual sums in another order);
+ * the fused step is slower on MI355X at 1024^2 (the halo recompute and one
+ * barrier per workgroup at 4 wavefronts per SIMD cost more than the 32 B per
+ * cell and step it saves). */
+#define VSE_ADJ_TWO_PASS 0
+#define VSE_ADJ_FUSED 1
+int vse_grad_set_schedule(vse_grad *g, int schedule, int shape);
 /* FP64 gradient map VSE_GRAD_* (n cells, grid layout) of the last backward. */
 int vse_grad_read(vse_grad *g, int which, double *out);
 /* Field 0..4 = vx, vy, sxx, syy, sxy of the last forward's final state. */

@@ -293,3 +293,42 @@ def test_autograd_function_and_fused_path_agree():
     inner = (slice(4, -4), slice(4, -4))
     assert oracle.rel_l2(g["lambda"][inner], (params[1].grad.numpy() * dcp_dlam)[inner]) < 1e-4
     eng.close()
+
+
+@pytest.mark.parametrize("acc,n,scaled", [(2, 70, False), (2, 90, True), (4, 77, True)])
+def test_fused_reverse_step_matches_two_kernels(acc, n, scaled):
+    # One launch per reverse step (velocity adjoint recomputed over each tile's
+    # halo, abar5..8 kept in LDS, the rest double-buffered) against the two
+    # launches per step through device memory: the same operations on the same
+    # operands, so every gradient map must have the same bits, for every tile
+    # shape, ragged grids, an odd step count (the buffer swap), voids, FD order
+    # 4 (acc 4) and a per-cell tau scale (the scaled-Zener kernels).
+    steps = 101
+    cfg = workloads.visco_2d_array(n=n, steps=steps, n_elem=4, n_voids=2)
+    cfg["simul_params"]["acc"] = acc
+    cfg["simul_params"]["dt"] = 0.8 * cfg["simul_params"]["dt"]
+    sim = Simulator(cfg)
+    att = attenuation_from_config(cfg)
+    true, start = _perturbed(sim)
+    if not scaled:
+        start = start[:3] + (np.ones_like(start[3]),)
+    eng = VSEGrad(VSESim.from_simulator(sim, variant="wgsl", att=att), seg_len=10)
+    eng.set_model(*true)
+    obs = eng.forward(steps)
+    eng.set_model(*start)
+    eng.forward(steps)
+    eng.misfit(obs)
+    print(f"acc {acc}, {sim._nx}x{sim._ny}, tau scale {scaled}: two-kernel reverse step", flush=True)
+    eng.set_schedule(fused=False)
+    ref = eng.backward()
+    assert all(np.abs(ref[k]).max() > 0 for k in GRAD_NAMES)
+    for shape in range(3):
+        print(f"fused reverse step, tile shape {shape}", flush=True)
+        eng.set_schedule(fused=True, shape=shape)
+        got = eng.backward()
+        for k in GRAD_NAMES:
+            assert np.array_equal(got[k], ref[k]), (shape, k)
+    for bad in ((True, 3), (True, -1)):
+        with pytest.raises(VSEError):
+            eng.set_schedule(*bad)
+    eng.close()

@@ -699,7 +699,8 @@ def measure_adjoint(n_shots, fd_dirs=0, store_all=False, schedules=False):
         ea.close()
         print(f"# every state stored: {stored}", file=sys.stderr, flush=True)
     # the reverse sweep of shot 0 under each schedule: two kernels per step (the
-    # default) and the fused step at each tile shape, bit-identical gradients
+    # default), serial or with the next segment recomputed on a second stream,
+    # and the fused step at each tile shape; bit-identical gradients
     sched = None
     if schedules:
         sim.select_law(0)
@@ -709,13 +710,15 @@ def measure_adjoint(n_shots, fd_dirs=0, store_all=False, schedules=False):
             es = VSEGrad(g, seg_len=seg)
             es.set_model(*start)
             ref = None
-            for fused, shape in ((False, 0), (True, 0), (True, 1), (True, 2)):
-                es.set_schedule(fused, shape)
+            for fused, shape, ov in ((False, 0, False), (False, 0, True), (True, 0, True), (True, 1, True),
+                                     (True, 2, True)):
+                es.set_schedule(fused, shape, ov)
                 es.forward(steps, check_every=500)
                 es.misfit(obs[0])
                 c = es.backward()
                 ref = c if ref is None else ref
-                key = f"{'sqrtN' if seg == 0 else 'every_state'}_{'fused' + str(shape) if fused else 'two_kernels'}"
+                key = (f"{'sqrtN' if seg == 0 else 'every_state'}_{'fused' + str(shape) if fused else 'two_kernels'}"
+                       f"{'' if ov else '_serial'}")
                 sched[key] = {"backward_ms": round(es.ms["backward"], 2),
                               "same_gradient_bits": all(np.array_equal(c[k], ref[k]) for k in ref)}
                 print(f"# schedule {key}: {sched[key]}", file=sys.stderr, flush=True)

@@ -354,6 +354,11 @@ int vse_grad_set_flush(vse_grad *g, int flush_every);
  * cell and step it saves). */
 #define VSE_ADJ_TWO_PASS 0
 #define VSE_ADJ_FUSED 1
+/* Or-ed into schedule: with sqrt(N) checkpoints, recompute each segment on the
+ * simulation's stream before its adjoint.  By default the reverse sweep
+ * recomputes segment s-1 on a second stream while the adjoint of segment s
+ * runs (one more segment of levels in device memory); the gradient is the same. */
+#define VSE_ADJ_SERIAL 2
 int vse_grad_set_schedule(vse_grad *g, int schedule, int shape);
 /* FP64 gradient map VSE_GRAD_* (n cells, grid layout) of the last backward. */
 int vse_grad_read(vse_grad *g, int which, double *out);

@@ -1506,6 +1506,14 @@ struct vse_grad {
   int n_rec = 0;                  // receivers of s when the trace buffers were sized
   bool last_seg_stored = false;   // the forward left the last segment's levels in seg[]
   std::vector<DLevel> ckpt, seg;
+  // With sqrt(N) checkpoints the reverse sweep recomputes segment sg - 1 on a
+  // second stream (rstream) while the adjoint of segment sg runs on the
+  // simulation's stream; consecutive segments alternate between seg and seg2.
+  std::vector<DLevel> seg2;
+  int top = 0;                    // last segment of the latest forward (its levels are in seg)
+  bool overlap = true;
+  hipStream_t rstream = nullptr;
+  hipEvent_t ev_go = nullptr, ev_rec[2] = {}, ev_adj[2] = {};
   DLevel bar{};
   float *abar[8] = {};
   // the fused reverse step's second copy of v_bar (vx, vy), of psi_bar's sigma
@@ -1567,7 +1575,10 @@ struct vse_grad {
     return hip(hipMemsetAsync(v.vx, 0, (size_t)level_floats() * sizeof(float), s->stream), "zero level");
   }
   // level j of segment sg (state sg*K + j)
-  const DLevel &lev(int sg, int j) const { return j == 0 ? ckpt[sg] : seg[j - 1]; }
+  const DLevel &lev(int sg, int j) const {
+    if (j == 0) return ckpt[sg];
+    return (!seg2.empty() && ((top - sg) & 1)) ? seg2[j - 1] : seg[j - 1];
+  }
 };
 
 static void make_diff(vse_grad *g, DiffArgs &A) {
@@ -1597,12 +1608,15 @@ static void make_diff(vse_grad *g, DiffArgs &A) {
   A.rec_delay = s->rec_delay; A.traces = g->traces; A.gbar = g->gbar;
 }
 
-// One out-of-place forward step n: level `in` -> level `out` (two launches).
-static int grad_step(vse_grad *g, DiffArgs &A, const DLevel &in, const DLevel &out, int n, bool record) {
+// One out-of-place forward step n: level `in` -> level `out` (two launches
+// on `st`, the simulation's stream by default).
+static int grad_step(vse_grad *g, DiffArgs &A, const DLevel &in, const DLevel &out, int n, bool record,
+                     hipStream_t st = nullptr) {
   A.in = in; A.out = out;
   A.n = n; A.record = record ? 1 : 0;
-  hipError_t e = vse_launch_dstress(A, g->s->stream);
-  if (e == hipSuccess) e = vse_launch_dvelocity(A, g->s->stream);
+  if (!st) st = g->s->stream;
+  hipError_t e = vse_launch_dstress(A, st);
+  if (e == hipSuccess) e = vse_launch_dvelocity(A, st);
   return g->hip(e, "forward step");
 }
 
@@ -1633,6 +1647,17 @@ static void grad_detach(vse_grad *g) {
   if (!s) return;
   OnDevice on_dev(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (g->rstream) {
+    (void)hipStreamSynchronize(g->rstream);
+    (void)hipStreamDestroy(g->rstream);
+    g->rstream = nullptr;
+  }
+  hipEvent_t *evs[5] = {&g->ev_go, &g->ev_rec[0], &g->ev_rec[1], &g->ev_adj[0], &g->ev_adj[1]};
+  for (hipEvent_t *e : evs)
+    if (*e) {
+      (void)hipEventDestroy(*e);
+      *e = nullptr;
+    }
   for (void *p : g->allocs) (void)hipFree(p);
   g->allocs.clear();
   if (g->host_ctrl) (void)hipHostFree(g->host_ctrl);
@@ -1680,6 +1705,15 @@ int vse_grad_create(vse_sim *s, int seg_len, vse_grad **out) {
     if (!rc) rc = g->alloc_level(v);
   for (auto &v : g->seg)
     if (!rc) rc = g->alloc_level(v);
+  if (g->n_ckpt > 1) {     // the second segment buffer, stream and events of the overlapped reverse sweep
+    g->seg2.resize(g->K);
+    for (auto &v : g->seg2)
+      if (!rc) rc = g->alloc_level(v);
+    if (!rc) rc = g->hip(hipStreamCreateWithFlags(&g->rstream, hipStreamNonBlocking), "hipStreamCreate");
+    hipEvent_t *evs[5] = {&g->ev_go, &g->ev_rec[0], &g->ev_rec[1], &g->ev_adj[0], &g->ev_adj[1]};
+    for (hipEvent_t *e : evs)
+      if (!rc) rc = g->hip(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
+  }
   if (!rc) rc = g->alloc_level(g->bar);
   for (int i = 0; i < 8 && !rc; i++) rc = g->alloc(&g->abar[i], s->ncell);
   for (int i = 0; i < 2 && !rc; i++) rc = g->alloc(&g->vbar2[i], s->ncell);
@@ -1759,6 +1793,7 @@ int vse_grad_forward(vse_grad *g, int n_steps, int check_every, float *traces, f
   // The last segment's steps write straight into its segment levels, which
   // the reverse sweep then uses without recomputing them.
   const int last = (n_steps - 1) / g->K;      // index of the last segment
+  g->top = last;                              // its levels go to seg (lev())
   const DLevel *cur = &g->ckpt[0];
   for (int n = 0; n < n_steps && !rc; n++) {
     const int m = n + 1;
@@ -1844,10 +1879,12 @@ static int grad_rec_table(vse_grad *g) {
 
 int vse_grad_set_schedule(vse_grad *g, int schedule, int shape) {
   if (int rc = grad_usable(g)) return rc;
-  if ((schedule != VSE_ADJ_TWO_PASS && schedule != VSE_ADJ_FUSED) || shape < 0 || shape >= VSE_ADJ_SHAPES)
+  const int kernels = schedule & ~VSE_ADJ_SERIAL;
+  if ((kernels != VSE_ADJ_TWO_PASS && kernels != VSE_ADJ_FUSED) || shape < 0 || shape >= VSE_ADJ_SHAPES)
     return VSE_ERR_ARG;
-  g->sched = schedule;
+  g->sched = kernels;
   g->shape = shape;
+  g->overlap = (schedule & VSE_ADJ_SERIAL) == 0;
   return VSE_OK;
 }
 
@@ -1885,10 +1922,42 @@ int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms) 
     if (!rc) rc = g->hip(hipEventRecord(e0, s->stream), "event");
   }
   const int nseg = (N + g->K - 1) / g->K;
+  // Overlapped sweep: segment sg - 1 is recomputed on the second stream while
+  // the adjoint of segment sg runs on the simulation's stream.  Segment sg's
+  // levels are in seg or seg2 by the parity of top - sg; a recompute waits for
+  // the adjoint that last read its buffer (segment sg + 1, event ev_adj), and
+  // an adjoint waits for its segment's recompute (event ev_rec).  The same
+  // kernels on the same inputs: the gradient is the same either way.
+  const bool ov = g->overlap && g->rstream && nseg > 1;
+  DiffArgs Ar = A;                             // the recompute's arguments
+  auto par = [&](int sg) { return (g->top - sg) & 1; };
+  auto recompute = [&](int sg, hipStream_t st) {
+    const int n0 = sg * g->K, n1 = std::min(n0 + g->K, N);
+    int r = 0;
+    for (int j = 0; j < n1 - n0 && !r; j++) r = grad_step(g, Ar, g->lev(sg, j), g->lev(sg, j + 1), n0 + j, false, st);
+    return r;
+  };
+  if (!rc && ov) {     // the second stream starts after everything issued so far on the simulation's stream
+    rc = g->hip(hipEventRecord(g->ev_go, s->stream), "event");
+    if (!rc) rc = g->hip(hipStreamWaitEvent(g->rstream, g->ev_go, 0), "stream wait");
+  }
   for (int sg = nseg - 1; sg >= 0 && !rc; sg--) {
     const int n0 = sg * g->K, n1 = std::min(n0 + g->K, N);
-    if (!(sg == nseg - 1 && g->last_seg_stored))
-      for (int j = 0; j < n1 - n0 && !rc; j++) rc = grad_step(g, A, g->lev(sg, j), g->lev(sg, j + 1), n0 + j, false);
+    const bool stored = sg == nseg - 1 && g->last_seg_stored;
+    if (!ov) {
+      if (!stored) rc = recompute(sg, s->stream);
+    } else {
+      if (sg == nseg - 1 && !stored) {
+        rc = recompute(sg, g->rstream);
+        if (!rc) rc = g->hip(hipEventRecord(g->ev_rec[par(sg)], g->rstream), "event");
+      }
+      if (!rc && sg >= 1) {
+        if (sg + 1 <= nseg - 1) rc = g->hip(hipStreamWaitEvent(g->rstream, g->ev_adj[par(sg + 1)], 0), "stream wait");
+        if (!rc) rc = recompute(sg - 1, g->rstream);
+        if (!rc) rc = g->hip(hipEventRecord(g->ev_rec[par(sg - 1)], g->rstream), "event");
+      }
+      if (!rc && !stored) rc = g->hip(hipStreamWaitEvent(s->stream, g->ev_rec[par(sg)], 0), "stream wait");
+    }
     for (int n = n1 - 1; n >= n0 && !rc; n--) {
       A.in = g->lev(sg, n - n0);
       A.out = g->lev(sg, n - n0 + 1);
@@ -1918,7 +1987,9 @@ int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms) 
       if (e == hipSuccess && n % g->flush == 0) e = vse_launch_grad_flush(A, s->stream);
       rc = g->hip(e, "adjoint step");
     }
+    if (!rc && ov) rc = g->hip(hipEventRecord(g->ev_adj[par(sg)], s->stream), "event");
   }
+  if (g->rstream) (void)hipStreamSynchronize(g->rstream);    // nothing queued there outlives this call
   g->last_seg_stored = false;    // the recomputed segments overwrote it
   if (!rc && elapsed_ms) {
     rc = g->hip(hipEventRecord(e1, s->stream), "event");

@@ -332,3 +332,36 @@ def test_fused_reverse_step_matches_two_kernels(acc, n, scaled):
         with pytest.raises(VSEError):
             eng.set_schedule(*bad)
     eng.close()
+
+
+def test_overlapped_reverse_sweep_matches_serial():
+    # With sqrt(N) checkpoints the default sweep recomputes segment s-1 on a
+    # second stream while the adjoint of segment s runs; serial recomputes
+    # each segment before its adjoint on one stream.  Same kernels on the same
+    # inputs: the gradient must have the same bits, with the last segment
+    # stored by the forward or recomputed (a second backward), for several
+    # spacings (a partial last segment included) and with the fused step.
+    steps = 130
+    sim, att = _setup(80, steps)
+    true, start = _perturbed(sim)
+    for seg in (2, 11, 0):
+        print(f"segment length {seg}", flush=True)
+        eng = VSEGrad(VSESim.from_simulator(sim, variant="wgsl", att=att), seg_len=seg)
+        eng.set_model(*true)
+        obs = eng.forward(steps)
+        eng.set_model(*start)
+        eng.forward(steps)
+        eng.misfit(obs)
+        eng.set_schedule(overlap=False)
+        ref = eng.backward()
+        eng.set_schedule(overlap=True)
+        runs = [eng.backward()]                 # last segment recomputed on the second stream
+        eng.forward(steps)
+        eng.misfit(obs)
+        runs.append(eng.backward())             # last segment as the forward stored it
+        eng.set_schedule(fused=True, shape=2, overlap=True)
+        runs.append(eng.backward())
+        for got in runs:
+            for k in GRAD_NAMES:
+                assert np.array_equal(got[k], ref[k]), (seg, k)
+        eng.close()

@@ -220,3 +220,36 @@ def test_batch_fast_math_matches_solo_fast_runs():
     batch.close()
     for g in gs:
         g.close()
+
+
+def test_batch_two_mechanisms_matches_solo_runs():
+    # L = 2 mechanisms: the batched and solo kernels' runtime mechanism loop
+    from taichi_sim_amd.sim_support.attenuation import fit_constant_q
+    steps = 150
+    att = fit_constant_q(50.0, 50.0, 2.0, 8.0, n_sls=2)
+    members = []
+    for cfg, laws in workloads.multi_shot_2d(sizes=(96, 131), n_shots=2, steps=steps, n_elem=8):
+        sim = Simulator(cfg)
+        sim._emission_laws = laws
+        for law in range(2):
+            sim.select_law(law)
+            members.append((sim, law, VSESim.from_simulator(sim, variant="wgsl", att=att)))
+    print("batch of 4 members with 2 mechanisms, lockstep", flush=True)
+    batch = VSEBatch([m[2] for m in members])
+    batch.set_group_bytes(0)
+    batch.run(steps, check_every=100)
+    batch.close()
+    for sim, law, g in members:
+        got = _state(g)
+        sim.select_law(law)
+        solo = VSESim.from_simulator(sim, variant="wgsl", att=att)
+        solo.run(steps, check_every=100)
+        _assert_same(got, _state(solo), (sim._nx, law))
+        solo.close()
+    sim, law, g = members[-1]
+    sim.select_law(law)
+    o = oracle.Oracle(sim, variant="wgsl", att=att).run(steps)
+    for k in ("vx", "vy", "syy"):
+        assert oracle.rel_l2(g.field(k), o.f[k]) <= 1e-5, k
+    for m in members:
+        m[2].close()

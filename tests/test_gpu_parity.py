@@ -425,3 +425,21 @@ def test_bandwidth_probes_run():
     for r, w in ((6, 24), (7, 3), (0, 1), (0, 32)):
         assert lib.vse_stream_bandwidth(1 << 22, r, w, 3, ctypes.byref(g)) == 0, (r, w)
         assert 100.0 < g.value < 20000.0, (r, w, g.value)
+
+
+@pytest.mark.parametrize("n_sls", [1, 2, 5])
+def test_2d_visco_mechanism_counts(n_sls):
+    # L != 3 runs the kernels' runtime mechanism loop (the L = 3 kernels issue
+    # every memory-variable load up front)
+    print(f"2D viscoelastic, {n_sls} mechanisms", flush=True)
+    cfg = workloads.visco_2d_array(n=80, steps=200, n_elem=8, n_voids=2)
+    cfg["attenuation"]["n_sls"] = n_sls
+    _compare(Simulator(cfg), "wgsl", 200, att=attenuation_from_config(cfg))
+
+
+@pytest.mark.parametrize("n_sls", [2, 5])
+def test_3d_visco_mechanism_counts(n_sls):
+    print(f"3D viscoelastic, {n_sls} mechanisms", flush=True)
+    cfg = workloads.visco_3d_block(n=48, steps=50, n_incl=2, n_rec_side=3, t0=0.3)
+    cfg["attenuation"]["n_sls"] = n_sls
+    _compare(Simulator3D(cfg), "wgsl", 50, att=attenuation_from_config(cfg), check_f64=False)

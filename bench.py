@@ -230,9 +230,9 @@ def main():
                     help="also time the gradient with every state stored, no recompute (--workload adjoint)")
     ap.add_argument("--adj-shots", type=int, default=8, help="shots (--workload adjoint)")
     ap.add_argument("--adj-schedules", type=int, default=0,
-                    help="--workload adjoint: also time shot 0's reverse sweep under each schedule "
-                         "(two kernels per step, fused tile shapes 0..2), with sqrt(N) checkpoints and "
-                         "with every state stored, and check that the gradients have the same bits")
+                    help="--workload adjoint: also time shot 0's reverse sweep with and without the "
+                         "second-stream recompute, with sqrt(N) checkpoints and with every state stored, "
+                         "and check that the gradients have the same bits")
     ap.add_argument("--sizes", default="512,1024,1536,2048",
                     help="specimen grid sizes (--workload shots), comma separated")
     ap.add_argument("--shots", type=int, default=16, help="array elements = FMC shots per specimen (--workload shots)")
@@ -698,9 +698,8 @@ def measure_adjoint(n_shots, fd_dirs=0, store_all=False, schedules=False):
                   "same_gradient_bits": all(np.array_equal(c_all[key], coef0[key]) for key in coef0)}
         ea.close()
         print(f"# every state stored: {stored}", file=sys.stderr, flush=True)
-    # the reverse sweep of shot 0 under each schedule: two kernels per step (the
-    # default), serial or with the next segment recomputed on a second stream,
-    # and the fused step at each tile shape; bit-identical gradients
+    # the reverse sweep of shot 0 with the next segment recomputed on a second
+    # stream (the default) and with one stream; bit-identical gradients
     sched = None
     if schedules:
         sim.select_law(0)
@@ -710,18 +709,16 @@ def measure_adjoint(n_shots, fd_dirs=0, store_all=False, schedules=False):
             es = VSEGrad(g, seg_len=seg)
             es.set_model(*start)
             ref = None
-            for fused, shape, ov in ((False, 0, False), (False, 0, True), (True, 0, True), (True, 1, True),
-                                     (True, 2, True)):
-                es.set_schedule(fused, shape, ov)
+            for ov in (False, True):
+                es.set_overlap(ov)
                 es.forward(steps, check_every=500)
                 es.misfit(obs[0])
                 c = es.backward()
                 ref = c if ref is None else ref
-                key = (f"{'sqrtN' if seg == 0 else 'every_state'}_{'fused' + str(shape) if fused else 'two_kernels'}"
-                       f"{'' if ov else '_serial'}")
+                key = f"{'sqrtN' if seg == 0 else 'every_state'}_{'overlapped' if ov else 'one_stream'}"
                 sched[key] = {"backward_ms": round(es.ms["backward"], 2),
                               "same_gradient_bits": all(np.array_equal(c[k], ref[k]) for k in ref)}
-                print(f"# schedule {key}: {sched[key]}", file=sys.stderr, flush=True)
+                print(f"# reverse sweep {key}: {sched[key]}", file=sys.stderr, flush=True)
             es.close()
 
     g.close()

@@ -343,23 +343,12 @@ int vse_grad_misfit(vse_grad *g, const float *observed, double *misfit);
 int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms);
 /* Steps per FP64 flush of the FP32 gradient partials, 1 .. 4096 (default 32). */
 int vse_grad_set_flush(vse_grad *g, int flush_every);
-/* Schedule of the reverse sweep.  VSE_ADJ_TWO_PASS (default): two launches
- * per step, abar5..8 through device memory.  VSE_ADJ_FUSED: one launch per
- * step, the velocity part recomputed over halo rows and columns of each tile
- * so that abar5..8 stay in LDS; shape 0..2 picks the tile (0: 8 wavefronts x
- * 32 rows, 1: 4 x 16, 2: 8 x 24).  Both give the same gradient bits unless a
- * cell holds two receiver points (then the residual sums in another order);
- * the fused step is slower on MI355X at 1024^2 (the halo recompute and one
- * barrier per workgroup at 4 wavefronts per SIMD cost more than the 32 B per
- * cell and step it saves). */
-#define VSE_ADJ_TWO_PASS 0
-#define VSE_ADJ_FUSED 1
-/* Or-ed into schedule: with sqrt(N) checkpoints, recompute each segment on the
- * simulation's stream before its adjoint.  By default the reverse sweep
- * recomputes segment s-1 on a second stream while the adjoint of segment s
- * runs (one more segment of levels in device memory); the gradient is the same. */
-#define VSE_ADJ_SERIAL 2
-int vse_grad_set_schedule(vse_grad *g, int schedule, int shape);
+/* With sqrt(N) checkpoints the reverse sweep recomputes segment s-1 on a
+ * second stream while the adjoint of segment s runs (overlap 1, the default;
+ * it holds one more segment of levels in device memory).  overlap 0
+ * recomputes each segment before its adjoint on the simulation's stream.  The
+ * gradient is the same either way. */
+int vse_grad_set_overlap(vse_grad *g, int overlap);
 /* FP64 gradient map VSE_GRAD_* (n cells, grid layout) of the last backward. */
 int vse_grad_read(vse_grad *g, int which, double *out);
 /* Field 0..4 = vx, vy, sxx, syy, sxy of the last forward's final state. */

@@ -57,7 +57,7 @@ EXPORTS = ["vse_abi_version", "vse_create", "vse_set_material", "vse_set_cpml", 
            "vse_stream_bandwidth",
            "vse_batch_create", "vse_batch_run", "vse_batch_profile", "vse_batch_set_rows", "vse_batch_set_group_bytes", "vse_batch_groups",
            "vse_batch_size", "vse_grad_create", "vse_grad_set_tau_scale", "vse_grad_forward", "vse_grad_misfit",
-           "vse_grad_backward", "vse_grad_set_flush", "vse_grad_set_schedule", "vse_grad_read", "vse_grad_read_state", "vse_grad_segment", "vse_grad_device_bytes",
+           "vse_grad_backward", "vse_grad_set_flush", "vse_grad_set_overlap", "vse_grad_read", "vse_grad_read_state", "vse_grad_segment", "vse_grad_device_bytes",
            "vse_grad_last_error", "vse_grad_destroy", "vse_batch_last_error", "vse_batch_destroy"]
 BATCH_MAX = 256
 ABI_VERSION = 3
@@ -132,7 +132,7 @@ def load():
         "vse_grad_misfit": (I, [P, fp, ctypes.POINTER(ctypes.c_double)]),
         "vse_grad_backward": (I, [P, fp, fp]),
         "vse_grad_set_flush": (I, [P, I]),
-        "vse_grad_set_schedule": (I, [P, I, I]),
+        "vse_grad_set_overlap": (I, [P, I]),
         "vse_grad_read": (I, [P, I, ctypes.POINTER(ctypes.c_double)]),
         "vse_grad_read_state": (I, [P, I, fp]),
         "vse_grad_segment": (I, [P]),

@@ -98,14 +98,11 @@ class VSEGrad:
         32; 1 accumulates every step's terms in FP64)."""
         self._check(self._lib.vse_grad_set_flush(self._h, int(every)))
 
-    def set_schedule(self, fused=False, shape=0, overlap=True):
-        """Reverse sweep: the velocity and stress adjoints as two launches per
-        step (default), or one fused launch per step (shape 0..2 picks the
-        tile; the same gradient bits, slower on MI355X at 1024^2).  overlap:
-        with sqrt(N) checkpoints, recompute the next segment on a second
-        stream while the adjoint of the current one runs (same gradient)."""
-        code = (1 if fused else 0) | (0 if overlap else 2)
-        self._check(self._lib.vse_grad_set_schedule(self._h, code, int(shape)))
+    def set_overlap(self, on=True):
+        """With sqrt(N) checkpoints, recompute the next segment on a second
+        stream while the adjoint of the current one runs (default), or each
+        segment before its adjoint on one stream.  The gradient is the same."""
+        self._check(self._lib.vse_grad_set_overlap(self._h, 1 if on else 0))
 
     def device_bytes(self):
         return int(self._lib.vse_grad_device_bytes(self._h))

@@ -443,35 +443,26 @@ __global__ __launch_bounds__(BX * BY) void k_dvelocity(DiffArgs A) {
 }
 
 // ---------------------------------------------------------------- adjoint
-// The velocity part of a reverse step reads the adjoint state it rewrites
-// (v_bar, psi_bar of the sigma slots) and abar1..4 through AdjVel, so the same
-// code serves the in-place two-kernel step (in == out) and the fused step,
-// which double-buffers them because other workgroups read them at halo cells.
-struct AdjVel {
-  const VSE_G float *vx_in, *vy_in;   // v_bar of step n+1
-  VSE_G float *vx_out, *vy_out;       // v_bar of step n
-  const VSE_G float *ps_in[4];        // psi_bar, sigma slots SL_SXXX..SL_SYYY
-  VSE_G float *ps_out[4];
-  const VSE_G float *a_in[4];         // abar1..4 of step n+1
-};
-
-// abar5..8 of one cell
-struct ABar4 {
-  float a5, a6, a7, a8;
-};
-
-// Adjoint of the velocity pass of step n at (x, y) inside the grid: v_bar +=
-// D^T(abar1..4), the adjoint of the velocity update, of the Dirichlet rule,
-// of the CPML recursion and of the source term; returns abar5..8.  own: the
-// cell's v_bar, psi_bar and density partials are stored (a halo copy in the
-// fused step only produces abar5..8).
 template <int ORD>
-__device__ __forceinline__ ABar4 adj_velocity_core(const DiffArgs &A, int x, int y, bool own, const AdjVel &P) {
+__global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
+  int x, y;
+  tile_cell(A, x, y);
   const int nx = A.nx, ny = A.ny;
+  // residual of step n into sigma_yy_bar at the receiver points (consumed by
+  // adj_stress, which runs next; this kernel does not touch sigma_bar)
+  if (blockIdx.x == 0) {
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    for (int r = t; r < A.n_rec; r += blockDim.x * blockDim.y) {
+      if (A.n + A.rec_gate < A.rec_delay[r]) continue;
+      const float gb = A.gbar[(long)A.n * A.n_rec + r];
+      for (int p = A.rec_beg[r]; p < A.rec_beg[r + 1]; p++) atomicAdd(&A.bar.syy[A.rec_cell[p]], gb);
+    }
+  }
+  if (x >= nx || y >= ny) return;
   constexpr int last = ORD - 1, off = ORD - 1;
   const long id = (long)x * ny + y;
   const float dt = A.dt;
-  const DLevel &O = A.out;
+  const DLevel &O = A.out, &B = A.bar;
   const int ex = owned_band(A.ax[0], x, off), ey = owned_band(A.ax[1], y, off);
   const long px = (long)ex * ny + y, py = (long)x * A.ax[1].nb + ey;
   const bool inX = x >= last + 1 && x < nx - last && y >= last + 1 && y < ny - last;
@@ -481,24 +472,24 @@ __device__ __forceinline__ ABar4 adj_velocity_core(const DiffArgs &A, int x, int
   // ---- loads
   // adjoint of v_{n+1}: carried part + transposed stencils of the stress
   // pass of step n+1 (a1 = D+x vx, a4 = D+y vx, a2 = D-y vy, a3 = D-x vy)
-  const float bvx = P.vx_in[id], bvy = P.vy_in[id];
-  const Taps<ORD> u0 = tapsT<ORD, true, 0>(A, P.a_in[0], x, y), u3 = tapsT<ORD, true, 1>(A, P.a_in[3], x, y);
-  const Taps<ORD> u1 = tapsT<ORD, false, 1>(A, P.a_in[1], x, y), u2 = tapsT<ORD, false, 0>(A, P.a_in[2], x, y);
+  const float bvx = B.vx[id], bvy = B.vy[id];
+  const Taps<ORD> u0 = tapsT<ORD, true, 0>(A, A.abar[0], x, y), u3 = tapsT<ORD, true, 1>(A, A.abar[3], x, y);
+  const Taps<ORD> u1 = tapsT<ORD, false, 1>(A, A.abar[1], x, y), u2 = tapsT<ORD, false, 0>(A, A.abar[2], x, y);
   Taps<ORD> t1, t2, t3, t4;
   float p5 = 0.f, p6 = 0.f, p7 = 0.f, p8 = 0.f;       // psi' of level n+1
   float b5 = 0.f, b6 = 0.f, b7 = 0.f, b8 = 0.f;       // their adjoints
   float rhoX = 0.f, r00 = 0.f, r10 = 0.f, r11 = 0.f, r01 = 0.f, gx = 0.f, gy = 0.f;
   if (inX) {
     rhoX = A.rho[id];
-    if (ex >= 0) b5 = P.ps_in[SL_SXXX - 4][px];
-    if (ey >= 0) b6 = P.ps_in[SL_SXYY - 4][py];
+    if (ex >= 0) b5 = B.psi[SL_SXXX][px];
+    if (ey >= 0) b6 = B.psi[SL_SXYY][py];
   }
   if (inXY) {
     t1 = taps<ORD, false>(O.sxx, id, ny);
     t2 = taps<ORD, false>(O.sxy, id, 1);
     if (ex >= 0) p5 = O.psi[SL_SXXX][px];
     if (ey >= 0) p6 = O.psi[SL_SXYY][py];
-    if (own) gx = A.gf[VSE_GR_RHO_VX][id];
+    gx = A.gf[VSE_GR_RHO_VX][id];
   }
   if (inY) {
     r00 = A.rho[id];
@@ -507,14 +498,14 @@ __device__ __forceinline__ ABar4 adj_velocity_core(const DiffArgs &A, int x, int
     r01 = A.rho[id + 1];
     t3 = taps<ORD, true>(O.sxy, id, ny);
     t4 = taps<ORD, true>(O.syy, id, 1);
-    if (own) gy = A.gf[VSE_GR_RHO_VY][id];
+    gy = A.gf[VSE_GR_RHO_VY][id];
     if (ex >= 0) {
       p7 = O.psi[SL_SXYX][px];
-      b7 = P.ps_in[SL_SXYX - 4][px];
+      b7 = B.psi[SL_SXYX][px];
     }
     if (ey >= 0) {
       p8 = O.psi[SL_SYYY][py];
-      b8 = P.ps_in[SL_SYYY - 4][py];
+      b8 = B.psi[SL_SYYY][py];
     }
   }
   const int col = A.src_map[id];
@@ -524,12 +515,12 @@ __device__ __forceinline__ ABar4 adj_velocity_core(const DiffArgs &A, int x, int
     rho_src = rho_vy_edge(A, x, y, id);
     sval = A.src[(long)A.n * A.src_cols + col];
   }
-  if (!inY && src_on && own) gy = A.gf[VSE_GR_RHO_VY][id];
+  if (!inY && src_on) gy = A.gf[VSE_GR_RHO_VY][id];
 
   // ---- adjoint of the velocity pass
   const float vxb = bvx + sumT<ORD, 0>(A, u0) + sumT<ORD, 1>(A, u3);
   const float vyb = bvy + sumT<ORD, 1>(A, u1) + sumT<ORD, 0>(A, u2);
-  ABar4 ab{0.f, 0.f, 0.f, 0.f};
+  float a5b = 0.f, a6b = 0.f, a7b = 0.f, a8b = 0.f;
   if (inX) {
     float eb = 0.f;
     if (inY && rhoX > 0.f) {
@@ -540,8 +531,8 @@ __device__ __forceinline__ ABar4 adj_velocity_core(const DiffArgs &A, int x, int
       const float e6 = ey >= 0 ? dx2 / A.ax[1].k[y - off] + p6 : dx2;
       gx = gx + -(e5 + e6) * eb / rhoX;
     }
-    ab.a5 = ex >= 0 ? cpml_back(A.ax[0].a, A.ax[0].b, A.ax[0].k, x - off, b5, eb, b5) : eb;
-    ab.a6 = ey >= 0 ? cpml_back(A.ax[1].a, A.ax[1].b, A.ax[1].k, y - off, b6, eb, b6) : eb;
+    a5b = ex >= 0 ? cpml_back(A.ax[0].a, A.ax[0].b, A.ax[0].k, x - off, b5, eb, b5) : eb;
+    a6b = ey >= 0 ? cpml_back(A.ax[1].a, A.ax[1].b, A.ax[1].k, y - off, b6, eb, b6) : eb;
   }
   float ty = 0.f;
   if (inY) {
@@ -554,76 +545,38 @@ __device__ __forceinline__ ABar4 adj_velocity_core(const DiffArgs &A, int x, int
       const float e8 = ey >= 0 ? dy2 / A.ax[1].kh[y - off] + p8 : dy2;
       ty += -(e7 + e8) * eb / rhoY;
     }
-    ab.a7 = ex >= 0 ? cpml_back(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, x - off, b7, eb, b7) : eb;
-    ab.a8 = ey >= 0 ? cpml_back(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, y - off, b8, eb, b8) : eb;
+    a7b = ex >= 0 ? cpml_back(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, x - off, b7, eb, b7) : eb;
+    a8b = ey >= 0 ? cpml_back(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, y - off, b8, eb, b8) : eb;
   }
   // the source term dt*src/rho_vy of step n
   if (src_on && rho_src > 0.f) ty += -(sval * dt / rho_src) * vyb / rho_src;
 
   // ---- stores
-  if (own) {
-    if (inXY) A.gf[VSE_GR_RHO_VX][id] = gx;
-    if (inY || src_on) A.gf[VSE_GR_RHO_VY][id] = gy + ty;
-    if (inX) {
-      if (ex >= 0) P.ps_out[SL_SXXX - 4][px] = b5;
-      if (ey >= 0) P.ps_out[SL_SXYY - 4][py] = b6;
-    }
-    if (inY) {
-      if (ex >= 0) P.ps_out[SL_SXYX - 4][px] = b7;
-      if (ey >= 0) P.ps_out[SL_SYYY - 4][py] = b8;
-    }
-    // v_{n+1} = v_n + ... inside the vy interior, 0 outside it
-    P.vx_out[id] = inY ? vxb : 0.f;
-    P.vy_out[id] = inY ? vyb : 0.f;
+  if (inXY) A.gf[VSE_GR_RHO_VX][id] = gx;
+  if (inY || src_on) A.gf[VSE_GR_RHO_VY][id] = gy + ty;
+  if (inX) {
+    if (ex >= 0) B.psi[SL_SXXX][px] = b5;
+    if (ey >= 0) B.psi[SL_SXYY][py] = b6;
   }
-  return ab;
+  if (inY) {
+    if (ex >= 0) B.psi[SL_SXYX][px] = b7;
+    if (ey >= 0) B.psi[SL_SYYY][py] = b8;
+  }
+  A.abar[4][id] = a5b;
+  A.abar[5][id] = a6b;
+  A.abar[6][id] = a7b;
+  A.abar[7][id] = a8b;
+  // v_{n+1} = v_n + ... inside the vy interior, 0 outside it
+  B.vx[id] = inY ? vxb : 0.f;
+  B.vy[id] = inY ? vyb : 0.f;
 }
 
-// The in-place adjoint state of the two-kernel step.
-__device__ __forceinline__ AdjVel adj_vel_in_place(const DiffArgs &A) {
-  const DLevel &B = A.bar;
-  AdjVel P;
-  P.vx_in = B.vx; P.vy_in = B.vy; P.vx_out = B.vx; P.vy_out = B.vy;
-  for (int k = 0; k < 4; k++) {
-    P.ps_in[k] = B.psi[SL_SXXX + k];
-    P.ps_out[k] = B.psi[SL_SXXX + k];
-    P.a_in[k] = A.abar[k];
-  }
-  return P;
-}
-
-template <int ORD>
-__global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
+template <int ORD, int LC, bool TS>
+__global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
   int x, y;
   tile_cell(A, x, y);
-  // residual of step n into sigma_yy_bar at the receiver points (consumed by
-  // adj_stress, which runs next; this kernel does not touch sigma_bar)
-  if (blockIdx.x == 0) {
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    for (int r = t; r < A.n_rec; r += blockDim.x * blockDim.y) {
-      if (A.n + A.rec_gate < A.rec_delay[r]) continue;
-      const float gb = A.gbar[(long)A.n * A.n_rec + r];
-      for (int p = A.rec_beg[r]; p < A.rec_beg[r + 1]; p++) atomicAdd(&A.bar.syy[A.rec_cell[p]], gb);
-    }
-  }
-  if (x >= A.nx || y >= A.ny) return;
-  const ABar4 ab = adj_velocity_core<ORD>(A, x, y, true, adj_vel_in_place(A));
-  const long id = (long)x * A.ny + y;
-  A.abar[4][id] = ab.a5;
-  A.abar[5][id] = ab.a6;
-  A.abar[6][id] = ab.a7;
-  A.abar[7][id] = ab.a8;
-}
-
-// Adjoint of the stress pass of step n at (x, y) inside the grid, given the
-// transposed stencils of abar5..8 (u4: D-x of a5, u5: D-y of a6, u6: D+x of
-// a7, u7: D+y of a8) and sigma_yy_bar's carried value bsyy: sigma_bar, r_bar,
-// psi_bar of the v slots and the partials in place; abar1..4 to a_out.
-template <int ORD, int LC, bool TS>
-__device__ __forceinline__ void adj_stress_core(const DiffArgs &A, int x, int y, const Taps<ORD> &u4,
-                                                const Taps<ORD> &u5, const Taps<ORD> &u6, const Taps<ORD> &u7,
-                                                float bsyy, VSE_G float *const *a_out) {
   const int nx = A.nx, ny = A.ny;
+  if (x >= nx || y >= ny) return;
   constexpr int last = ORD - 1, off = ORD - 1;
   constexpr int LR = LC > 0 ? LC : 1;
   const long id = (long)x * ny + y;
@@ -639,7 +592,9 @@ __device__ __forceinline__ void adj_stress_core(const DiffArgs &A, int x, int y,
   // ---- loads
   // adjoint of sigma_{n+1}: carried part (incl. receivers) + transposed
   // stencils of the velocity pass (a5 = D-x sxx, a6 = D-y sxy, a7 = D+x sxy, a8 = D+y syy)
-  const float bsxx = B.sxx[id], bsxy = B.sxy[id];
+  const float bsxx = B.sxx[id], bsyy = B.syy[id], bsxy = B.sxy[id];
+  const Taps<ORD> u4 = tapsT<ORD, false, 0>(A, A.abar[4], x, y), u7 = tapsT<ORD, true, 1>(A, A.abar[7], x, y);
+  const Taps<ORD> u5 = tapsT<ORD, false, 1>(A, A.abar[5], x, y), u6 = tapsT<ORD, true, 0>(A, A.abar[6], x, y);
   Taps<ORD> t1, t2, t3, t4;
   float q1 = 0.f, q2 = 0.f, q3 = 0.f, q4 = 0.f;       // psi of level n (v slots)
   float b1 = 0.f, b2 = 0.f, b3 = 0.f, b4 = 0.f;       // adjoints of psi'
@@ -793,97 +748,10 @@ __device__ __forceinline__ void adj_stress_core(const DiffArgs &A, int x, int y,
     A.gf[VSE_GR_MU_XY][id] = gmxy;
   }
   if (inN || inS) A.gf[VSE_GR_TAU][id] = gt + gs;
-  a_out[0][id] = a1b;
-  a_out[1][id] = a2b;
-  a_out[2][id] = a3b;
-  a_out[3][id] = a4b;
-}
-
-template <int ORD, int LC, bool TS>
-__global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
-  int x, y;
-  tile_cell(A, x, y);
-  if (x >= A.nx || y >= A.ny) return;
-  const Taps<ORD> u4 = tapsT<ORD, false, 0>(A, A.abar[4], x, y), u7 = tapsT<ORD, true, 1>(A, A.abar[7], x, y);
-  const Taps<ORD> u5 = tapsT<ORD, false, 1>(A, A.abar[5], x, y), u6 = tapsT<ORD, true, 0>(A, A.abar[6], x, y);
-  adj_stress_core<ORD, LC, TS>(A, x, y, u4, u5, u6, u7, A.bar.syy[(long)x * A.ny + y], A.abar);
-}
-
-// ---- fused reverse step: one launch per step.  A workgroup of W wavefronts
-// owns XT = NV - 2*ORD rows by YT = 64 - 2*ORD columns.  Lane j sits on column
-// y0 - ORD + j.  Phase 1 runs the velocity part over NV rows (the tile and
-// ORD halo rows on each side) on all 64 lanes and keeps abar5..8 in LDS;
-// halo cells compute the same values as their owners from the same inputs.
-// Phase 2 runs the stress part over the tile, taking the transposed stencils
-// of abar5..8 from LDS.  abar5..8 never reach memory; v_bar, psi_bar of the
-// sigma slots and abar1..4, which phase 1 reads at other workgroups' cells,
-// are double-buffered (F.v: read from in, written to out).  The receivers'
-// residual is added to sigma_yy_bar by the owner of each receiver point, in
-// the points' table order, where the two-kernel step adds it with atomics.
-// Same operations on the same operands: bit-identical to the two-kernel step
-// whenever no cell holds two receiver points.
-template <int ORD, int LC, bool TS, int W, int NV>
-__global__ __launch_bounds__(64 * W) void k_adj_fused(DiffArgs A, AdjFused F, int nty) {
-  constexpr int XT = NV - 2 * ORD, YT = 64 - 2 * ORD;
-  __shared__ float sa[4][NV][64];
-  const int nb = gridDim.x, b = blockIdx.x;
-  const int t = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;    // XCD-contiguous tile order
-  const int x0 = (t / nty) * XT, y0 = (t % nty) * YT;
-  const int j = threadIdx.x, w = threadIdx.y;
-  const int y = y0 - ORD + j;
-  const int nx = A.nx, ny = A.ny;
-  AdjVel P;
-  P.vx_in = F.v_in[0]; P.vy_in = F.v_in[1]; P.vx_out = F.v_out[0]; P.vy_out = F.v_out[1];
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    P.ps_in[k] = F.ps_in[k];
-    P.ps_out[k] = F.ps_out[k];
-    P.a_in[k] = F.a_in[k];
-  }
-  for (int v = w; v < NV; v += W) {
-    const int x = x0 - ORD + v;
-    ABar4 ab{0.f, 0.f, 0.f, 0.f};
-    if (x >= 0 && x < nx && y >= 0 && y < ny)
-      ab = adj_velocity_core<ORD>(A, x, y, v >= ORD && v < ORD + XT && j >= ORD && j < ORD + YT, P);
-    sa[0][v][j] = ab.a5;
-    sa[1][v][j] = ab.a6;
-    sa[2][v][j] = ab.a7;
-    sa[3][v][j] = ab.a8;
-  }
-  __syncthreads();
-  if (j < ORD || j >= ORD + YT || y >= ny) return;
-  for (int s = w; s < XT; s += W) {
-    const int x = x0 + s;
-    if (x >= nx) return;
-    const int v = s + ORD;
-    Taps<ORD> u4, u5, u6, u7;
-#pragma unroll
-    for (int c = 0; c < ORD; c++) {
-      u4.a[c] = sa[0][v - c][j];
-      u4.b[c] = sa[0][v + c + 1][j];
-      u5.a[c] = sa[1][v][j - c];
-      u5.b[c] = sa[1][v][j + c + 1];
-      u6.a[c] = sa[2][v - c - 1][j];
-      u6.b[c] = sa[2][v + c][j];
-      u7.a[c] = sa[3][v][j - c - 1];
-      u7.b[c] = sa[3][v][j + c];
-    }
-    const long id = (long)x * ny + y;
-    float bsyy = A.bar.syy[id];
-    if (F.rec_row[x]) {
-      // receiver points sorted by cell: find this cell's, add their residuals
-      int lo = 0, hi = F.n_rp;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (F.rp_cell[mid] < id) lo = mid + 1; else hi = mid;
-      }
-      for (int p = lo; p < F.n_rp && F.rp_cell[p] == id; p++) {
-        const int r = F.rp_rec[p];
-        if (A.n + A.rec_gate >= A.rec_delay[r]) bsyy = bsyy + A.gbar[(long)A.n * A.n_rec + r];
-      }
-    }
-    adj_stress_core<ORD, LC, TS>(A, x, y, u4, u5, u6, u7, bsyy, F.a_out);
-  }
+  A.abar[0][id] = a1b;
+  A.abar[1][id] = a2b;
+  A.abar[2][id] = a3b;
+  A.abar[3][id] = a4b;
 }
 
 // FP64 time accumulation: g += partial, partial = 0 (a partial that is 0
@@ -969,33 +837,6 @@ hipError_t vse_launch_adj_stress(const DiffArgs &A, hipStream_t st) {
   if (A.L == 3 && A.ts_unit) hipLaunchKernelGGL((k_adj_stress<O, 3, false>), grid_of(A), dim3(BY, BX), 0, st, A); \
   else if (A.L == 3) hipLaunchKernelGGL((k_adj_stress<O, 3, true>), grid_of(A), dim3(BY, BX), 0, st, A);      \
   else hipLaunchKernelGGL((k_adj_stress<O, 0, true>), grid_of(A), dim3(BY, BX), 0, st, A)
-  VSE_ORD_SWITCH(A.ord, CALL)
-#undef CALL
-  return hipGetLastError();
-}
-
-// Tile shapes (wavefronts, velocity rows) of the fused reverse step; a tile
-// updates NV - 2*ORD rows by 64 - 2*ORD columns.
-template <int O, int LC, bool TS, int W, int NV>
-static void launch_adj_fused(const DiffArgs &A, const AdjFused &F, hipStream_t st) {
-  constexpr int XT = NV - 2 * O, YT = 64 - 2 * O;
-  const int ntx = (A.nx + XT - 1) / XT, nty = (A.ny + YT - 1) / YT;
-  hipLaunchKernelGGL((k_adj_fused<O, LC, TS, W, NV>), dim3((unsigned)(ntx * nty)), dim3(64, W), 0, st, A, F, nty);
-}
-
-template <int O, int LC, bool TS>
-static void launch_adj_fused_shape(const DiffArgs &A, const AdjFused &F, int shape, hipStream_t st) {
-  if (shape == 1) launch_adj_fused<O, LC, TS, 4, 16>(A, F, st);
-  else if (shape == 2) launch_adj_fused<O, LC, TS, 8, 24>(A, F, st);
-  else launch_adj_fused<O, LC, TS, 8, 32>(A, F, st);
-}
-
-hipError_t vse_launch_adj_fused(const DiffArgs &A, const AdjFused &F, int shape, hipStream_t st) {
-  if (shape < 0 || shape >= VSE_ADJ_SHAPES) return hipErrorInvalidValue;
-#define CALL(O)                                                                  \
-  if (A.L == 3 && A.ts_unit) launch_adj_fused_shape<O, 3, false>(A, F, shape, st); \
-  else if (A.L == 3) launch_adj_fused_shape<O, 3, true>(A, F, shape, st);         \
-  else launch_adj_fused_shape<O, 0, true>(A, F, shape, st)
   VSE_ORD_SWITCH(A.ord, CALL)
 #undef CALL
   return hipGetLastError();

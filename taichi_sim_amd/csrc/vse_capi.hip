@@ -1502,7 +1502,6 @@ struct vse_grad {
   vse_sim *s = nullptr;
   int K = 0, n_ckpt = 0, n_fwd = 0;
   int flush = VSE_GRAD_FLUSH;     // steps per FP64 flush of the FP32 partials
-  int sched = VSE_ADJ_TWO_PASS, shape = 0;   // reverse-sweep schedule and fused tile shape
   int n_rec = 0;                  // receivers of s when the trace buffers were sized
   bool last_seg_stored = false;   // the forward left the last segment's levels in seg[]
   std::vector<DLevel> ckpt, seg;
@@ -1516,12 +1515,6 @@ struct vse_grad {
   hipEvent_t ev_go = nullptr, ev_rec[2] = {}, ev_adj[2] = {};
   DLevel bar{};
   float *abar[8] = {};
-  // the fused reverse step's second copy of v_bar (vx, vy), of psi_bar's sigma
-  // slots and of abar1..4 (the first copy: bar and abar[0..3]), and the
-  // receiver points sorted by cell (rebuilt by every backward)
-  float *vbar2[2] = {}, *psbar2[4] = {}, *abar2[4] = {};
-  int *rec_row = nullptr, *rp_cell = nullptr, *rp_rec = nullptr;
-  int n_rp = 0, rp_cap = 0;
   double *g[VSE_GR_COUNT] = {};
   float *gf[VSE_GR_COUNT] = {};   // FP32 per-step partials of g
   float *ts = nullptr, *traces = nullptr, *gbar = nullptr, *obs = nullptr;
@@ -1716,10 +1709,6 @@ int vse_grad_create(vse_sim *s, int seg_len, vse_grad **out) {
   }
   if (!rc) rc = g->alloc_level(g->bar);
   for (int i = 0; i < 8 && !rc; i++) rc = g->alloc(&g->abar[i], s->ncell);
-  for (int i = 0; i < 2 && !rc; i++) rc = g->alloc(&g->vbar2[i], s->ncell);
-  for (int i = 0; i < 4 && !rc; i++) rc = g->alloc(&g->psbar2[i], (size_t)g->psi_elems(4 + i));
-  for (int i = 0; i < 4 && !rc; i++) rc = g->alloc(&g->abar2[i], s->ncell);
-  if (!rc) rc = g->alloc(&g->rec_row, s->d.nx);
   for (int i = 0; i < VSE_GR_COUNT && !rc; i++) rc = g->alloc(&g->g[i], s->ncell);
   for (int i = 0; i < VSE_GR_COUNT && !rc; i++) rc = g->alloc(&g->gf[i], s->ncell);
   if (!rc) rc = g->alloc(&g->ts, s->ncell);
@@ -1837,54 +1826,10 @@ int vse_grad_misfit(vse_grad *g, const float *observed, double *misfit) {
   return rc;
 }
 
-// The simulation's receiver points sorted by cell (then by table order), with
-// their receiver, and a per-row flag, for the fused reverse step's residual.
-static int grad_rec_table(vse_grad *g) {
-  vse_sim *s = g->s;
-  const int nr = s->n_rec, ny = s->d.ny;
-  std::vector<int> beg(nr + 1, 0);
-  int rc = 0;
-  if (nr > 0) rc = g->hip(hipMemcpyAsync(beg.data(), s->rec_beg, (nr + 1) * sizeof(int), hipMemcpyDeviceToHost, s->stream), "receiver table");
-  if (!rc) rc = g->hip(hipStreamSynchronize(s->stream), "receiver table");
-  const int np = nr > 0 ? beg[nr] : 0;
-  std::vector<int> cell(np);
-  if (!rc && np > 0) rc = g->hip(hipMemcpyAsync(cell.data(), s->rec_cell, np * sizeof(int), hipMemcpyDeviceToHost, s->stream), "receiver cells");
-  if (!rc) rc = g->hip(hipStreamSynchronize(s->stream), "receiver cells");
-  if (rc) return rc;
-  std::vector<int> order(np), rec_of(np), rp_cell(np), rp_rec(np), row(s->d.nx, 0);
-  for (int r = 0; r < nr; r++)
-    for (int p = beg[r]; p < beg[r + 1]; p++) rec_of[p] = r;
-  for (int p = 0; p < np; p++) order[p] = p;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cell[a] < cell[b]; });
-  for (int k = 0; k < np; k++) {
-    rp_cell[k] = cell[order[k]];
-    rp_rec[k] = rec_of[order[k]];
-    row[cell[order[k]] / ny] = 1;
-  }
-  if (np > g->rp_cap) {
-    rc = g->alloc(&g->rp_cell, np);
-    if (!rc) rc = g->alloc(&g->rp_rec, np);
-    if (rc) return rc;
-    g->rp_cap = np;
-  }
-  if (np > 0) {
-    rc = g->hip(hipMemcpyAsync(g->rp_cell, rp_cell.data(), np * sizeof(int), hipMemcpyHostToDevice, s->stream), "receiver points");
-    if (!rc) rc = g->hip(hipMemcpyAsync(g->rp_rec, rp_rec.data(), np * sizeof(int), hipMemcpyHostToDevice, s->stream), "receiver points");
-  }
-  if (!rc) rc = g->hip(hipMemcpyAsync(g->rec_row, row.data(), row.size() * sizeof(int), hipMemcpyHostToDevice, s->stream), "receiver rows");
-  if (!rc) rc = g->hip(hipStreamSynchronize(s->stream), "receiver table");    // the host vectors go out of scope
-  g->n_rp = rc ? 0 : np;
-  return rc;
-}
-
-int vse_grad_set_schedule(vse_grad *g, int schedule, int shape) {
+int vse_grad_set_overlap(vse_grad *g, int overlap) {
   if (int rc = grad_usable(g)) return rc;
-  const int kernels = schedule & ~VSE_ADJ_SERIAL;
-  if ((kernels != VSE_ADJ_TWO_PASS && kernels != VSE_ADJ_FUSED) || shape < 0 || shape >= VSE_ADJ_SHAPES)
-    return VSE_ERR_ARG;
-  g->sched = kernels;
-  g->shape = shape;
-  g->overlap = (schedule & VSE_ADJ_SERIAL) == 0;
+  if (overlap != 0 && overlap != 1) return VSE_ERR_ARG;
+  g->overlap = overlap != 0;
   return VSE_OK;
 }
 
@@ -1901,13 +1846,6 @@ int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms) 
   if (!rc) rc = g->zero_level(g->bar);
   for (int i = 0; i < 8 && !rc; i++)
     rc = g->hip(hipMemsetAsync(g->abar[i], 0, s->ncell * sizeof(float), s->stream), "zero abar");
-  for (int i = 0; i < 2 && !rc; i++)
-    rc = g->hip(hipMemsetAsync(g->vbar2[i], 0, s->ncell * sizeof(float), s->stream), "zero v_bar copy");
-  for (int i = 0; i < 4 && !rc; i++) {
-    rc = g->hip(hipMemsetAsync(g->psbar2[i], 0, (size_t)g->psi_elems(4 + i) * sizeof(float), s->stream), "zero psi_bar copy");
-    if (!rc) rc = g->hip(hipMemsetAsync(g->abar2[i], 0, s->ncell * sizeof(float), s->stream), "zero abar copy");
-  }
-  if (!rc && g->sched == VSE_ADJ_FUSED) rc = grad_rec_table(g);
   for (int i = 0; i < VSE_GR_COUNT && !rc; i++) {
     rc = g->hip(hipMemsetAsync(g->g[i], 0, s->ncell * sizeof(double), s->stream), "zero gradient");
     if (!rc) rc = g->hip(hipMemsetAsync(g->gf[i], 0, s->ncell * sizeof(float), s->stream), "zero partials");
@@ -1962,28 +1900,8 @@ int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms) 
       A.in = g->lev(sg, n - n0);
       A.out = g->lev(sg, n - n0 + 1);
       A.n = n;
-      hipError_t e;
-      if (g->sched == VSE_ADJ_FUSED) {
-        // copy 0 (bar, abar[0..3]) is read at even reverse steps N-1-n, copy 1 at odd ones
-        const bool odd = ((N - 1 - n) & 1) != 0;
-        AdjFused F;
-        float *c0v[2] = {g->bar.vx, g->bar.vy};
-        for (int k = 0; k < 2; k++) {
-          F.v_in[k] = odd ? g->vbar2[k] : c0v[k];
-          F.v_out[k] = odd ? c0v[k] : g->vbar2[k];
-        }
-        for (int k = 0; k < 4; k++) {
-          F.ps_in[k] = odd ? g->psbar2[k] : g->bar.psi[4 + k];
-          F.ps_out[k] = odd ? g->bar.psi[4 + k] : g->psbar2[k];
-          F.a_in[k] = odd ? g->abar2[k] : g->abar[k];
-          F.a_out[k] = odd ? g->abar[k] : g->abar2[k];
-        }
-        F.rec_row = g->rec_row; F.rp_cell = g->rp_cell; F.rp_rec = g->rp_rec; F.n_rp = g->n_rp;
-        e = vse_launch_adj_fused(A, F, g->shape, s->stream);
-      } else {
-        e = vse_launch_adj_velocity(A, s->stream);
-        if (e == hipSuccess) e = vse_launch_adj_stress(A, s->stream);
-      }
+      hipError_t e = vse_launch_adj_velocity(A, s->stream);
+      if (e == hipSuccess) e = vse_launch_adj_stress(A, s->stream);
       if (e == hipSuccess && n % g->flush == 0) e = vse_launch_grad_flush(A, s->stream);
       rc = g->hip(e, "adjoint step");
     }

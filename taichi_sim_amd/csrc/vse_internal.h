@@ -214,25 +214,11 @@ struct DiffArgs {
   int record;                        // forward: record traces of step n
 };
 
-// Buffers of the fused reverse step (k_adj_fused): the adjoint state its
-// velocity part reads at other workgroups' cells, read from *_in and written
-// to *_out, and the receiver points sorted by cell for the residual.
-struct AdjFused {
-  VSE_G float *v_in[2], *v_out[2];     // v_bar: vx, vy
-  VSE_G float *ps_in[4], *ps_out[4];   // psi_bar of the sigma slots (SL_SXXX .. SL_SYYY)
-  VSE_G float *a_in[4], *a_out[4];     // abar1..4
-  const VSE_G int *rec_row;            // per x row: 1 if a receiver point lies in it
-  const VSE_G int *rp_cell, *rp_rec;   // receiver points sorted by cell (then table order), their receiver
-  int n_rp;
-};
-constexpr int VSE_ADJ_SHAPES = 3;
-
 hipError_t vse_launch_dstress(const DiffArgs &A, hipStream_t st);
 hipError_t vse_launch_dvelocity(const DiffArgs &A, hipStream_t st);
 hipError_t vse_launch_adj_velocity(const DiffArgs &A, hipStream_t st);
 hipError_t vse_launch_src_map(int *map, long ncell, const int *cells, const int *cols, int n, hipStream_t st);
 hipError_t vse_launch_adj_stress(const DiffArgs &A, hipStream_t st);
-hipError_t vse_launch_adj_fused(const DiffArgs &A, const AdjFused &F, int shape, hipStream_t st);
 hipError_t vse_launch_grad_flush(const DiffArgs &A, hipStream_t st);
 hipError_t vse_launch_misfit(const float *traces, const float *obs, float *resid, long n, double *acc,
                              hipStream_t st);

@@ -295,52 +295,13 @@ def test_autograd_function_and_fused_path_agree():
     eng.close()
 
 
-@pytest.mark.parametrize("acc,n,scaled", [(2, 70, False), (2, 90, True), (4, 77, True)])
-def test_fused_reverse_step_matches_two_kernels(acc, n, scaled):
-    # One launch per reverse step (velocity adjoint recomputed over each tile's
-    # halo, abar5..8 kept in LDS, the rest double-buffered) against the two
-    # launches per step through device memory: the same operations on the same
-    # operands, so every gradient map must have the same bits, for every tile
-    # shape, ragged grids, an odd step count (the buffer swap), voids, FD order
-    # 4 (acc 4) and a per-cell tau scale (the scaled-Zener kernels).
-    steps = 101
-    cfg = workloads.visco_2d_array(n=n, steps=steps, n_elem=4, n_voids=2)
-    cfg["simul_params"]["acc"] = acc
-    cfg["simul_params"]["dt"] = 0.8 * cfg["simul_params"]["dt"]
-    sim = Simulator(cfg)
-    att = attenuation_from_config(cfg)
-    true, start = _perturbed(sim)
-    if not scaled:
-        start = start[:3] + (np.ones_like(start[3]),)
-    eng = VSEGrad(VSESim.from_simulator(sim, variant="wgsl", att=att), seg_len=10)
-    eng.set_model(*true)
-    obs = eng.forward(steps)
-    eng.set_model(*start)
-    eng.forward(steps)
-    eng.misfit(obs)
-    print(f"acc {acc}, {sim._nx}x{sim._ny}, tau scale {scaled}: two-kernel reverse step", flush=True)
-    eng.set_schedule(fused=False)
-    ref = eng.backward()
-    assert all(np.abs(ref[k]).max() > 0 for k in GRAD_NAMES)
-    for shape in range(3):
-        print(f"fused reverse step, tile shape {shape}", flush=True)
-        eng.set_schedule(fused=True, shape=shape)
-        got = eng.backward()
-        for k in GRAD_NAMES:
-            assert np.array_equal(got[k], ref[k]), (shape, k)
-    for bad in ((True, 3), (True, -1)):
-        with pytest.raises(VSEError):
-            eng.set_schedule(*bad)
-    eng.close()
-
-
 def test_overlapped_reverse_sweep_matches_serial():
     # With sqrt(N) checkpoints the default sweep recomputes segment s-1 on a
     # second stream while the adjoint of segment s runs; serial recomputes
     # each segment before its adjoint on one stream.  Same kernels on the same
     # inputs: the gradient must have the same bits, with the last segment
     # stored by the forward or recomputed (a second backward), for several
-    # spacings (a partial last segment included) and with the fused step.
+    # spacings (a partial last segment included).
     steps = 130
     sim, att = _setup(80, steps)
     true, start = _perturbed(sim)
@@ -352,18 +313,18 @@ def test_overlapped_reverse_sweep_matches_serial():
         eng.set_model(*start)
         eng.forward(steps)
         eng.misfit(obs)
-        eng.set_schedule(overlap=False)
+        eng.set_overlap(False)
         ref = eng.backward()
-        eng.set_schedule(overlap=True)
+        eng.set_overlap(True)
         runs = [eng.backward()]                 # last segment recomputed on the second stream
         eng.forward(steps)
         eng.misfit(obs)
         runs.append(eng.backward())             # last segment as the forward stored it
-        eng.set_schedule(fused=True, shape=2, overlap=True)
-        runs.append(eng.backward())
         for got in runs:
             for k in GRAD_NAMES:
                 assert np.array_equal(got[k], ref[k]), (seg, k)
+        with pytest.raises(VSEError):
+            eng._check(eng._lib.vse_grad_set_overlap(eng._h, 2))
         eng.close()
 
 

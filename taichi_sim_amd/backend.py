@@ -39,6 +39,12 @@ class HIPBackend:
     _variant = "wgsl"
     check_every = 100
 
+    # Called as frame_hook(step, frame) at the reference's display steps when the
+    # configuration sets show_anim: frame is the displayed slice of sigma_yy
+    # (2D) or sigma_zz (3D).  None: the frame goes to the Qt windows the
+    # simulator opened, if any (windows_qt.py needs PyQt6 and pyqtgraph).
+    frame_hook = None
+
     def implementation(self):
         super().implementation()
         att = attenuation_from_config(self._configs)
@@ -46,13 +52,55 @@ class HIPBackend:
         sim = VSESim.from_simulator(self, variant=self._variant, att=att)
         self._setup_time = time() - t_setup
         try:
-            ms = sim.run(self._n_steps, check_every=self.check_every, timed=True)
+            ms = 0.0
+            done = 0
+            # the reference shows a debug line and a frame after step it when
+            # it % it_display == 0 or it == 5 (sim_webgpu_viscoelastic.py:519-531,
+            # sim_taichi_elastic.py:320-328, sim_webgpu_3D_viscoelastic.py:680-689)
+            for it in self._display_steps():
+                ms += sim.run(it - done, check_every=self.check_every, timed=True)
+                done = it
+                self._display(sim, it)
+            ms += sim.run(self._n_steps - done, check_every=self.check_every, timed=True)
             field = sim.field("syy" if sim.ndim == 2 else "szz")
             traces = sim.traces()
             gpu = sim.device_name()
         finally:
             sim.close()
         return self._result(field, traces, gpu, ms / 1000.0, f"libvse {self._variant} variant, {sim.ndim}D, hipGraph step loop\n")
+
+    def _display_steps(self):
+        """Steps (1-based) after which show_debug / show_anim act: it % it_display
+        == 0 or it == 5; none when neither is set."""
+        if not (getattr(self, "_show_debug", False) or getattr(self, "_show_anim", False)):
+            return []
+        n, k = int(self._n_steps), max(1, int(self._it_display))
+        return sorted(set(range(k, n + 1, k)) | ({5} if n >= 5 else set()))
+
+    def _display(self, sim, it):
+        """The reference's per-display-step output: two debug lines (the running
+        max of |v|^2 as the WGSL backends print it, its square root; the Taichi
+        backend prints the squared value) and the field frame."""
+        if self._show_debug:
+            v2 = sim.v2max()
+            print(f'Time step # {it} out of {self._n_steps}')
+            if self._variant == "taichi":
+                print(f'Max norm velocity vector (m/s) = {v2}')
+            else:
+                print(f'Max norm velocity vector V (m/s) = {np.sqrt(v2)}')
+        if self._show_anim:
+            roi = self._roi
+            if sim.ndim == 2:
+                frame = sim.field("syy")[roi.get_ix_min():roi.get_ix_max(), roi.get_iz_min():roi.get_iz_max()]
+            else:
+                # the reference shows the y index gcd(ny, 16) // 2 (its workgroup width, halved)
+                frame = sim.field("szz")[roi.get_ix_min():roi.get_ix_max(), int(np.gcd(int(roi.get_ny()), 16)) // 2,
+                                         roi.get_iz_min():roi.get_iz_max()]
+            if self.frame_hook is not None:
+                self.frame_hook(it, frame)
+            elif getattr(self, "_windows_gpu", None):
+                self._windows_gpu[0].imv.setImage(frame, levels=[self._min_val_fields, self._max_val_fields])
+                self._app.processEvents()
 
     @staticmethod
     def _result(field, traces, gpu, sim_time, msg):

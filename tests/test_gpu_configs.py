@@ -2,7 +2,8 @@
 
 * ``implementation()`` of each backend class (the reference's plugin API,
   sim_support/simulator.py:337,608-611; Simulator3D.py:356): both result-key
-  families, the field and traces against the CPU oracle, a positive loop time.
+  families, the field and traces against the CPU oracle, a positive loop time;
+  and its show_debug / show_anim output at the reference's display steps.
 * Full-matrix capture batches (the batched multi-shot configuration):
   reduced sizes against solo runs bit for bit and one member against the
   oracle; the full sizes (512^2 .. 2048^2, 16 shots each) against solo runs
@@ -50,6 +51,55 @@ def test_implementation_2d_elastic_taichi():
     res = sim.implementation()
     o = oracle.Oracle(sim, variant="taichi").run(400)
     _check_result(res, o, "syy")
+
+
+@pytest.mark.parametrize("cls,dims", [(SimulatorHIP, 2), (SimulatorHIPTaichiElastic, 2), (SimulatorHIP3D, 3)])
+def test_implementation_display_steps(cls, dims, capsys):
+    # show_debug / show_anim: after step it with it % it_display == 0 or it == 5
+    # the reference prints two lines and shows sigma_yy (2D) or sigma_zz at y
+    # index gcd(ny, 16) // 2 (3D) over the ROI (sim_webgpu_viscoelastic.py
+    # :519-531, sim_taichi_elastic.py:320-328, sim_webgpu_3D_viscoelastic.py
+    # :680-689).  The run is split at those steps; its results must not change.
+    if dims == 2:
+        cfg = (workloads.elastic_2d_point(n=96, steps=230, n_rec=12) if cls is SimulatorHIPTaichiElastic
+               else workloads.visco_2d_array(n=96, steps=230, n_elem=8, n_voids=2))
+        field = "syy"
+    else:
+        cfg = workloads.visco_3d_block(n=(52, 48, 56), steps=46, n_incl=2, n_rec_side=3, t0=0.3)
+        field = "szz"
+    n = cfg["simul_params"]["time_steps"]
+    cfg["simul_params"]["it_display"] = 20 if dims == 2 else 9
+    plain = cls(cfg).implementation()
+    cfg["simul_configs"]["show_debug"] = True
+    cfg["simul_configs"]["show_anim"] = True
+    sim = cls(cfg)
+    frames = []
+    sim.frame_hook = lambda it, frame: frames.append((it, frame.copy()))
+    capsys.readouterr()
+    res = sim.implementation()
+    out = capsys.readouterr().out
+    k = cfg["simul_params"]["it_display"]
+    steps = sorted(set(range(k, n + 1, k)) | {5})
+    assert [it for it, _ in frames] == steps
+    assert out.count("Time step # ") == len(steps) and f"Time step # {steps[-1]} out of {n}" in out
+    assert ("Max norm velocity vector (m/s) = " if cls is SimulatorHIPTaichiElastic
+            else "Max norm velocity vector V (m/s) = ") in out
+    for key in ("stress", "sens_stress"):
+        assert np.array_equal(res[key], plain[key]), key
+    # a frame is the displayed slice of the field after its step
+    roi = sim._roi
+    it, last = frames[-1]
+    g = VSESim.from_simulator(sim, variant=sim._variant, att=attenuation_from_config(cfg))
+    g.run(it)
+    full = g.field(field)
+    g.close()
+    if dims == 2:
+        want = full[roi.get_ix_min():roi.get_ix_max(), roi.get_iz_min():roi.get_iz_max()]
+    else:
+        want = full[roi.get_ix_min():roi.get_ix_max(), int(np.gcd(int(roi.get_ny()), 16)) // 2,
+                    roi.get_iz_min():roi.get_iz_max()]
+    assert last.shape == want.shape and np.array_equal(last, want)
+    assert np.abs(last).max() > 0
 
 
 def test_implementation_3d_visco():

@@ -185,3 +185,25 @@ def test_timing_csv_and_description(tmp_path, capsys, n_iter):
         mse = sim._mse_values[-1]
         assert text[14:] == [f"MSE do campo de pressao: {mse[0]:.4}", f"MSE medio dos sensores de pressao: {mse[1]:.4}"]
         assert "Tempo medio de execucao" not in out
+
+
+def test_display_steps_follow_the_reference_rule():
+    # show_debug / show_anim act after step it when it % it_display == 0 or
+    # it == 5, it = 1..n_steps (sim_webgpu_viscoelastic.py:471,519); nothing
+    # when neither flag is set
+    from taichi_sim_amd.backend import HIPBackend
+
+    class Fake(HIPBackend):
+        pass
+
+    f = Fake()
+    f._n_steps, f._it_display = 230, 20
+    f._show_debug, f._show_anim = False, False
+    assert f._display_steps() == []
+    f._show_debug = True
+    assert f._display_steps() == [5, 20, 40, 60, 80, 100, 120, 140, 160, 180, 200, 220]
+    f._show_debug, f._show_anim = False, True
+    f._n_steps, f._it_display = 4, 2
+    assert f._display_steps() == [2, 4]
+    f._n_steps, f._it_display = 12, 5
+    assert f._display_steps() == [5, 10]

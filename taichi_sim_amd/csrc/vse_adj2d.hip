@@ -81,22 +81,36 @@ __device__ __forceinline__ int owned_band(const AxisProf &P, int c, int off) {
   return band_index(P, j);
 }
 
-// CPML stage of a forward pass at profile index j: psi' = b psi + a D from
-// psi of level n (q); returns D/k + psi' and leaves psi' in pn.
-__device__ __forceinline__ float cpml_step(const VSE_G float *a, const VSE_G float *b, const VSE_G float *k, int j,
-                                           float q, float D, float &pn) {
-  pn = b[j] * q + a[j] * D;
-  return D / k[j] + pn;
+// The profile values of one axis at a band cell (profile index j), fetched
+// with the cell's other operands: a wavefront holding band cells then has
+// them in flight with everything else instead of waiting on them one stage
+// at a time once its arithmetic has begun.  Outside the band they are unused.
+struct ProfAt {
+  float a, b, k, ah, bh, kh;
+};
+__device__ __forceinline__ ProfAt prof_at(const AxisProf &P, int j, bool band) {
+  ProfAt p{0.f, 0.f, 1.f, 0.f, 0.f, 1.f};
+  if (band) {
+    p.a = P.a[j]; p.b = P.b[j]; p.k = P.k[j];
+    p.ah = P.ah[j]; p.bh = P.bh[j]; p.kh = P.kh[j];
+  }
+  return p;
+}
+
+// CPML stage of a forward pass with profile values a, b, k: psi' = b psi + a D
+// from psi of level n (q); returns D/k + psi' and leaves psi' in pn.
+__device__ __forceinline__ float cpml_step(float a, float b, float k, float q, float D, float &pn) {
+  pn = b * q + a * D;
+  return D / k + pn;
 }
 
 // Adjoint of that stage: given the adjoint eb of the stretched derivative and
 // qb = adjoint of psi', returns the adjoint of the raw derivative and leaves
 // the adjoint of psi in qn.
-__device__ __forceinline__ float cpml_back(const VSE_G float *a, const VSE_G float *b, const VSE_G float *k, int j,
-                                           float qb, float eb, float &qn) {
+__device__ __forceinline__ float cpml_back(float a, float b, float k, float qb, float eb, float &qn) {
   const float pt = qb + eb;
-  qn = b[j] * pt;
-  return eb / k[j] + a[j] * pt;
+  qn = b * pt;
+  return eb / k + a * pt;
 }
 
 __device__ __forceinline__ float ld(const VSE_G float *f, int x, int y, int nx, int ny) {
@@ -236,6 +250,7 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
       for (int l = 0; l < LC; l++) oxy[l] = I.r[2][l * nc + id];
     }
   }
+  const ProfAt fx = prof_at(A.ax[0], jx, ex >= 0), fy = prof_at(A.ax[1], jy, ey >= 0);
   const float rs = TS ? 1.0f / ts : 1.0f;
 
   // ---- normal stresses
@@ -243,11 +258,11 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
     float pn;
     float dvx = diff<ORD>(A, tvx, A.dx), dvy = diff<ORD>(A, tvy, A.dy);
     if (ex >= 0) {
-      dvx = cpml_step(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, jx, qxx, dvx, pn);
+      dvx = cpml_step(fx.ah, fx.bh, fx.kh, qxx, dvx, pn);
       O.psi[SL_VXX][px] = pn;
     }
     if (ey >= 0) {
-      dvy = cpml_step(A.ax[1].a, A.ax[1].b, A.ax[1].k, jy, qyy, dvy, pn);
+      dvy = cpml_step(fy.a, fy.b, fy.k, qyy, dvy, pn);
       O.psi[SL_VYY][py] = pn;
     }
     const float rho_h = 0.5f * (rhoN + rho0);
@@ -285,11 +300,11 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
     float pn;
     float dvydx = diff<ORD>(A, tyx, A.dx), dvxdy = diff<ORD>(A, txy, A.dy);
     if (ex >= 0) {
-      dvydx = cpml_step(A.ax[0].a, A.ax[0].b, A.ax[0].k, jx, qyx, dvydx, pn);
+      dvydx = cpml_step(fx.a, fx.b, fx.k, qyx, dvydx, pn);
       O.psi[SL_VYX][px] = pn;
     }
     if (ey >= 0) {
-      dvxdy = cpml_step(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, jy, qxy, dvxdy, pn);
+      dvxdy = cpml_step(fy.ah, fy.bh, fy.kh, qxy, dvxdy, pn);
       O.psi[SL_VXY][py] = pn;
     }
     const float rho_h = 0.5f * (rhoS + rho0);
@@ -329,7 +344,8 @@ __device__ __forceinline__ unsigned dvelocity_cell(const DiffArgs &A, int x, int
   const bool inX = x >= last + 1 && x < nx - last && y >= last + 1 && y < ny - last;   // wgsl2d :998-1025
   const bool inY = x >= last && x < nx - last - 1 && y >= last && y < ny - last - 1;   // wgsl2d :1027-1057
 
-  // ---- loads
+  // ---- loads (the source map first: the source branch waits only for it)
+  const int col = A.src_map[id];
   float vxn = I.vx[id], vyn = I.vy[id];
   Taps<ORD> t1, t2, t3, t4;
   float q5 = 0.f, q6 = 0.f, q7 = 0.f, q8 = 0.f;
@@ -351,8 +367,8 @@ __device__ __forceinline__ unsigned dvelocity_cell(const DiffArgs &A, int x, int
     r11 = A.rho[id + ny + 1];
     r01 = A.rho[id + 1];
   }
+  const ProfAt fx = prof_at(A.ax[0], x - off, ex >= 0), fy = prof_at(A.ax[1], y - off, ey >= 0);
   // source after the update, anywhere (wgsl2d :1059-1065)
-  const int col = A.src_map[id];
   const bool src_on = col >= 0 && A.n < A.src_rows;
   float rho_src = 0.f, sval = 0.f;
   if (src_on) {
@@ -365,11 +381,11 @@ __device__ __forceinline__ unsigned dvelocity_cell(const DiffArgs &A, int x, int
     float pn;
     float dx1 = diff<ORD>(A, t1, A.dx), dx2 = diff<ORD>(A, t2, A.dy);
     if (ex >= 0) {
-      dx1 = cpml_step(A.ax[0].a, A.ax[0].b, A.ax[0].k, x - off, q5, dx1, pn);
+      dx1 = cpml_step(fx.a, fx.b, fx.k, q5, dx1, pn);
       O.psi[SL_SXXX][px] = pn;
     }
     if (ey >= 0) {
-      dx2 = cpml_step(A.ax[1].a, A.ax[1].b, A.ax[1].k, y - off, q6, dx2, pn);
+      dx2 = cpml_step(fy.a, fy.b, fy.k, q6, dx2, pn);
       O.psi[SL_SXYY][py] = pn;
     }
     if (rhoX > 0.f) vxn = (dx1 + dx2) * dt / rhoX + vxn;
@@ -382,11 +398,11 @@ __device__ __forceinline__ unsigned dvelocity_cell(const DiffArgs &A, int x, int
     float dy1 = diff<ORD>(A, t3, A.dx), dy2 = diff<ORD>(A, t4, A.dy);
     const float rhoY = 0.25f * (r00 + r10 + r11 + r01);
     if (ex >= 0) {
-      dy1 = cpml_step(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, x - off, q7, dy1, pn);
+      dy1 = cpml_step(fx.ah, fx.bh, fx.kh, q7, dy1, pn);
       O.psi[SL_SXYX][px] = pn;
     }
     if (ey >= 0) {
-      dy2 = cpml_step(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, y - off, q8, dy2, pn);
+      dy2 = cpml_step(fy.ah, fy.bh, fy.kh, q8, dy2, pn);
       O.psi[SL_SYYY][py] = pn;
     }
     if (rhoY > 0.f) vyn = (dy1 + dy2) * dt / rhoY + vyn;
@@ -472,6 +488,7 @@ __global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
   // ---- loads
   // adjoint of v_{n+1}: carried part + transposed stencils of the stress
   // pass of step n+1 (a1 = D+x vx, a4 = D+y vx, a2 = D-y vy, a3 = D-x vy)
+  const int col = A.src_map[id];     // first: the source branch waits only for it
   const float bvx = B.vx[id], bvy = B.vy[id];
   const Taps<ORD> u0 = tapsT<ORD, true, 0>(A, A.abar[0], x, y), u3 = tapsT<ORD, true, 1>(A, A.abar[3], x, y);
   const Taps<ORD> u1 = tapsT<ORD, false, 1>(A, A.abar[1], x, y), u2 = tapsT<ORD, false, 0>(A, A.abar[2], x, y);
@@ -508,7 +525,7 @@ __global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
       b8 = B.psi[SL_SYYY][py];
     }
   }
-  const int col = A.src_map[id];
+  const ProfAt fx = prof_at(A.ax[0], x - off, ex >= 0), fy = prof_at(A.ax[1], y - off, ey >= 0);
   const bool src_on = col >= 0 && A.n < A.src_rows;
   float rho_src = 0.f, sval = 0.f;
   if (src_on) {
@@ -527,12 +544,12 @@ __global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
       eb = vxb * dt / rhoX;
       const float dx1 = diffA<ORD, 0>(A, t1), dx2 = diffA<ORD, 1>(A, t2);
       // e = D/k + psi', psi' as stored in level n+1
-      const float e5 = ex >= 0 ? dx1 / A.ax[0].k[x - off] + p5 : dx1;
-      const float e6 = ey >= 0 ? dx2 / A.ax[1].k[y - off] + p6 : dx2;
+      const float e5 = ex >= 0 ? dx1 / fx.k + p5 : dx1;
+      const float e6 = ey >= 0 ? dx2 / fy.k + p6 : dx2;
       gx = gx + -(e5 + e6) * eb / rhoX;
     }
-    a5b = ex >= 0 ? cpml_back(A.ax[0].a, A.ax[0].b, A.ax[0].k, x - off, b5, eb, b5) : eb;
-    a6b = ey >= 0 ? cpml_back(A.ax[1].a, A.ax[1].b, A.ax[1].k, y - off, b6, eb, b6) : eb;
+    a5b = ex >= 0 ? cpml_back(fx.a, fx.b, fx.k, b5, eb, b5) : eb;
+    a6b = ey >= 0 ? cpml_back(fy.a, fy.b, fy.k, b6, eb, b6) : eb;
   }
   float ty = 0.f;
   if (inY) {
@@ -541,12 +558,12 @@ __global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
     if (rhoY > 0.f) {
       eb = vyb * dt / rhoY;
       const float dy1 = diffA<ORD, 0>(A, t3), dy2 = diffA<ORD, 1>(A, t4);
-      const float e7 = ex >= 0 ? dy1 / A.ax[0].kh[x - off] + p7 : dy1;
-      const float e8 = ey >= 0 ? dy2 / A.ax[1].kh[y - off] + p8 : dy2;
+      const float e7 = ex >= 0 ? dy1 / fx.kh + p7 : dy1;
+      const float e8 = ey >= 0 ? dy2 / fy.kh + p8 : dy2;
       ty += -(e7 + e8) * eb / rhoY;
     }
-    a7b = ex >= 0 ? cpml_back(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, x - off, b7, eb, b7) : eb;
-    a8b = ey >= 0 ? cpml_back(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, y - off, b8, eb, b8) : eb;
+    a7b = ex >= 0 ? cpml_back(fx.ah, fx.bh, fx.kh, b7, eb, b7) : eb;
+    a8b = ey >= 0 ? cpml_back(fy.ah, fy.bh, fy.kh, b8, eb, b8) : eb;
   }
   // the source term dt*src/rho_vy of step n
   if (src_on && rho_src > 0.f) ty += -(sval * dt / rho_src) * vyb / rho_src;
@@ -656,6 +673,7 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
       }
     }
   }
+  const ProfAt fx = prof_at(A.ax[0], x - off, ex >= 0), fy = prof_at(A.ax[1], y - off, ey >= 0);
   const float rs = TS ? 1.0f / ts : 1.0f;
 
   // ---- adjoint of the stress pass
@@ -667,8 +685,8 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
   if (inN) {
     float pn;
     const float d1 = diffA<ORD, 0>(A, t1), d2 = diffA<ORD, 1>(A, t2);
-    const float e1 = ex >= 0 ? cpml_step(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, x - off, q1, d1, pn) : d1;
-    const float e2 = ey >= 0 ? cpml_step(A.ax[1].a, A.ax[1].b, A.ax[1].k, y - off, q2, d2, pn) : d2;
+    const float e1 = ex >= 0 ? cpml_step(fx.ah, fx.bh, fx.kh, q1, d1, pn) : d1;
+    const float e2 = ey >= 0 ? cpml_step(fy.a, fy.b, fy.k, q2, d2, pn) : d2;
     const float rho_h = 0.5f * (rhoN + rho0);
     const float cp_h = 0.5f * (cpN + cp0);
     const float cs_l = 0.5f * (csN + cs0);
@@ -701,14 +719,14 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
     }
     gl = gl + (P + Q) * (e1 + e2 + 0.5f * sum_xx) * dt;
     gm = gm + (P * (2.0f * e1 + 0.5f * sum_xx + sum_yy) + Q * (2.0f * e2 + 0.5f * sum_xx - sum_yy)) * dt;
-    a1b = ex >= 0 ? cpml_back(A.ax[0].ah, A.ax[0].bh, A.ax[0].kh, x - off, b1, e1b, b1) : e1b;
-    a2b = ey >= 0 ? cpml_back(A.ax[1].a, A.ax[1].b, A.ax[1].k, y - off, b2, e2b, b2) : e2b;
+    a1b = ex >= 0 ? cpml_back(fx.ah, fx.bh, fx.kh, b1, e1b, b1) : e1b;
+    a2b = ey >= 0 ? cpml_back(fy.a, fy.b, fy.k, b2, e2b, b2) : e2b;
   }
   if (inS) {
     float pn;
     const float d3 = diffA<ORD, 0>(A, t3), d4 = diffA<ORD, 1>(A, t4);
-    const float e3 = ex >= 0 ? cpml_step(A.ax[0].a, A.ax[0].b, A.ax[0].k, x - off, q3, d3, pn) : d3;
-    const float e4 = ey >= 0 ? cpml_step(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, y - off, q4, d4, pn) : d4;
+    const float e3 = ex >= 0 ? cpml_step(fx.a, fx.b, fx.k, q3, d3, pn) : d3;
+    const float e4 = ey >= 0 ? cpml_step(fy.ah, fy.bh, fy.kh, q4, d4, pn) : d4;
     const float rho_h = 0.5f * (rhoS + rho0);
     const float cs_h = (fminf(csS, cs0) == 0.f) ? 0.f : 0.5f * (csS + cs0);
     const float mu = rho_h * (cs_h * cs_h);
@@ -728,8 +746,8 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
       gs += Rl * (z.m_s * rs) * (o * z.h_s - X * z.phi_s + n * z.h_s);
     }
     gmxy = gmxy + R * (X + 0.5f * sum_xy) * dt;
-    a3b = ex >= 0 ? cpml_back(A.ax[0].a, A.ax[0].b, A.ax[0].k, x - off, b3, eb, b3) : eb;
-    a4b = ey >= 0 ? cpml_back(A.ax[1].ah, A.ax[1].bh, A.ax[1].kh, y - off, b4, eb, b4) : eb;
+    a3b = ex >= 0 ? cpml_back(fx.a, fx.b, fx.k, b3, eb, b3) : eb;
+    a4b = ey >= 0 ? cpml_back(fy.ah, fy.bh, fy.kh, b4, eb, b4) : eb;
   }
 
   // ---- stores

@@ -344,8 +344,8 @@ __device__ __forceinline__ unsigned dvelocity_cell(const DiffArgs &A, int x, int
   const bool inX = x >= last + 1 && x < nx - last && y >= last + 1 && y < ny - last;   // wgsl2d :998-1025
   const bool inY = x >= last && x < nx - last - 1 && y >= last && y < ny - last - 1;   // wgsl2d :1027-1057
 
-  // ---- loads (the source map first: the source branch waits only for it)
-  const int col = A.src_map[id];
+  // ---- loads (the source row flag first: the source-map load waits only for it)
+  const int srow = A.src_rowflag[x];
   float vxn = I.vx[id], vyn = I.vy[id];
   Taps<ORD> t1, t2, t3, t4;
   float q5 = 0.f, q6 = 0.f, q7 = 0.f, q8 = 0.f;
@@ -369,7 +369,8 @@ __device__ __forceinline__ unsigned dvelocity_cell(const DiffArgs &A, int x, int
   }
   const ProfAt fx = prof_at(A.ax[0], x - off, ex >= 0), fy = prof_at(A.ax[1], y - off, ey >= 0);
   // source after the update, anywhere (wgsl2d :1059-1065)
-  const bool src_on = col >= 0 && A.n < A.src_rows;
+  const int col = (A.n < A.src_rows && srow) ? A.src_map[id] : -1;
+  const bool src_on = col >= 0;
   float rho_src = 0.f, sval = 0.f;
   if (src_on) {
     rho_src = rho_vy_edge(A, x, y, id);
@@ -488,7 +489,7 @@ __global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
   // ---- loads
   // adjoint of v_{n+1}: carried part + transposed stencils of the stress
   // pass of step n+1 (a1 = D+x vx, a4 = D+y vx, a2 = D-y vy, a3 = D-x vy)
-  const int col = A.src_map[id];     // first: the source branch waits only for it
+  const int srow = A.src_rowflag[x];   // first: the source-map load waits only for it
   const float bvx = B.vx[id], bvy = B.vy[id];
   const Taps<ORD> u0 = tapsT<ORD, true, 0>(A, A.abar[0], x, y), u3 = tapsT<ORD, true, 1>(A, A.abar[3], x, y);
   const Taps<ORD> u1 = tapsT<ORD, false, 1>(A, A.abar[1], x, y), u2 = tapsT<ORD, false, 0>(A, A.abar[2], x, y);
@@ -526,7 +527,8 @@ __global__ __launch_bounds__(BX * BY) void k_adj_velocity(DiffArgs A) {
     }
   }
   const ProfAt fx = prof_at(A.ax[0], x - off, ex >= 0), fy = prof_at(A.ax[1], y - off, ey >= 0);
-  const bool src_on = col >= 0 && A.n < A.src_rows;
+  const int col = (A.n < A.src_rows && srow) ? A.src_map[id] : -1;
+  const bool src_on = col >= 0;
   float rho_src = 0.f, sval = 0.f;
   if (src_on) {
     rho_src = rho_vy_edge(A, x, y, id);

@@ -462,7 +462,19 @@ int vse_set_sources(vse_sim *s, int n_pts, const int *cells, const int *cols, in
     if (!rc && n_pts) rc = s->hip(hipMemcpyAsync(s->src_rcol, rcol.data(), n_pts * sizeof(int), hipMemcpyHostToDevice, s->stream), "src row col");
   }
   if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "set_sources");
-  if (rc) return rc;
+  if (rc) {
+    // leave no partial source set: vse_run then refuses the simulation instead
+    // of launching kernels on a missing map or row table
+    (void)hipStreamSynchronize(s->stream);
+    s->release(s->src_cell); s->release(s->src_col); s->release(s->src); s->release(s->src_map);
+    s->release(s->src_rowflag);
+    s->release(s->src_rbeg); s->release(s->src_rz); s->release(s->src_rcol);
+    s->src_cell = s->src_col = s->src_map = s->src_rowflag = nullptr;
+    s->src_rbeg = s->src_rz = s->src_rcol = nullptr;
+    s->src = nullptr;
+    s->n_src_pts = 0;
+    return rc;
+  }
   s->n_src_pts = n_pts;
   s->src_cols = n_cols;
   s->drop_graph();
@@ -1597,6 +1609,7 @@ static void make_diff(vse_grad *g, DiffArgs &A) {
   }
   A.v2max = g->ctrl;
   A.src_map = g->src_map; A.src = s->src; A.src_cols = s->src_cols; A.src_rows = d.n_rows;
+  A.src_rowflag = s->src_rowflag;
   A.n_rec = s->n_rec; A.rec_gate = s->rec_gate; A.rec_beg = s->rec_beg; A.rec_cell = s->rec_cell;
   A.rec_delay = s->rec_delay; A.traces = g->traces; A.gbar = g->gbar;
 }
@@ -1839,6 +1852,9 @@ int vse_grad_backward(vse_grad *g, const float *grad_traces, float *elapsed_ms) 
   const int N = g->n_fwd;
   if (N < 1) return g->fail(VSE_ERR_STATE, "vse_grad_forward not run");
   vse_sim *s = g->s;
+  // the velocity adjoint reads the source row table (a failed vse_set_sources
+  // since the forward leaves none)
+  if (!s->src || !s->src_rowflag) return g->fail(VSE_ERR_STATE, "vse_set_sources not called");
   int rc = 0;
   if (grad_traces)
     rc = g->hip(hipMemcpyAsync(g->gbar, grad_traces, (size_t)N * s->n_rec * sizeof(float), hipMemcpyHostToDevice,

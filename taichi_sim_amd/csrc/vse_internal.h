@@ -205,6 +205,7 @@ struct DiffArgs {
   // sources and receivers, handled inside the velocity kernels
   int n;                             // the step being advanced / reversed
   const VSE_G int *src_map;          // per cell: source column or -1
+  const VSE_G int *src_rowflag;      // per x-row: 1 if it holds a source cell
   const VSE_G float *src;            // (src_rows, src_cols)
   int src_cols, src_rows;
   int n_rec, rec_gate;

@@ -21,13 +21,6 @@ __device__ __forceinline__ float dterm2(const StepArgs &A, int c, float a, float
   else return fd_term<VAR>(A.C[c], a, b, AX == 0 ? A.dx : A.dy, AX == 0 ? A.one_dx : A.one_dy);
 }
 
-// D / k, or with FAST D times the precomputed f32(1/k)
-template <bool FAST>
-__device__ __forceinline__ float kdiv(float D, const float *k, const float *ki, int j) {
-  if constexpr (FAST) return D * ki[j];
-  else return D / k[j];
-}
-
 
 // LC > 0: the mechanism count is the compile-time LC and every memory-variable
 // load of a point is issued before its arithmetic (same operations, same
@@ -171,12 +164,26 @@ struct VelOps2 {
   float r00, r10, r11, r01;
 };
 
+// The WGSL-variant source flag of row x (1 on rows holding a source cell),
+// loaded unconditionally before a cell's stencil operands so that the
+// source-map load waits only for it; 0 for the Taichi variant (its source is
+// part of the update).
+template <int VAR>
+__device__ __forceinline__ int source_row(const StepArgs &A, int x) {
+  if constexpr (VAR == VSE_VAR_WGSL) return A.src_rowflag[x];
+  else return 0;
+}
+
 // The velocity update of one cell (x, y) < (nx, ny) from its operands: the
-// CPML band loads, the two updates, the Dirichlet rules and the stores;
-// returns |v|^2 as float bits.  s: the step being advanced (the
-// Taichi-variant source row).
+// CPML band loads, the two updates, the Dirichlet rules, the WGSL-variant
+// source and the stores.  Returns |v|^2 as float bits, the larger of its
+// values before and after the source.  s: the step being advanced; srow:
+// source_row of the cell's row.  The WGSL source comes after the update,
+// anywhere on the grid (wgsl2d :1059-1065): vy += src[s, col] * dt / rho at
+// each source cell, rho the vy-grid density with out-of-grid neighbours 0.
 template <int VAR, bool VISCO, int ORD, bool FAST = false>
-__device__ __forceinline__ unsigned velocity2d_apply(const StepArgs &A, int x, int y, int s, VelOps2<ORD> &o) {
+__device__ __forceinline__ unsigned velocity2d_apply(const StepArgs &A, int x, int y, int s, int srow,
+                                                     VelOps2<ORD> &o) {
   const int nx = A.nx, ny = A.ny;
   constexpr int last = ORD - 1, off = ORD - 1;
   const long id = (long)x * ny + y;
@@ -198,6 +205,25 @@ __device__ __forceinline__ unsigned velocity2d_apply(const StepArgs &A, int x, i
     if (by >= 0) q4 = A.psi[P_DSYY_DY][pyi];
     if constexpr (VAR == VSE_VAR_TAICHI) is = A.src_map[id];
   }
+  if constexpr (VAR == VSE_VAR_WGSL)
+    if (s < A.src_rows && srow) is = A.src_map[id];
+  // the band's profile values, with the other loads (a wavefront holding band
+  // cells would otherwise wait on each of them inside its arithmetic)
+  float xa = 0.f, xb = 0.f, xk = 1.f, xah = 0.f, xbh = 0.f, xkh = 1.f;
+  float ya = 0.f, yb = 0.f, yk = 1.f, yah = 0.f, ybh = 0.f, ykh = 1.f;
+  if (bx >= 0) {
+    const AxisProf &P = A.ax[0];
+    xa = P.a[jx]; xb = P.b[jx]; xk = FAST ? P.ki[jx] : P.k[jx];
+    xah = P.ah[jx]; xbh = P.bh[jx]; xkh = FAST ? P.khi[jx] : P.kh[jx];
+  }
+  if (by >= 0) {
+    const AxisProf &P = A.ax[1];
+    ya = P.a[jy]; yb = P.b[jy]; yk = FAST ? P.ki[jy] : P.k[jy];
+    yah = P.ah[jy]; ybh = P.bh[jy]; ykh = FAST ? P.khi[jy] : P.kh[jy];
+  }
+  float sval = 0.f;
+  if constexpr (VAR == VSE_VAR_WGSL)
+    if (is >= 0) sval = A.src[(long)s * A.src_cols + is];
   // ---- vx (wgsl2d :998-1025)
   if (in_vx) {
     float d1 = 0.f, d2 = 0.f;
@@ -207,14 +233,14 @@ __device__ __forceinline__ unsigned velocity2d_apply(const StepArgs &A, int x, i
       d2 += dterm2<VAR, FAST, 1>(A, c, o.xy_p[c], o.xy_m[c]);
     }
     if (bx >= 0) {
-      const float pn = A.ax[0].b[jx] * q1 + A.ax[0].a[jx] * d1;
+      const float pn = xb * q1 + xa * d1;
       A.psi[P_DSXX_DX][pxi] = pn;
-      d1 = kdiv<FAST>(d1, A.ax[0].k, A.ax[0].ki, jx) + pn;
+      d1 = (FAST ? d1 * xk : d1 / xk) + pn;
     }
     if (by >= 0) {
-      const float pn = A.ax[1].b[jy] * q2 + A.ax[1].a[jy] * d2;
+      const float pn = yb * q2 + ya * d2;
       A.psi[P_DSXY_DY][pyi] = pn;
-      d2 = kdiv<FAST>(d2, A.ax[1].k, A.ax[1].ki, jy) + pn;
+      d2 = (FAST ? d2 * yk : d2 / yk) + pn;
     }
     if (o.r00 > 0.f) {
       if constexpr (VAR == VSE_VAR_TAICHI) vxn += (d1 + d2) * dt / o.r00;
@@ -232,14 +258,14 @@ __device__ __forceinline__ unsigned velocity2d_apply(const StepArgs &A, int x, i
       d2 += dterm2<VAR, FAST, 1>(A, c, o.yy_p[c], o.yy_m[c]);
     }
     if (bx >= 0) {
-      const float pn = A.ax[0].bh[jx] * q3 + A.ax[0].ah[jx] * d1;
+      const float pn = xbh * q3 + xah * d1;
       A.psi[P_DSXY_DX][pxi] = pn;
-      d1 = kdiv<FAST>(d1, A.ax[0].kh, A.ax[0].khi, jx) + pn;
+      d1 = (FAST ? d1 * xkh : d1 / xkh) + pn;
     }
     if (by >= 0) {
-      const float pn = A.ax[1].bh[jy] * q4 + A.ax[1].ah[jy] * d2;
+      const float pn = ybh * q4 + yah * d2;
       A.psi[P_DSYY_DY][pyi] = pn;
-      d2 = kdiv<FAST>(d2, A.ax[1].kh, A.ax[1].khi, jy) + pn;
+      d2 = (FAST ? d2 * ykh : d2 / ykh) + pn;
     }
     if constexpr (VAR == VSE_VAR_TAICHI) {
       const float rho = 0.25f * (o.r01 + o.r00 + o.r11 + o.r10);
@@ -256,9 +282,21 @@ __device__ __forceinline__ unsigned velocity2d_apply(const StepArgs &A, int x, i
     if constexpr (VAR == VSE_VAR_WGSL) vxn = 0.f;   // the vy else zeroes both (wgsl2d :1054-1057)
     vyn = 0.f;
   }
+  unsigned u = __float_as_uint(vxn * vxn + vyn * vyn);
+  if constexpr (VAR == VSE_VAR_WGSL) {
+    if (is >= 0) {
+      const bool xe = x + 1 < nx, ye = y + 1 < ny;
+      const float rho = 0.25f * (o.r00 + (xe ? o.r10 : 0.f) + ((xe && ye) ? o.r11 : 0.f) + (ye ? o.r01 : 0.f));
+      if (rho > 0.f) {
+        vyn = vyn + sval * A.dt / rho;
+        const unsigned w = __float_as_uint(vxn * vxn + vyn * vyn);
+        u = u > w ? u : w;
+      }
+    }
+  }
   A.vx[id] = vxn;
   A.vy[id] = vyn;
-  return __float_as_uint(vxn * vxn + vyn * vyn);
+  return u;
 }
 
 // All loads of a cell are issued before its arithmetic: raw stencil operands
@@ -267,7 +305,9 @@ __device__ __forceinline__ unsigned velocity2d_apply(const StepArgs &A, int x, i
 // the wavefront wait on it before issuing the loads that follow.  Every
 // stencil operand is loaded unconditionally through the field buffers (reads
 // outside the grid give 0; operands outside an update's interior are loaded
-// but not used); the CPML band and source map only where they apply.
+// but not used); the CPML band and source map only where they apply.  The
+// source row flag is loaded first, so that the source-map load waits for it
+// alone and not for the stencil operands issued after it.
 template <int VAR, bool VISCO, int ORD, bool FAST = false>
 __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, int y, int s) {
   const int nx = A.nx, ny = A.ny;
@@ -275,6 +315,7 @@ __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, in
   const FieldBuf Bvx(A.vx, A.ncell), Bvy(A.vy, A.ncell), Bsxx(A.sxx, A.ncell), Bsxy(A.sxy, A.ncell),
       Bsyy(A.syy, A.ncell), Brho(A.rho, A.ncell);
   const int i = (int)((long)x * ny + y);
+  const int srow = source_row<VAR>(A, x);
   VelOps2<ORD> o;
   o.vxn = Bvx[i];
   o.vyn = Bvy[i];
@@ -291,7 +332,7 @@ __device__ __forceinline__ unsigned velocity2d_cell(const StepArgs &A, int x, in
   }
   // in the vy range x+1 and y+1 are inside the grid
   o.r00 = Brho[i]; o.r10 = Brho[i + ny]; o.r11 = Brho[i + ny + 1]; o.r01 = Brho[i + 1];
-  return velocity2d_apply<VAR, VISCO, ORD, FAST>(A, x, y, s, o);
+  return velocity2d_apply<VAR, VISCO, ORD, FAST>(A, x, y, s, srow, o);
 }
 
 // Tile of a 1D launch over nty (y) x ntx (x) tiles of BY x BX cells, y-tile
@@ -322,10 +363,7 @@ __global__ __launch_bounds__(BX * BY) void k_stress2d(StepArgs A, int nty, int r
 
 // The step's sparse work, done inside the velocity pass (vse_sparse.hip does
 // it for 3D, in the same way and order):
-// * WGSL-variant source: vy += src[s, col] * dt / rho at each source cell,
-//   after the update, anywhere on the grid, rho the vy-grid density with
-//   out-of-grid neighbours 0 (wgsl2d :1059-1065); |v|^2 of the cell is
-//   counted before and after;
+// * the WGSL-variant source, part of each cell's update (velocity2d_apply);
 // * receivers of step s, by one workgroup: traces[s, r] += the channel
 //   (sigma_yy by default) over the receiver's points in table order when
 //   s + gate >= delay (wgsl2d :1117-1134; sim_taichi_elastic.py:187-190).  A
@@ -335,28 +373,6 @@ __global__ __launch_bounds__(BX * BY) void k_stress2d(StepArgs A, int nty, int r
 // The step is s = *A.step + i: a graph of k steps passes i = 0..k-1 and ends
 // with an advance of the counter by k, so it moves once per graph launch and
 // no workgroup waits for another (wgsl2d :1136-1141 advances it per step).
-template <int VAR>
-__device__ __forceinline__ void source_cell(const StepArgs &A, int x, int y, int s, unsigned &v2) {
-  if constexpr (VAR == VSE_VAR_WGSL) {
-    if (x < A.nx && y < A.ny && s < A.src_rows && A.src_rowflag[x]) {
-      const long id = (long)x * A.ny + y;
-      const int col = A.src_map[id];
-      if (col >= 0) {
-        const bool xe = x + 1 < A.nx, ye = y + 1 < A.ny;
-        const float rho = 0.25f * (A.rho[id] + (xe ? A.rho[id + A.ny] : 0.f) +
-                                   ((xe && ye) ? A.rho[id + A.ny + 1] : 0.f) + (ye ? A.rho[id + 1] : 0.f));
-        if (rho > 0.f) {
-          const float vy = A.vy[id] + A.src[(long)s * A.src_cols + col] * A.dt / rho;
-          A.vy[id] = vy;
-          const float vx = A.vx[id];
-          const unsigned u = __float_as_uint(vx * vx + vy * vy);
-          v2 = v2 > u ? v2 : u;
-        }
-      }
-    }
-  }
-}
-
 __device__ __forceinline__ void record_receivers(const StepArgs &A, int s) {
   if (s >= A.trace_rows) return;
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
@@ -378,8 +394,7 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d(StepArgs A, int nty, int
   const int s = *A.step + i;
   unsigned v2 = 0;
   for (int k = 0; k < rows; k++) {
-    unsigned u = velocity2d_cell<VAR, VISCO, ORD, FAST>(A, x0 + k * BX, y, s);
-    source_cell<VAR>(A, x0 + k * BX, y, s, u);
+    const unsigned u = velocity2d_cell<VAR, VISCO, ORD, FAST>(A, x0 + k * BX, y, s);
     v2 = v2 > u ? v2 : u;
   }
   if (blockIdx.x == 0 && !A.rec_after) record_receivers(A, s);
@@ -418,6 +433,7 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d_march(StepArgs A, int nt
 #pragma unroll 1
     for (int x = x0; x < x1; x++) {
       const int ic = (int)((long)x * ny + y);
+      const int srow = source_row<VAR>(A, x);
 #pragma unroll
       for (int j = 0; j < 2 * ORD - 1; j++) { wxx[j] = wxx[j + 1]; wxy[j] = wxy[j + 1]; }
       wxx[2 * ORD - 1] = Bsxx[ic + (ORD - 1) * ny];
@@ -442,8 +458,7 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d_march(StepArgs A, int nt
       rn1 = Brho[ic + ny + 1];
       o.r10 = rn0;
       o.r11 = rn1;
-      unsigned u = velocity2d_apply<VAR, VISCO, ORD, FAST>(A, x, y, s, o);
-      source_cell<VAR>(A, x, y, s, u);
+      const unsigned u = velocity2d_apply<VAR, VISCO, ORD, FAST>(A, x, y, s, srow, o);
       v2 = v2 > u ? v2 : u;
     }
   }
@@ -513,8 +528,7 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B, int i
   unsigned v2 = 0;
   const int s = *A.step + i;
   for (int k = 0; k < B.rows; k++) {
-    unsigned u = velocity2d_cell<VAR, VISCO, ORD, FAST>(A, x0 + k * BX, y, s);
-    source_cell<VAR>(A, x0 + k * BX, y, s, u);
+    const unsigned u = velocity2d_cell<VAR, VISCO, ORD, FAST>(A, x0 + k * BX, y, s);
     v2 = v2 > u ? v2 : u;
   }
   if (t == 0 && !A.rec_after) record_receivers(A, s);

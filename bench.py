@@ -74,7 +74,7 @@ STRESS_TRAFFIC_512 = {(4, 0, 4, 2): 31.86e9, (4, 0, 8, 2): 32.02e9, (4, 0, 16, 2
                       (1, 0, 8, 1): 31.75e9}
 # The same for the 2048^2 2D viscoelastic stress kernel at its tuned shape
 # (profiles/visco2d_2048_kernels.md).
-STRESS2D_TRAFFIC_2048 = 494.8e6
+STRESS2D_TRAFFIC_2048 = 489.2e6
 
 
 def bytes_per_cell(g, L):

@@ -386,6 +386,7 @@ def main():
         # own configuration, measured after the 3D handle is released
         s = measure_shots((512, 1024, 1536, 2048), 16, args.shots_window, profile_steps=0)
         line["shots_per_s"] = {"batched": s["value"], "per_shot_loop": s["per_shot_loop_shots_per_s"],
+                               "batched_persistent": s["batched_persistent_shots_per_s"],
                                "speedup": s["batch_speedup"], "shots": s["config"]["shots"],
                                "window_steps": s["config"]["window_steps"], "shot_steps": 3000,
                                "cache_groups": s["config"]["cache_groups"], "tuning": s["tuning"],
@@ -504,6 +505,13 @@ def measure_shots(sizes, n_shots, steps, warmup=0, profile_steps=10, group_sweep
           file=sys.stderr, flush=True)
     wall_l, _ = timed(lambda n: [g.run(n, check_every=check) for g in sims])
     print(f"# per-shot loop: {wall_l:.3f} s", file=sys.stderr, flush=True)
+    # the same batch under the persistent schedule: each group's window as
+    # cooperative launches of up to `check` steps (grid-wide barriers between passes)
+    batch.set_schedule("persistent")
+    wall_p, dev_p = timed(lambda n: batch.run(n, check_every=check, timed=True))
+    batch.set_schedule("graph")
+    print(f"# batch, persistent schedule: {steps} steps in {wall_p:.3f} s (device {dev_p / 1e3:.3f} s)",
+          file=sys.stderr, flush=True)
     sweep = None
     if group_sweep:
         # batch Gcell/s at each cache-group budget (MB of member device memory per group)
@@ -537,6 +545,7 @@ def measure_shots(sizes, n_shots, steps, warmup=0, profile_steps=10, group_sweep
                    "shots": len(sims), "cells_per_step": cells, "shot_steps": shot_len, "window_steps": steps,
                    "parallelism": "single GPU, one batch", "cache_groups": len(groups)},
         "per_shot_loop_shots_per_s": round(shots / wall_l, 4),
+        "batched_persistent_shots_per_s": round(shots / wall_p, 4),
         "group_sweep_gcell_per_s": sweep,
         "batch_speedup": round(wall_l / wall_b, 3),
         "gcell_per_s_batched": round(cells * steps / wall_b / 1e9, 3),

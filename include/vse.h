@@ -294,6 +294,19 @@ int vse_batch_set_rows(vse_batch *b, int rows);
 /* Cache budget of a group in bytes of member device memory; 0 puts every
  * member in one group (all members advance in lockstep).  Speed only. */
 int vse_batch_set_group_bytes(vse_batch *b, size_t bytes);
+/* Schedule of vse_batch_run.  VSE_BATCH_SCHED_GRAPH (the default) is the one
+ * described above.  VSE_BATCH_SCHED_PERSISTENT runs every group, a group of
+ * one member included, as cooperative launches of up to check_every steps
+ * (at most 1024): the workgroups, all resident at once, stride over the
+ * group's tiles, and grid-wide barriers separate the passes, so a group's
+ * steps are one launch instead of two or three per step.  Results are
+ * identical to the graph schedule; only speed changes.  VSE_ERR_HIP when
+ * the device cannot launch cooperative kernels.  No reference counterpart:
+ * the reference runs one simulation per emission law in a host loop
+ * (sim_support/simulator.py:328-337). */
+#define VSE_BATCH_SCHED_GRAPH 0
+#define VSE_BATCH_SCHED_PERSISTENT 1
+int vse_batch_set_schedule(vse_batch *b, int schedule);
 /* Number of groups; writes the first member index of each (up to max_groups). */
 int vse_batch_groups(vse_batch *b, int *first, int max_groups);
 int vse_batch_size(vse_batch *b);

@@ -55,7 +55,7 @@ EXPORTS = ["vse_abi_version", "vse_create", "vse_set_material", "vse_set_cpml", 
            "vse_read_memory", "vse_read_v2max", "vse_get_step", "vse_field_device_ptr",
            "vse_device_bytes", "vse_device_name", "vse_last_error", "vse_destroy", "vse_copy_bandwidth",
            "vse_stream_bandwidth",
-           "vse_batch_create", "vse_batch_run", "vse_batch_profile", "vse_batch_set_rows", "vse_batch_set_group_bytes", "vse_batch_groups",
+           "vse_batch_create", "vse_batch_run", "vse_batch_profile", "vse_batch_set_rows", "vse_batch_set_group_bytes", "vse_batch_set_schedule", "vse_batch_groups",
            "vse_batch_size", "vse_grad_create", "vse_grad_set_tau_scale", "vse_grad_forward", "vse_grad_misfit",
            "vse_grad_backward", "vse_grad_set_flush", "vse_grad_set_overlap", "vse_grad_read", "vse_grad_read_state", "vse_grad_segment", "vse_grad_device_bytes",
            "vse_grad_last_error", "vse_grad_destroy", "vse_batch_last_error", "vse_batch_destroy"]
@@ -124,6 +124,7 @@ def load():
         "vse_batch_profile": (I, [P, I, fp]),
         "vse_batch_set_rows": (I, [P, I]),
         "vse_batch_set_group_bytes": (I, [P, ctypes.c_size_t]),
+        "vse_batch_set_schedule": (I, [P, I]),
         "vse_batch_groups": (I, [P, ip, I]),
         "vse_batch_size": (I, [P]),
         "vse_grad_create": (I, [P, I, ctypes.POINTER(P)]),

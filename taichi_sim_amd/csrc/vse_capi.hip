@@ -1143,12 +1143,14 @@ struct vse_batch {
   int variant = 0, visco = 0, ord = 0;
   int math = VSE_MATH_STRICT;                       // the members' arithmetic mode (they must agree)
   int rows = 1;
+  int schedule = VSE_BATCH_SCHED_GRAPH;
   size_t group_bytes = 0;
   std::vector<BatchGroup> groups;
   BatchGroup all;                                   // every member in one launch (vse_batch_profile)
   StepArgs *d_args = nullptr;
   SparseArgs *d_sparse = nullptr;
   unsigned *d_ctrl = nullptr, *h_ctrl = nullptr;   // (v2max bits, step) per member
+  unsigned *d_bar = nullptr;                        // grid-barrier words of the persistent schedule
   bool orphan = false;                              // a member was destroyed: the batch can only be destroyed
   std::string err;
 
@@ -1353,6 +1355,63 @@ static int run_group(vse_batch *b, BatchGroup &g, int n_steps, int check_every, 
   return rc;
 }
 
+// Steps per cooperative launch of the persistent schedule at most (a launch
+// is one kernel; this bounds how long one runs).
+constexpr int kPersistentMaxSteps = 1024;
+
+// n_steps of group g (one member or many) under the persistent schedule:
+// cooperative launches of up to check_every steps, each running every step
+// of every member of the group with grid-wide barriers between the passes.
+static int run_group_persistent(vse_batch *b, BatchGroup &g, int n_steps, int check_every, float *elapsed_ms) {
+  int grid = 0;
+  int rc = b->hip(vse_batch2d_persistent_grid(b->variant, b->visco, b->ord, b->math, g.B.L, g.total_tiles, &grid),
+                  "persistent launch shape (cooperative launch)");
+  if (rc) return rc;
+  const int chunk = std::min(kPersistentMaxSteps, check_every > 0 ? check_every : kPersistentMaxSteps);
+  unsigned *const fault = b->d_bar + VSE_GRID_BAR_FAULT;
+  unsigned *const h_fault = b->h_ctrl + 2 * b->sims.size();
+  // a barrier that gave up leaves the fault word set: read it back after the launches
+  auto faulted = [&]() -> int {
+    int r = b->hip(hipMemcpyAsync(h_fault, fault, sizeof(unsigned), hipMemcpyDeviceToHost, b->stream), "read fault");
+    if (!r) r = b->hip(hipStreamSynchronize(b->stream), "sync");
+    if (!r && *h_fault) r = b->fail(VSE_ERR_HIP, "persistent launch: a grid barrier wait exceeded its limit");
+    return r;
+  };
+  rc = b->hip(hipMemsetAsync(fault, 0, sizeof(unsigned), b->stream), "clear fault");
+  if (rc) return rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (elapsed_ms) {
+    rc = b->hip(hipEventCreate(&e0), "event");
+    if (!rc) rc = b->hip(hipEventCreate(&e1), "event");
+    if (!rc) rc = b->hip(hipEventRecord(e0, b->stream), "event record");
+  }
+  int done = 0, since_check = 0;
+  while (!rc && done < n_steps) {
+    const int k = std::min(chunk, n_steps - done);
+    rc = b->hip(vse_launch_batch2d_persistent(b->variant, b->visco, b->ord, b->math, g.B, g.total_tiles, grid, k,
+                                              g.rec_after, b->d_bar, b->stream),
+                "persistent launch");
+    done += k;
+    since_check += k;
+    if (!rc && check_every > 0 && since_check >= check_every && done < n_steps) {
+      rc = faulted();
+      if (!rc) rc = check_group(b, g);
+      since_check = 0;
+    }
+  }
+  if (elapsed_ms && !rc) {
+    rc = b->hip(hipEventRecord(e1, b->stream), "event record");
+    if (!rc) rc = b->hip(hipEventSynchronize(e1), "event sync");
+    if (!rc) rc = b->hip(hipEventElapsedTime(elapsed_ms, e0, e1), "elapsed");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (!rc) rc = faulted();
+  if (!rc) rc = check_group(b, g);
+  else (void)hipStreamSynchronize(b->stream);
+  return rc;
+}
+
 extern "C" {
 
 void vse_batch_destroy(vse_batch *b) {
@@ -1365,6 +1424,7 @@ void vse_batch_destroy(vse_batch *b) {
   if (b->d_args) (void)hipFree(b->d_args);
   if (b->d_sparse) (void)hipFree(b->d_sparse);
   if (b->d_ctrl) (void)hipFree(b->d_ctrl);
+  if (b->d_bar) (void)hipFree(b->d_bar);
   if (b->h_ctrl) (void)hipHostFree(b->h_ctrl);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -1397,7 +1457,9 @@ int vse_batch_create(vse_sim *const *sims, int n, vse_batch **out) {
   if (!rc) rc = b->hip(hipMalloc((void **)&b->d_args, n * sizeof(StepArgs)), "hipMalloc");
   if (!rc) rc = b->hip(hipMalloc((void **)&b->d_sparse, n * sizeof(SparseArgs)), "hipMalloc");
   if (!rc) rc = b->hip(hipMalloc((void **)&b->d_ctrl, 2 * n * sizeof(unsigned)), "hipMalloc");
-  if (!rc) rc = b->hip(hipHostMalloc((void **)&b->h_ctrl, 2 * n * sizeof(unsigned), hipHostMallocDefault), "hipHostMalloc");
+  if (!rc) rc = b->hip(hipHostMalloc((void **)&b->h_ctrl, (2 * n + 1) * sizeof(unsigned), hipHostMallocDefault),
+                       "hipHostMalloc");
+  if (!rc) rc = b->hip(hipMalloc((void **)&b->d_bar, VSE_GRID_BAR_WORDS * sizeof(unsigned)), "hipMalloc");
   if (!rc) rc = upload_batch_args(b);   // finalizes the members: their device bytes are complete
   if (!rc) rc = regroup(b);
   if (rc) {
@@ -1418,7 +1480,9 @@ int vse_batch_run(vse_batch *b, int n_steps, int check_every, float *elapsed_ms)
   for (size_t k = 0; k < b->groups.size() && !rc && n_steps > 0; k++) {
     BatchGroup &g = b->groups[k];
     float ms = 0.f;
-    if (g.n == 1) {
+    if (b->schedule == VSE_BATCH_SCHED_PERSISTENT) {
+      rc = run_group_persistent(b, g, n_steps, check_every, elapsed_ms ? &ms : nullptr);
+    } else if (g.n == 1) {
       vse_sim *s = b->sims[g.first];
       rc = vse_run(s, n_steps, check_every, elapsed_ms ? &ms : nullptr);
       if (rc) {
@@ -1486,6 +1550,20 @@ int vse_batch_set_group_bytes(vse_batch *b, size_t bytes) {
   if (int rc = b->usable()) return rc;
   b->group_bytes = bytes;
   return regroup(b);
+}
+
+int vse_batch_set_schedule(vse_batch *b, int schedule) {
+  if (!b || (schedule != VSE_BATCH_SCHED_GRAPH && schedule != VSE_BATCH_SCHED_PERSISTENT)) return VSE_ERR_ARG;
+  OnDevice on_dev(b->device);
+  if (int rc = b->usable()) return rc;
+  if (schedule == VSE_BATCH_SCHED_PERSISTENT) {
+    int coop = 0;
+    int rc = b->hip(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, b->device), "device attribute");
+    if (rc) return rc;
+    if (!coop) return b->fail(VSE_ERR_HIP, "the device cannot launch cooperative kernels");
+  }
+  b->schedule = schedule;
+  return VSE_OK;
 }
 
 int vse_batch_groups(vse_batch *b, int *first, int max_groups) {

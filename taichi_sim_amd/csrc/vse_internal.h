@@ -255,6 +255,27 @@ hipError_t vse_launch_sparse(const SparseArgs &S, hipStream_t st);
 hipError_t vse_launch_batch2d(bool stress, int variant, int visco, int ord, int math, const BatchArgs &B,
                               int total_tiles, int i, hipStream_t st);
 hipError_t vse_launch_advance_batch(const BatchArgs &B, int n, hipStream_t st);
+// persistent schedule: nsteps steps of every member in one cooperative launch
+// of `grid` workgroups striding over the total_tiles tiles (grid-wide
+// barriers between the passes), then the counters advance by nsteps;
+// vse_batch2d_persistent_grid gives the largest grid that can be resident
+// (at most total_tiles, a multiple of 8 from 8 on) on the current device
+// `bar` holds the launch's grid-barrier words (VSE_GRID_BAR_WORDS; the
+// launcher clears the counters before the kernel).  Word VSE_GRID_BAR_FAULT
+// is left to the caller: a launch sets it when a barrier wait gave up (its
+// results are then invalid), so the caller clears it and reads it back.
+#define VSE_GRID_BAR_SUB 64
+#define VSE_GRID_BAR_STRIDE 64
+#define VSE_GRID_BAR_FAULT ((VSE_GRID_BAR_SUB + 2) * VSE_GRID_BAR_STRIDE)
+#define VSE_GRID_BAR_WORDS ((VSE_GRID_BAR_SUB + 3) * VSE_GRID_BAR_STRIDE)
+#define VSE_GRID_BAR_SPIN (1u << 18)
+// barriers per launch stay below 2^32 / (workgroups per first-level counter)
+#define VSE_GRID_BAR_MAX_STEPS 4096
+hipError_t vse_batch2d_persistent_grid(int variant, int visco, int ord, int math, int L, int total_tiles, int *grid);
+hipError_t vse_launch_batch2d_persistent(int variant, int visco, int ord, int math, const BatchArgs &B,
+                                         int total_tiles, int grid, int nsteps, int rec_after, unsigned *bar,
+                                         hipStream_t st);
+
 hipError_t vse_launch_batch_ctrl(const BatchArgs &B, unsigned *out, hipStream_t st);
 hipError_t vse_launch_fold_v2(const unsigned *slots, int n, unsigned *v2max, hipStream_t st);
 long vse_tiles2d(int nx, int ny, int rows);

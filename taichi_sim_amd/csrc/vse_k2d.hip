@@ -6,6 +6,8 @@
 // x 4 (x axis): every wavefront reads 256 contiguous bytes per field, and the
 // +-ORD stencil neighbours come from L1/L2, so each field streams from HBM
 // about once per pass.
+#include <algorithm>
+
 #include "vse_device.h"
 
 namespace {
@@ -502,9 +504,10 @@ __device__ __forceinline__ int batch_instance(const BatchArgs &B, int b) {
   return lo;
 }
 
-template <int VAR, bool VISCO, int ORD, int LC = 0, bool FAST = false>
-__global__ __launch_bounds__(BX * BY) void k_stress2d_batch(BatchArgs B) {
-  const int b = xcd_tile(blockIdx.x, gridDim.x, 1);     // XCD-contiguous tile order, as tile2d
+// The stress update of batch tile b (of the launch's tiles in dispatch order
+// b, mapped to an XCD-contiguous tile order as tile2d does).
+template <int VAR, bool VISCO, int ORD, int LC, bool FAST>
+__device__ __forceinline__ void batch_stress_tile(const BatchArgs &B, int b) {
   const int i = batch_instance(B, b);
   const StepArgs &A = B.args[i];
   const int t = b - B.tile_beg[i];
@@ -514,11 +517,10 @@ __global__ __launch_bounds__(BX * BY) void k_stress2d_batch(BatchArgs B) {
   for (int k = 0; k < B.rows; k++) stress2d_cell<VAR, VISCO, ORD, LC, FAST>(A, x0 + k * BX, y);
 }
 
-// One launch over the tiles of every member; each member's step is
-// *step + i and its first tile records its receivers.
-template <int VAR, bool VISCO, int ORD, bool FAST = false>
-__global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B, int i) {
-  const int b = xcd_tile(blockIdx.x, gridDim.x, 1);
+// The velocity update of batch tile b at each member's step *step + i; a
+// member's first tile records its receivers (stress channels).
+template <int VAR, bool VISCO, int ORD, bool FAST>
+__device__ __forceinline__ void batch_velocity_tile(const BatchArgs &B, int b, int i) {
   const int m = batch_instance(B, b);
   const StepArgs &A = B.args[m];
   const int t = b - B.tile_beg[m];
@@ -533,6 +535,100 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B, int i
   }
   if (t == 0 && !A.rec_after) record_receivers(A, s);
   wave_max_to_slot(v2, A.v2slots + t);
+}
+
+template <int VAR, bool VISCO, int ORD, int LC = 0, bool FAST = false>
+__global__ __launch_bounds__(BX * BY) void k_stress2d_batch(BatchArgs B) {
+  batch_stress_tile<VAR, VISCO, ORD, LC, FAST>(B, xcd_tile(blockIdx.x, gridDim.x, 1));
+}
+
+// One launch over the tiles of every member; each member's step is
+// *step + i and its first tile records its receivers.
+template <int VAR, bool VISCO, int ORD, bool FAST = false>
+__global__ __launch_bounds__(BX * BY) void k_velocity2d_batch(BatchArgs B, int i) {
+  batch_velocity_tile<VAR, VISCO, ORD, FAST>(B, xcd_tile(blockIdx.x, gridDim.x, 1), i);
+}
+
+// ---- persistent batch: nsteps steps of a group in one cooperative launch ----
+// Grid-wide barrier of a persistent launch: a two-level arrival tree over
+// device-scope atomics in `bar` (VSE_GRID_BAR_WORDS words, zero at the launch),
+// counters VSE_GRID_BAR_STRIDE words apart so no two share a cache line.
+// Workgroup b arrives on first-level counter b % ns (ns = min(grid, 64)); the
+// last arrival of each counter for barrier k (counter == k * its members)
+// arrives on the top counter, and the last of those publishes epoch k.  The
+// counters only grow, so a barrier needs no reset.  Every wavefront first
+// waits until its own stores are acknowledged; then thread 0 alone fences at
+// device scope (the L2s of the XCDs are not coherent with each other): a
+// release before its arrival (its XCD's L2 written back), a full fence
+// between the levels, an acquire after the wait (stale lines invalidated).
+// The wait is bounded: past VSE_GRID_BAR_SPIN polls the workgroup sets the
+// fault word and goes on, so a launch always ends, and the host reports the
+// fault (the results of such a launch are not valid).
+__device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned k) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned g = gridDim.x, ns = g < VSE_GRID_BAR_SUB ? g : VSE_GRID_BAR_SUB, sub = blockIdx.x % ns;
+    const unsigned members = g / ns + (sub < g % ns ? 1u : 0u);
+    unsigned *const top = bar + VSE_GRID_BAR_SUB * VSE_GRID_BAR_STRIDE;
+    unsigned *const epoch = top + VSE_GRID_BAR_STRIDE;
+    unsigned *const fault = epoch + VSE_GRID_BAR_STRIDE;
+    const unsigned a = atomicAdd(bar + sub * VSE_GRID_BAR_STRIDE, 1u) + 1u;
+    if (a == k * members) {
+      __threadfence();
+      const unsigned t = atomicAdd(top, 1u) + 1u;
+      if (t == k * ns) {
+        __threadfence();
+        atomicMax(epoch, k);
+      }
+    }
+    unsigned spins = 0;
+    while (__hip_atomic_load(epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k) {
+      __builtin_amdgcn_s_sleep(2);
+      ++spins;
+      // give up past the spin limit, or at once after another workgroup did
+      if (spins == VSE_GRID_BAR_SPIN ||
+          ((spins & 255u) == 0 && __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        atomicMax(fault, 1u);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// The graph schedule launches one stress and one velocity kernel per step.
+// Here one launch runs nsteps steps of every member: its workgroups (all
+// resident at once -- the cooperative launch guarantees that or refuses to
+// start) stride over the same `total` tiles in the same order, b = vb mapped
+// as xcd_tile(vb, total, 1), and a grid-wide barrier stands where each kernel
+// boundary was.  The cell updates are the same code, so the results are the
+// same bits.  Every workgroup runs every barrier (the loop bounds are
+// uniform over the grid).  Velocity-channel receivers are gathered after the
+// barrier that ends each velocity pass; the next stress pass does not write
+// them, and the barrier after it orders the gather before the velocity pass
+// that does.  The counters move by nsteps after a last barrier, once no
+// workgroup reads them any more.
+template <int VAR, bool VISCO, int ORD, int LC, bool FAST>
+__global__ __launch_bounds__(BX * BY) void k_batch2d_persistent(BatchArgs B, int total, int nsteps, int rec_after,
+                                                                unsigned *bar) {
+  unsigned k = 0;
+  for (int i = 0; i < nsteps; i++) {
+    for (int vb = blockIdx.x; vb < total; vb += gridDim.x)
+      batch_stress_tile<VAR, VISCO, ORD, LC, FAST>(B, xcd_tile(vb, total, 1));
+    grid_barrier(bar, ++k);
+    for (int vb = blockIdx.x; vb < total; vb += gridDim.x)
+      batch_velocity_tile<VAR, VISCO, ORD, FAST>(B, xcd_tile(vb, total, 1), i);
+    grid_barrier(bar, ++k);
+    if (rec_after)
+      for (int m = blockIdx.x; m < B.n; m += gridDim.x)
+        if (B.args[m].rec_after) record_receivers(B.args[m], *B.args[m].step + i);
+  }
+  grid_barrier(bar, ++k);
+  if (blockIdx.x == 0)
+    for (int m = threadIdx.x + threadIdx.y * BY; m < B.n; m += BX * BY) *B.sparse[m].step += nsteps;
 }
 
 template <int VAR, bool VISCO, bool FAST>
@@ -577,6 +673,30 @@ hipError_t launch_batch_ord(bool stress, int ord, const BatchArgs &B, int total_
   return hipGetLastError();
 }
 
+// The persistent kernel of one instantiation (for the launch and the
+// occupancy query).
+template <int VAR, bool VISCO, bool FAST>
+const void *persistent_fn(int ord, bool l3) {
+#define VSE_CASE(O)                                                               \
+  case O:                                                                         \
+    return l3 ? (const void *)&k_batch2d_persistent<VAR, VISCO, O, 3, FAST>        \
+              : (const void *)&k_batch2d_persistent<VAR, VISCO, O, 0, FAST>;
+  switch (ord) {
+    VSE_CASE(2) VSE_CASE(3) VSE_CASE(4) VSE_CASE(5) VSE_CASE(6)
+    default: return nullptr;
+  }
+#undef VSE_CASE
+}
+
+const void *persistent_kernel(int variant, int visco, int ord, int math, int L) {
+  const bool fast = math == VSE_MATH_FAST;
+  if (variant == VSE_VAR_TAICHI) return persistent_fn<VSE_VAR_TAICHI, false, false>(ord, false);
+  if (visco) return fast ? persistent_fn<VSE_VAR_WGSL, true, true>(ord, L == 3)
+                         : persistent_fn<VSE_VAR_WGSL, true, false>(ord, L == 3);
+  return fast ? persistent_fn<VSE_VAR_WGSL, false, true>(ord, false)
+              : persistent_fn<VSE_VAR_WGSL, false, false>(ord, false);
+}
+
 // FAST applies to the WGSL variant (the Taichi variant already multiplies by f32(1/d)).
 hipError_t dispatch(bool stress, int variant, int visco, int ord, int math, const StepArgs &A, const KTune &t, int i,
                     hipStream_t st) {
@@ -599,6 +719,39 @@ hipError_t vse_launch_batch2d(bool stress, int variant, int visco, int ord, int 
                          : launch_batch_ord<VSE_VAR_WGSL, true, false>(stress, ord, B, total_tiles, i, st);
   return fast ? launch_batch_ord<VSE_VAR_WGSL, false, true>(stress, ord, B, total_tiles, i, st)
               : launch_batch_ord<VSE_VAR_WGSL, false, false>(stress, ord, B, total_tiles, i, st);
+}
+
+hipError_t vse_batch2d_persistent_grid(int variant, int visco, int ord, int math, int L, int total_tiles, int *grid) {
+  *grid = 0;
+  const void *fn = persistent_kernel(variant, visco, ord, math, L);
+  if (!fn || total_tiles < 1) return hipErrorInvalidValue;
+  int dev = 0, ncu = 0, coop = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
+  if (e == hipSuccess && !coop) e = hipErrorNotSupported;
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BX * BY, 0);
+  if (e != hipSuccess) return e;
+  long g = std::min<long>((long)per_cu * ncu, total_tiles);
+  if (g >= 8) g &= ~7L;
+  if (g < 1) return hipErrorInvalidConfiguration;
+  *grid = (int)g;
+  return hipSuccess;
+}
+
+hipError_t vse_launch_batch2d_persistent(int variant, int visco, int ord, int math, const BatchArgs &B,
+                                         int total_tiles, int grid, int nsteps, int rec_after, unsigned *bar,
+                                         hipStream_t st) {
+  const void *fn = persistent_kernel(variant, visco, ord, math, B.L);
+  if (!fn || !bar || grid < 1 || grid > total_tiles || nsteps < 1 || nsteps > VSE_GRID_BAR_MAX_STEPS)
+    return hipErrorInvalidValue;
+  // the counters start at zero for every launch (the fault word is the caller's)
+  hipError_t e = hipMemsetAsync(bar, 0, VSE_GRID_BAR_FAULT * sizeof(unsigned), st);
+  if (e != hipSuccess) return e;
+  BatchArgs b = B;
+  int total = total_tiles, n = nsteps, ra = rec_after ? 1 : 0;
+  void *args[] = {(void *)&b, (void *)&total, (void *)&n, (void *)&ra, (void *)&bar};
+  return hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3(BY, BX), args, 0, st);
 }
 
 hipError_t vse_launch_advance_batch(const BatchArgs &B, int n, hipStream_t st) {

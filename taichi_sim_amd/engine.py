@@ -360,6 +360,7 @@ class VSEBatch:
             raise _lib.VSEError(rc, "vse_batch_create failed: members must be open 2D simulations on one device "
                                     "with the same variant, viscoelastic flag, FD order and mechanism count")
         self._h = h
+        self.schedule = "graph"
         for s in self.sims:
             s._dependents.add(self)
 
@@ -390,6 +391,17 @@ class VSEBatch:
         rc = self._lib.vse_batch_set_group_bytes(self._open(), int(nbytes))
         if rc != 0:
             raise _lib.VSEError(rc, self._lib.vse_batch_last_error(self._h).decode(errors="replace"))
+
+    SCHEDULES = {"graph": 0, "persistent": 1}
+
+    def set_schedule(self, schedule):
+        """'graph' (default): hipGraphs of one stress and one velocity launch per
+        step for each group.  'persistent': each group's steps as cooperative
+        launches with grid-wide barriers between the passes.  Same results."""
+        rc = self._lib.vse_batch_set_schedule(self._open(), self.SCHEDULES[schedule])
+        if rc != 0:
+            raise _lib.VSEError(rc, self._lib.vse_batch_last_error(self._h).decode(errors="replace"))
+        self.schedule = schedule
 
     def groups(self):
         """First member index of each group."""

@@ -18,7 +18,7 @@ from taichi_sim_amd.sim_support.simulator import Simulator
 pytestmark = pytest.mark.gpu
 
 
-def _shots(sizes, n_shots, steps, variant="wgsl"):
+def _shots(sizes, n_shots, steps, variant="wgsl", channel=None):
     """One VSESim per (specimen, focal law), plus the Simulator and law of each."""
     out = []
     for cfg, laws in workloads.multi_shot_2d(sizes=sizes, n_shots=n_shots, steps=steps, n_elem=8):
@@ -27,7 +27,7 @@ def _shots(sizes, n_shots, steps, variant="wgsl"):
         att = attenuation_from_config(cfg)
         for law in range(n_shots):
             sim.select_law(law)
-            out.append((sim, law, att, VSESim.from_simulator(sim, variant=variant, att=att)))
+            out.append((sim, law, att, VSESim.from_simulator(sim, variant=variant, att=att, channel=channel)))
     return out
 
 
@@ -84,6 +84,39 @@ def test_batch_uneven_specimens_match_solo_runs(variant, grouping):
         m[3].close()
 
 
+@pytest.mark.parametrize("variant,grouping,channel",
+                         [("wgsl", "mixed", None), ("wgsl", "lockstep", "vy"), ("taichi", "default", None)])
+def test_batch_persistent_schedule_bit_identical(variant, grouping, channel):
+    """The persistent schedule (each group's steps in cooperative launches,
+    grid-wide barriers between the passes; groups of one member included)
+    leaves every member exactly where the graph schedule does: fields,
+    memory variables, traces (sigma or a velocity channel), |v|^2 max and the
+    step counter, over launches of 64, 36 and 140 steps."""
+    steps = 240
+    print(f"persistent batch {variant} {grouping} {channel}", flush=True)
+    runs = []
+    for schedule in ("graph", "persistent"):
+        members = _shots((96, 160, 131), 2, steps, variant, channel)
+        batch = VSEBatch([m[3] for m in members])
+        if grouping == "lockstep":
+            batch.set_group_bytes(0)
+        elif grouping == "mixed":
+            sizes = [m[3].device_bytes() for m in members]
+            batch.set_group_bytes(sizes[0] + sizes[1])
+            assert batch.groups() == [0, 2, 3, 4, 5]
+        batch.set_schedule(schedule)
+        batch.run(100, check_every=64)
+        batch.run(steps - 100, check_every=0)
+        batch.close()
+        assert [m[3].step() for m in members] == [steps] * len(members)
+        runs.append([_state(m[3]) for m in members])
+        for m in members:
+            m[3].close()
+    assert any(np.abs(f["traces"]).max() > 0 for f in runs[0])
+    for i, (a, b) in enumerate(zip(*runs)):
+        _assert_same(b, a, ("member", i))
+
+
 def test_batch_resumes_across_calls_and_reset():
     steps = 150
     members = _shots((112,), 3, steps)
@@ -110,7 +143,9 @@ def test_batch_resumes_across_calls_and_reset():
         g.close()
 
 
-def test_batch_instability_names_member():
+@pytest.mark.parametrize("schedule", ["graph", "persistent"])
+def test_batch_instability_names_member(schedule):
+    print(f"batch instability {schedule}", flush=True)
     members = _shots((96,), 2, 80)
     gs = [m[3] for m in members]
     sim, law = members[1][0], members[1][1]
@@ -119,6 +154,7 @@ def test_batch_instability_names_member():
     gs[1].load_sources(sim)
     gs[1].reset()
     batch = VSEBatch(gs)
+    batch.set_schedule(schedule)
     with pytest.raises(StabilityError) as e:
         batch.run(80, check_every=20)
     assert "member 1" in str(e.value)

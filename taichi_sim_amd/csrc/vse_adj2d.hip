@@ -200,7 +200,7 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
   constexpr int last = ORD - 1, off = ORD - 1;
   constexpr int LR = LC > 0 ? LC : 1;
   const long id = (long)x * ny + y;
-  const long nc = A.ncell;
+  const long ms = A.rs;          // mechanism stride of the memory-variable planes
   const float dt = A.dt;
   const DLevel &I = A.in, &O = A.out;
   const int L = A.L;
@@ -233,8 +233,8 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
     if constexpr (LC > 0) {
 #pragma unroll
       for (int l = 0; l < LC; l++) {
-        oxx[l] = I.r[0][l * nc + id];
-        oyy[l] = I.r[1][l * nc + id];
+        oxx[l] = I.r[0][l * ms + id];
+        oyy[l] = I.r[1][l * ms + id];
       }
     }
   }
@@ -247,7 +247,7 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
     csS = A.cs[id + 1];
     if constexpr (LC > 0) {
 #pragma unroll
-      for (int l = 0; l < LC; l++) oxy[l] = I.r[2][l * nc + id];
+      for (int l = 0; l < LC; l++) oxy[l] = I.r[2][l * ms + id];
     }
   }
   const ProfAt fx = prof_at(A.ax[0], jx, ex >= 0), fy = prof_at(A.ax[1], jy, ey >= 0);
@@ -276,20 +276,20 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
 #pragma unroll
     for (int l = 0; l < (LC > 0 ? LC : L); l++) {
       const ZCell z = zener_cell(A, l, rs);
-      const float ox = LC > 0 ? oxx[l] : I.r[0][l * nc + id], oy = LC > 0 ? oyy[l] : I.r[1][l * nc + id];
+      const float ox = LC > 0 ? oxx[l] : I.r[0][l * ms + id], oy = LC > 0 ? oyy[l] : I.r[1][l * ms + id];
       const float nxx = ((ox + (dvx + dvy) * z.phi_p - ox * z.h_p) * z.m_p);
       const float nyy = ((oy + 0.5f * (dvx - dvy) * z.phi_s - oy * z.h_s) * z.m_s);
       sum_xx += ox + nxx;
       sum_yy += oy + nyy;
-      O.r[0][l * nc + id] = nxx;
-      O.r[1][l * nc + id] = nyy;
+      O.r[0][l * ms + id] = nxx;
+      O.r[1][l * ms + id] = nyy;
     }
     sxx = sxx + (lp2m * dvx + lambda * dvy + 0.5f * lpm * sum_xx + mu * sum_yy) * dt;
     syy = syy + (lambda * dvx + lp2m * dvy + 0.5f * lpm * sum_xx - mu * sum_yy) * dt;
   } else {
     for (int l = 0; l < L; l++) {
-      O.r[0][l * nc + id] = I.r[0][l * nc + id];
-      O.r[1][l * nc + id] = I.r[1][l * nc + id];
+      O.r[0][l * ms + id] = I.r[0][l * ms + id];
+      O.r[1][l * ms + id] = I.r[1][l * ms + id];
     }
     if (ex >= 0) O.psi[SL_VXX][px] = I.psi[SL_VXX][px];
     if (ey >= 0) O.psi[SL_VYY][py] = I.psi[SL_VYY][py];
@@ -314,14 +314,14 @@ __device__ __forceinline__ void dstress_cell(const DiffArgs &A, int x, int y) {
 #pragma unroll
     for (int l = 0; l < (LC > 0 ? LC : L); l++) {
       const ZCell z = zener_cell(A, l, rs);
-      const float o = LC > 0 ? oxy[l] : I.r[2][l * nc + id];
+      const float o = LC > 0 ? oxy[l] : I.r[2][l * ms + id];
       const float n = ((o + (dvydx + dvxdy) * z.phi_s - o * z.h_s) * z.m_s);
       sum_xy += o + n;
-      O.r[2][l * nc + id] = n;
+      O.r[2][l * ms + id] = n;
     }
     sxy = sxy + (mu * (dvydx + dvxdy) + 0.5f * mu * sum_xy) * dt;
   } else {
-    for (int l = 0; l < L; l++) O.r[2][l * nc + id] = I.r[2][l * nc + id];
+    for (int l = 0; l < L; l++) O.r[2][l * ms + id] = I.r[2][l * ms + id];
     if (ex >= 0) O.psi[SL_VYX][px] = I.psi[SL_VYX][px];
     if (ey >= 0) O.psi[SL_VXY][py] = I.psi[SL_VXY][py];
   }
@@ -599,7 +599,7 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
   constexpr int last = ORD - 1, off = ORD - 1;
   constexpr int LR = LC > 0 ? LC : 1;
   const long id = (long)x * ny + y;
-  const long nc = A.ncell;
+  const long ms = A.rs;          // mechanism stride of the memory-variable planes
   const float dt = A.dt;
   const int L = A.L;
   const DLevel &I = A.in, &B = A.bar;
@@ -646,10 +646,10 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
     if constexpr (LC > 0) {
 #pragma unroll
       for (int l = 0; l < LC; l++) {
-        pxx[l] = I.r[0][l * nc + id];
-        pyy[l] = I.r[1][l * nc + id];
-        bxx[l] = B.r[0][l * nc + id];
-        byy[l] = B.r[1][l * nc + id];
+        pxx[l] = I.r[0][l * ms + id];
+        pyy[l] = I.r[1][l * ms + id];
+        bxx[l] = B.r[0][l * ms + id];
+        byy[l] = B.r[1][l * ms + id];
       }
     }
   }
@@ -670,8 +670,8 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
     if constexpr (LC > 0) {
 #pragma unroll
       for (int l = 0; l < LC; l++) {
-        po[l] = I.r[2][l * nc + id];
-        pb[l] = B.r[2][l * nc + id];
+        po[l] = I.r[2][l * ms + id];
+        pb[l] = B.r[2][l * ms + id];
       }
     }
   }
@@ -704,15 +704,15 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
 #pragma unroll
     for (int l = 0; l < (LC > 0 ? LC : L); l++) {
       const ZCell z = zener_cell(A, l, rs);
-      const float oxx = LC > 0 ? pxx[l] : I.r[0][l * nc + id], oyy = LC > 0 ? pyy[l] : I.r[1][l * nc + id];
+      const float oxx = LC > 0 ? pxx[l] : I.r[0][l * ms + id], oyy = LC > 0 ? pyy[l] : I.r[1][l * ms + id];
       const float nxx = ((oxx + Xp * z.phi_p - oxx * z.h_p) * z.m_p);
       const float nyy = ((oyy + 0.5f * (e1 - e2) * z.phi_s - oyy * z.h_s) * z.m_s);
       sum_xx += oxx + nxx;
       sum_yy += oyy + nyy;
-      const float Rxx = (LC > 0 ? bxx[l] : B.r[0][l * nc + id]) + Sxxb;      // adjoint of r'_xx
-      const float Ryy = (LC > 0 ? byy[l] : B.r[1][l * nc + id]) + Syyb;
-      B.r[0][l * nc + id] = Sxxb + Rxx * ((1.0f - z.h_p) * z.m_p);
-      B.r[1][l * nc + id] = Syyb + Ryy * ((1.0f - z.h_s) * z.m_s);
+      const float Rxx = (LC > 0 ? bxx[l] : B.r[0][l * ms + id]) + Sxxb;      // adjoint of r'_xx
+      const float Ryy = (LC > 0 ? byy[l] : B.r[1][l * ms + id]) + Syyb;
+      B.r[0][l * ms + id] = Sxxb + Rxx * ((1.0f - z.h_p) * z.m_p);
+      B.r[1][l * ms + id] = Syyb + Ryy * ((1.0f - z.h_s) * z.m_s);
       e1b += Rxx * z.phi_p * z.m_p + 0.5f * Ryy * z.phi_s * z.m_s;
       e2b += Rxx * z.phi_p * z.m_p - 0.5f * Ryy * z.phi_s * z.m_s;
       // d r'/d s = (m/s)(r h - X phi + r' h)
@@ -739,11 +739,11 @@ __global__ __launch_bounds__(BX * BY) void k_adj_stress(DiffArgs A) {
 #pragma unroll
     for (int l = 0; l < (LC > 0 ? LC : L); l++) {
       const ZCell z = zener_cell(A, l, rs);
-      const float o = LC > 0 ? po[l] : I.r[2][l * nc + id];
+      const float o = LC > 0 ? po[l] : I.r[2][l * ms + id];
       const float n = ((o + X * z.phi_s - o * z.h_s) * z.m_s);
       sum_xy += o + n;
-      const float Rl = (LC > 0 ? pb[l] : B.r[2][l * nc + id]) + Sb;
-      B.r[2][l * nc + id] = Sb + Rl * ((1.0f - z.h_s) * z.m_s);
+      const float Rl = (LC > 0 ? pb[l] : B.r[2][l * ms + id]) + Sb;
+      B.r[2][l * ms + id] = Sb + Rl * ((1.0f - z.h_s) * z.m_s);
       eb += Rl * z.phi_s * z.m_s;
       gs += Rl * (z.m_s * rs) * (o * z.h_s - X * z.phi_s + n * z.h_s);
     }

@@ -1612,6 +1612,7 @@ struct vse_grad {
   int *src_map = nullptr;
   double *misfit = nullptr;
   unsigned *ctrl = nullptr, *host_ctrl = nullptr;
+  long rs = 0;                    // mechanism stride of the memory-variable planes (floats)
   std::vector<void *> allocs;
   size_t bytes = 0;
   std::string err;
@@ -1622,23 +1623,35 @@ struct vse_grad {
     err = std::string(what) + ": " + hipGetErrorString(e);
     return VSE_ERR_HIP;
   }
+  // Zeroed device array of n elements, its base skewed by 256 B modulo 4 KiB
+  // from the previous array of the process, as the simulation's arrays are:
+  // the differentiable kernels stream 20-50 arrays at the same cell index,
+  // and with every base on one 4 KiB alignment those streams meet in the same
+  // HBM channels.
   template <typename T>
   int alloc(T **p, size_t n) {
+    constexpr size_t kSkewStep = 256, kSkewSpan = 16;
     const size_t nb = std::max<size_t>(n, 1) * sizeof(T);
+    const size_t skew = kSkewStep * (vse_sim::g_allocs.fetch_add(1, std::memory_order_relaxed) % kSkewSpan);
     void *q = nullptr;
-    int rc = hip(hipMalloc(&q, nb), "hipMalloc");
+    int rc = hip(hipMalloc(&q, nb + kSkewStep * kSkewSpan), "hipMalloc");
     if (rc) return rc;
     allocs.push_back(q);
     bytes += nb;
-    *p = (T *)q;
-    return hip(hipMemsetAsync(q, 0, nb, s->stream), "hipMemsetAsync");
+    *p = (T *)((char *)q + skew);
+    return hip(hipMemsetAsync(*p, 0, nb, s->stream), "hipMemsetAsync");
   }
   long psi_elems(int i) const {   // local slot i of kPsiSlots2D: even = x axis, odd = y axis
     return (i % 2 == 0) ? (long)s->ax[0].nb * s->d.ny : (long)s->d.nx * s->ax[1].nb;
   }
+  // A level is one allocation.  Its arrays follow each other with a 256 B gap,
+  // and the mechanism planes of a memory variable are rs floats apart (1.5 KiB
+  // more than a plane), so no two streams of a cell start on the same 4 KiB
+  // alignment (the simulation's own layout does the same).
+  static constexpr long kGap = 64;
   long level_floats() const {
-    long total = (5 + 3L * s->d.n_sls) * s->ncell;
-    for (int i = 0; i < 8; i++) total += psi_elems(i);
+    long total = 5 * (s->ncell + kGap) + 3 * ((long)s->d.n_sls * rs + kGap);
+    for (int i = 0; i < 8; i++) total += psi_elems(i) + kGap;
     return total;
   }
   int alloc_level(DLevel &v) {
@@ -1649,9 +1662,9 @@ struct vse_grad {
     if (rc) return rc;
     float *p = base;
     float **f[5] = {&v.vx, &v.vy, &v.sxx, &v.syy, &v.sxy};
-    for (int i = 0; i < 5; i++) { *f[i] = p; p += nc; }
-    for (int c = 0; c < 3; c++) { v.r[c] = p; p += (long)L * nc; }
-    for (int i = 0; i < 8; i++) { v.psi[i] = p; p += psi_elems(i); }
+    for (int i = 0; i < 5; i++) { *f[i] = p; p += nc + kGap; }
+    for (int c = 0; c < 3; c++) { v.r[c] = p; p += (long)L * rs + kGap; }
+    for (int i = 0; i < 8; i++) { v.psi[i] = p; p += psi_elems(i) + kGap; }
     return 0;
   }
   int zero_level(const DLevel &v) {
@@ -1668,7 +1681,7 @@ static void make_diff(vse_grad *g, DiffArgs &A) {
   vse_sim *s = g->s;
   const vse_desc &d = s->d;
   memset(&A, 0, sizeof(A));
-  A.nx = d.nx; A.ny = d.ny; A.ord = d.ord; A.ncell = s->ncell;
+  A.nx = d.nx; A.ny = d.ny; A.ord = d.ord; A.ncell = s->ncell; A.rs = g->rs;
   A.dx = d.dx; A.dy = d.dy; A.dt = d.dt;
   for (int c = 0; c < d.ord; c++) {
     A.C[c] = d.coefs[c];
@@ -1779,6 +1792,7 @@ int vse_grad_create(vse_sim *s, int seg_len, vse_grad **out) {
   if (rc) return rc;
   vse_grad *g = new vse_grad();
   g->s = s;
+  g->rs = s->ncell + 384;
   s->grads.push_back(g);      // vse_grad_destroy (also on failure below) unregisters
   const int N = s->d.n_rows;
   g->K = seg_len ? seg_len : std::max(2, (int)std::ceil(std::sqrt((double)N)));

@@ -171,7 +171,7 @@ struct BatchArgs {
 // kPsiSlots2D[i] in the compact band layout of the forward kernels.
 struct DLevel {
   VSE_G float *vx, *vy, *sxx, *syy, *sxy;
-  VSE_G float *r[3];                 // xx, yy, xy: L planes of ncell each
+  VSE_G float *r[3];                 // xx, yy, xy: L planes each, plane l at r[c] + l * DiffArgs::rs
   VSE_G float *psi[8];
 };
 
@@ -187,6 +187,7 @@ constexpr int VSE_GRAD_FLUSH_MAX = 4096;
 struct DiffArgs {
   int nx, ny, ord;
   long ncell;
+  long rs;                           // mechanism stride of r in every level and in bar (rs >= ncell)
   float dx, dy, dt;
   float C[VSE_MAX_ORD];
   float Cd[2][VSE_MAX_ORD];          // f32(C[c] / d_axis): the adjoint kernels' differences

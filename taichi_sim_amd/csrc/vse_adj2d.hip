@@ -456,7 +456,7 @@ __global__ __launch_bounds__(BX * BY) void k_dvelocity(DiffArgs A) {
   tile_cell(A, x, y);
   const unsigned v2 = dvelocity_cell<ORD>(A, x, y);
   if (A.record && blockIdx.x == 0) record_traces(A);
-  block_max_u32(v2, A.v2max);
+  wave_max_running(v2, A.v2max);
 }
 
 // ---------------------------------------------------------------- adjoint

@@ -97,6 +97,20 @@ __device__ __forceinline__ void wave_max_to_slot(unsigned v, unsigned *slot) {
   if (((threadIdx.x + threadIdx.y * blockDim.x) & 63) == 0 && v) atomicMax(slot, v);
 }
 
+// Max over a wavefront, then lane 0 reads the running max at dst and issues
+// an atomicMax only when its value would raise it: no LDS and no barrier, and
+// once the running max is up to date almost no wavefront touches the shared
+// word.  Same result as block_max_u32 (the max does not depend on order).
+__device__ __forceinline__ void wave_max_running(unsigned v, unsigned *dst) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned o = __shfl_xor(v, off, 64);
+    v = v > o ? v : o;
+  }
+  if (((threadIdx.x + threadIdx.y * blockDim.x) & 63) == 0 && v &&
+      v > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(dst, v);
+}
+
 // Max over a 64-wide wavefront, then at most one atomicMax per block.  For |v|^2 >= 0
 // the IEEE bit pattern orders like the value; NaN sorts above +inf, so the
 // running max also flags NaN (the reference's `vsn2 > 1e38` test does not).

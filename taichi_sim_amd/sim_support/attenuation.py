@@ -57,23 +57,42 @@ def _f32_sum(a):
 
 
 def fit_constant_q(q_p, q_s, f_min, f_max, n_sls=3, n_freq=64):
-    """Relaxation times for an approximately frequency-independent Q.
+    """Relaxation times that give an approximately frequency-independent
+    quality factor under the reference's Zener update.
 
-    Standard least-squares fit (e.g. Blanch, Robertsson & Symes 1995): the
-    relaxation frequencies are spread logarithmically over [f_min, f_max]
-    (units: MHz, so tau comes out in microseconds) and the strengths y_l
-    minimise sum_f (Q^-1(f) - sum_l y_l w tau_l / (1 + w^2 tau_l^2))^2 with
-    Q^-1 ~ sum_l y_l w tau_l / (1 + w^2 tau_l^2).  Returns an
-    ``AttenuationCoefficients`` whose tau_epsilon = tau_sigma (1 + y_l).
+    The relaxation frequencies are spread logarithmically over [f_min, f_max]
+    (MHz, so tau comes out in microseconds).  The reference's memory-variable
+    update (wgsl2d :879-913, wgsl3d :1405-1445) is the Crank-Nicolson step of
+    dr/dt = -r / tau_sigma + X (1 - alpha) / (tau_sigma sum_alpha), with
+    alpha = tau_epsilon / tau_sigma, and the stress takes the time average of r
+    (S = r + r_new, times one half).  A modulus M it relaxes (M from cp, cs, the
+    unrelaxed value) therefore behaves as
+
+        M(w) = M (1 - sum_l s_l / (1 + i w tau_sigma_l)),  s_l = (alpha_l - 1) / sum_alpha,
+
+    so each mechanism's strength is divided by sum_alpha, about L.  The s_l
+    solve Q Im M(w) = Re M(w) in the least-squares sense over the band, which
+    is linear in s: sum_l s_l (w tau_l + 1/Q) / (1 + (w tau_l)^2) = 1/Q.  Then
+    alpha_l = 1 + s_l L / (1 - sum s) (so that sum_alpha = L / (1 - sum s))
+    and tau_epsilon = alpha tau_sigma.
+
+    The P family (q_p) relaxes the bulk part of the reference's normal
+    stresses (lambda + mu in 2D, lambda + 2 mu / 3 in 3D) and the S family (q_s)
+    the shear modulus, so q_p = q_s = Q gives both body waves the quality
+    factor Q (in 3D along x and y; the reference's sigma_zz Zener terms,
+    wgsl3d :1440-1445, relax P waves along z more strongly).
+    Returns an ``AttenuationCoefficients``.
     """
     f_relax = np.geomspace(f_min, f_max, n_sls)
     tau_s = 1.0 / (2.0 * np.pi * f_relax)
     w = 2.0 * np.pi * np.geomspace(f_min, f_max, n_freq)
-    A = (w[:, None] * tau_s[None, :]) / (1.0 + (w[:, None] * tau_s[None, :]) ** 2)
+    wt = w[:, None] * tau_s[None, :]
     rows = []
     for q in (q_p, q_s):
-        y, *_ = np.linalg.lstsq(A, np.full(n_freq, 1.0 / q), rcond=None)
-        rows.append((tau_s * (1.0 + y), tau_s.copy()))
+        A = (wt + 1.0 / q) / (1.0 + wt ** 2)
+        s_l, *_ = np.linalg.lstsq(A, np.full(n_freq, 1.0 / q), rcond=None)
+        alpha = 1.0 + s_l * n_sls / (1.0 - s_l.sum())
+        rows.append((tau_s * alpha, tau_s.copy()))
     tau = (rows[0][0], rows[0][1], rows[1][0], rows[1][1])
     return AttenuationCoefficients(tau=tau)
 

@@ -174,19 +174,26 @@ int vse_profile(vse_sim *s, int n_steps, float *ms);
  * sim_webgpu_3D_viscoelastic.py:40-42). */
 int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap);
 
-/* Schedule of the 3D step.  VSE_SCHED_TWO_PASS (default): a stress pass and
- * a velocity pass per step, as the reference dispatches them
- * (sim_webgpu_3D_viscoelastic.py:646-651).  VSE_SCHED_FUSED: one kernel per
- * step does the velocity update of step n and the stress update of step n+1
- * (velocities recomputed on a 2-cell halo of each tile; sigma, v and the
- * velocity-pass CPML memory double-buffered), so the fields cross HBM once
- * per step; results are bit-identical to the two-pass schedule.  It covers
- * 3D viscoelastic simulations with 3 mechanisms and FD order 2 (acc = 2);
+/* Schedule of the step.  VSE_SCHED_TWO_PASS (default): a stress pass and a
+ * velocity pass per step, as the reference dispatches them
+ * (sim_webgpu_3D_viscoelastic.py:646-651, sim_webgpu_viscoelastic.py:488-509).
+ * VSE_SCHED_FUSED: one kernel per step does the velocity update of step n and
+ * the stress update of step n+1 (velocities recomputed on a halo of each tile;
+ * sigma, v and the velocity-pass CPML memory double-buffered), so the fields
+ * cross HBM once per step; results are bit-identical to the two-pass
+ * schedule.
+ * 3D: viscoelastic simulations with 3 mechanisms and FD order 2 (acc = 2);
  * anything else returns VSE_ERR_ARG.  Its launch shape is vse_set_tuning
- * kernel 2 (ty in {4, 8}, nt 0, xchunk >= 1, remap); vse_profile then reports
- * (fused kernel, 0, sparse kernel).  After a run the handle's fields may sit in
- * the second buffer set: vse_field_device_ptr changes, vse_read_* follow.
- * vse_run_noalloc always runs the two-pass schedule. */
+ * kernel 2 (ty in {4, 8}, nt 0, xchunk >= 1, remap).
+ * 2D: every variant and order; a wavefront owns a strip of 64 - 2*ord columns
+ * and marches xchunk rows (vse_set_tuning kernel 2: nt 0, xchunk at least
+ * about 4 so that the launch keeps one |v|^2 slot per workgroup; default 16).
+ * vse_profile then reports (fused kernel, 0, sparse kernel or step-counter
+ * kernel).  After a run the handle's fields may sit in the second buffer set:
+ * vse_field_device_ptr changes, vse_read_* follow.  vse_run_noalloc always
+ * runs the two-pass schedule.  Both fused kernels measured slower than the
+ * two-pass step on MI355X (register pressure: fewer waves per SIMD); the
+ * two-pass schedule stays the default. */
 #define VSE_SCHED_TWO_PASS 0
 #define VSE_SCHED_FUSED 1
 int vse_set_schedule(vse_sim *s, int sched);

@@ -120,6 +120,8 @@ struct vse_sim {
     (void)hipFree(it->second);
     allocs.erase(it);
   }
+  // the device step counter (ctrl[1])
+  int *ctrl_step() { return (int *)(ctrl + 1); }
   void drop_graph() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     gexec = nullptr;
@@ -266,6 +268,7 @@ int vse_create(const vse_desc *desc, vse_sim **out) {
   if (d.ndim == 2) {
     s->d.nz = 1;
     s->tune[0] = s->tune[1] = KTune{4, 0, 0, 1};   // XCD-contiguous tile order
+    s->tune[2] = KTune{4, 0, 16, 1};                // fused step: 16 x-rows per workgroup
   } else {
     s->rec_channel = VSE_F_SZZ;
   }
@@ -564,9 +567,11 @@ int vse_reset(vse_sim *s) {
   if (!rc && s->v2slots)
     rc = s->hip(hipMemsetAsync(s->v2slots, 0, (size_t)s->n_slots * sizeof(unsigned), s->stream), "reset slots");
   for (int f = 0; f < 9 && !rc && s->alt_ready; f++)
-    rc = s->hip(hipMemsetAsync(s->alt[f], 0, s->ncell * sizeof(float), s->stream), "reset");
+    if (s->alt[f]) rc = s->hip(hipMemsetAsync(s->alt[f], 0, s->ncell * sizeof(float), s->stream), "reset");
   for (int p = P_DSXX_DX; p < P_COUNT && !rc && s->alt_ready; p++)
-    rc = s->hip(hipMemsetAsync(s->psi_alt[p], 0, std::max<long>(s->psi_elems(p), 1) * sizeof(float), s->stream), "reset psi");
+    if (s->psi_alt[p])
+      rc = s->hip(hipMemsetAsync(s->psi_alt[p], 0, std::max<long>(s->psi_elems(p), 1) * sizeof(float), s->stream),
+                  "reset psi");
   if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "reset");
   return rc;
 }
@@ -693,9 +698,16 @@ static int check_stable(vse_sim *s) {
 // last step in place on the set in use.  The handle's field[] / psi[] then
 // point at the final state (the two sets are swapped when it ends in set 1).
 
+// The 2D fused kernel keeps one |v|^2 slot per workgroup: its grid must fit
+// the slots the two-pass velocity launch allocated.
+static bool fused2d_fits(const vse_sim *s, int xchunk) {
+  return s->d.ndim == 2 && xchunk >= 1 && vse_fused2d_blocks(s->d.nx, s->d.ny, s->d.ord, xchunk) <= s->n_slots;
+}
+
 static bool fused_active(const vse_sim *s) {
-  return s->sched == VSE_SCHED_FUSED && s->d.ndim == 3 && s->d.variant == VSE_VARIANT_WGSL &&
-         vse_fused3d_supported(s->ncell, s->rs, s->d.n_sls, s->d.visco, s->d.ord);
+  if (s->sched != VSE_SCHED_FUSED) return false;
+  if (s->d.ndim == 2) return fused2d_fits(s, s->tune[2].xchunk);
+  return s->d.variant == VSE_VARIANT_WGSL && vse_fused3d_supported(s->ncell, s->rs, s->d.n_sls, s->d.visco, s->d.ord);
 }
 
 static float *set_field(vse_sim *s, int set, int f) { return set ? s->alt[f] : s->field[f]; }
@@ -705,8 +717,10 @@ static float *set_psi(vse_sim *s, int set, int p) { return set && p >= P_DSXX_DX
 static int ensure_alt(vse_sim *s) {
   if (s->alt_ready) return 0;
   int rc = 0;
-  for (int f = 0; f < 9 && !rc; f++) rc = s->alloc(&s->alt[f], s->ncell);
-  for (int p = P_DSXX_DX; p < P_COUNT && !rc; p++) rc = s->alloc(&s->psi_alt[p], s->psi_elems(p));
+  for (int f = 0; f < 9 && !rc; f++)
+    if (s->field[f]) rc = s->alloc(&s->alt[f], s->ncell);
+  for (int p = P_DSXX_DX; p < P_COUNT && !rc; p++)
+    if (s->psi[p]) rc = s->alloc(&s->psi_alt[p], s->psi_elems(p));
   if (!rc) rc = s->hip(hipStreamSynchronize(s->stream), "second buffer set");
   if (rc) {
     for (int f = 0; f < 9; f++) { s->release(s->alt[f]); s->alt[f] = nullptr; }
@@ -734,7 +748,10 @@ static void make_args_set(vse_sim *s, StepArgs &A, SparseArgs &S, int set) {
 // that step without sources (the fused kernel injected them): a stress channel
 // is read from set p (step m+1's stress is in set 1-p already), a velocity
 // channel from set 1-p.
-static hipError_t enqueue_fused(vse_sim *s, int p, hipStream_t st, hipEvent_t mid = nullptr) {
+// 2D: the fused launch advances step *step + i (the counter moves once per
+// graph, as in the two-pass loop) and records a stress channel itself; a
+// velocity channel is gathered from set 1-p after it.
+static hipError_t enqueue_fused(vse_sim *s, int p, hipStream_t st, hipEvent_t mid = nullptr, int i = 0) {
   FusedArgs F;
   SparseArgs S;
   make_args_set(s, F.A, S, p);
@@ -742,7 +759,17 @@ static hipError_t enqueue_fused(vse_sim *s, int p, hipStream_t st, hipEvent_t mi
   const int sig[6] = {VSE_F_SXX, VSE_F_SYY, VSE_F_SZZ, VSE_F_SXY, VSE_F_SXZ, VSE_F_SYZ};
   for (int c = 0; c < 3; c++) F.vo[c] = set_field(s, o, VSE_F_VX + c);
   for (int c = 0; c < 6; c++) F.so[c] = set_field(s, o, sig[c]);
-  for (int i = 0; i < 9; i++) F.psio[i] = set_psi(s, o, P_DSXX_DX + i);
+  for (int k = 0; k < 9; k++) F.psio[k] = set_psi(s, o, P_DSXX_DX + k);
+  if (s->d.ndim == 2) {
+    hipError_t e = vse_launch_fused2d(s->d.variant, s->d.visco, s->d.ord, s->math, F, s->tune[2].xchunk, i, st);
+    if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
+    if (e == hipSuccess && F.A.rec_after) {
+      StepArgs B;
+      make_args_set(s, B, S, o);
+      e = vse_launch_record2d(B, i, st);
+    }
+    return e;
+  }
   hipError_t e = vse_launch_fused3d(s->math, F, s->tune[2], st);
   if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
   if (e != hipSuccess) return e;
@@ -761,7 +788,8 @@ static int get_fused_graph(vse_sim *s, int steps) {
   int rc = s->hip(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   if (rc) return rc;
   hipError_t e = hipSuccess;
-  for (int i = 0; i < steps && e == hipSuccess; i++) e = enqueue_fused(s, i & 1, s->stream);
+  for (int i = 0; i < steps && e == hipSuccess; i++) e = enqueue_fused(s, i & 1, s->stream, nullptr, i);
+  if (e == hipSuccess && s->d.ndim == 2) e = vse_launch_advance(s->ctrl_step(), steps, s->stream);
   hipError_t e2 = hipStreamEndCapture(s->stream, &g);
   if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return s->hip(e, "capture launch"); }
   rc = s->hip(e2, "end capture");
@@ -784,8 +812,19 @@ static hipError_t enqueue_fused_tail(vse_sim *s, int p) {
   StepArgs A;
   SparseArgs S;
   make_args_set(s, A, S, p);
+  if (s->d.ndim == 2) {
+    hipError_t e = vse_launch_velocity2d(s->d.variant, s->d.visco, s->d.ord, s->math, A, s->tune[1], 0, s->stream);
+    if (e == hipSuccess && A.rec_after) e = vse_launch_record2d(A, 0, s->stream);
+    return e == hipSuccess ? vse_launch_advance(s->ctrl_step(), 1, s->stream) : e;
+  }
   hipError_t e = vse_launch_velocity3d(s->d.visco, s->d.ord, s->math, A, s->tune[1], s->stream);
   return e == hipSuccess ? vse_launch_sparse(S, s->stream) : e;
+}
+
+// The stress pass that opens a fused run, in place on set 0.
+static hipError_t enqueue_fused_head(vse_sim *s, const StepArgs &A) {
+  if (s->d.ndim == 2) return vse_launch_stress2d(s->d.variant, s->d.visco, s->d.ord, s->math, A, s->tune[0], s->stream);
+  return vse_launch_stress3d(s->d.visco, s->d.ord, s->math, A, s->tune[0], s->stream);
 }
 
 static int run_fused(vse_sim *s, int n_steps, int check_every, float *elapsed_ms) {
@@ -800,7 +839,7 @@ static int run_fused(vse_sim *s, int n_steps, int check_every, float *elapsed_ms
     if (!rc) rc = s->hip(hipEventCreate(&e1), "event");
     if (!rc) rc = s->hip(hipEventRecord(e0, s->stream), "event record");
   }
-  if (!rc) rc = s->hip(vse_launch_stress3d(s->d.visco, s->d.ord, s->math, A, s->tune[0], s->stream), "stress launch");
+  if (!rc) rc = s->hip(enqueue_fused_head(s, A), "stress launch");
   const int nf = n_steps - 1;
   int chunk = std::min(std::min(check_every > 0 ? check_every : 256, 256), nf) & ~1;
   if (!rc && chunk >= 2) rc = get_fused_graph(s, chunk);
@@ -812,7 +851,8 @@ static int run_fused(vse_sim *s, int n_steps, int check_every, float *elapsed_ms
       done += chunk;
       since_check += chunk;
     } else {
-      for (int i = 0; i < left && !rc; i++, p ^= 1) rc = s->hip(enqueue_fused(s, p, s->stream), "fused launch");
+      for (int i = 0; i < left && !rc; i++, p ^= 1) rc = s->hip(enqueue_fused(s, p, s->stream, nullptr, i), "fused launch");
+      if (!rc && s->d.ndim == 2) rc = s->hip(vse_launch_advance(s->ctrl_step(), left, s->stream), "step launch");
       done += left;
       since_check += left;
     }
@@ -848,12 +888,13 @@ static int profile_fused(vse_sim *s, int n_steps, float *ms) {
   make_args(s, A, S);
   hipEvent_t ev[3] = {};
   for (int i = 0; i < 3 && !rc; i++) rc = s->hip(hipEventCreate(&ev[i]), "event");
-  if (!rc) rc = s->hip(vse_launch_stress3d(s->d.visco, s->d.ord, s->math, A, s->tune[0], s->stream), "stress launch");
+  if (!rc) rc = s->hip(enqueue_fused_head(s, A), "stress launch");
   double acc[2] = {0, 0};
   int p = 0;
   for (int it = 0; it < n_steps - 1 && !rc; it++, p ^= 1) {
     hipError_t e = hipEventRecord(ev[0], s->stream);
     if (e == hipSuccess) e = enqueue_fused(s, p, s->stream, ev[1]);
+    if (e == hipSuccess && s->d.ndim == 2) e = vse_launch_advance(s->ctrl_step(), 1, s->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
     rc = s->hip(e, "profile step");
@@ -880,8 +921,10 @@ static int profile_fused(vse_sim *s, int n_steps, float *ms) {
 int vse_set_schedule(vse_sim *s, int sched) {
   if (check_handle(s) || (sched != VSE_SCHED_TWO_PASS && sched != VSE_SCHED_FUSED)) return VSE_ERR_ARG;
   OnDevice on_dev(s->device);
-  if (sched == VSE_SCHED_FUSED &&
-      !(s->d.ndim == 3 && s->d.variant == VSE_VARIANT_WGSL && vse_fused3d_supported(s->ncell, s->rs, s->d.n_sls, s->d.visco, s->d.ord)))
+  if (sched == VSE_SCHED_FUSED && s->d.ndim == 2 && !fused2d_fits(s, s->tune[2].xchunk))
+    return s->fail(VSE_ERR_ARG, "the fused 2D step needs x-chunks of at least ~4 rows (one |v|^2 slot per workgroup)");
+  if (sched == VSE_SCHED_FUSED && s->d.ndim == 3 &&
+      !(s->d.variant == VSE_VARIANT_WGSL && vse_fused3d_supported(s->ncell, s->rs, s->d.n_sls, s->d.visco, s->d.ord)))
     return s->fail(VSE_ERR_ARG, "the fused step covers 3D viscoelastic simulations with 3 mechanisms and FD order 2 "
                                 "(acc = 2), with 3*cells*4 bytes below 2^32");
   s->sched = sched;
@@ -1018,9 +1061,12 @@ int vse_set_tuning(vse_sim *s, int kernel, int ty, int nt, int xchunk, int remap
   if ((nt & 2) && (s->d.ndim != 3 || kernel != 1 || xchunk < 1 || ty < 2 * s->d.ord))
     return s->fail(VSE_ERR_ARG, "tuning: nt bit 1 (LDS-staged y-neighbours) needs the 3D velocity pass (kernel 1), "
                                 "xchunk >= 1 and ty >= 2 * FD order");
-  if (kernel == 2 && (ty == 1 || nt != 0 || xchunk < 1))
+  if (kernel == 2 && s->d.ndim == 3 && (ty == 1 || nt != 0 || xchunk < 1))
     return s->fail(VSE_ERR_ARG, "fused-step tuning: ty in {4,8}, nt 0, xchunk >= 1");
-  if (s->d.ndim == 2 && (kernel == 2 || xchunk > 16 || nt > 1 || (nt == 1 && kernel != 1)))
+  if (kernel == 2 && s->d.ndim == 2 && (nt != 0 || !fused2d_fits(s, xchunk)))
+    return s->fail(VSE_ERR_ARG, "fused 2D tuning: nt 0, xchunk = x-rows per workgroup, at least ~4 "
+                                "(one |v|^2 slot per workgroup)");
+  if (s->d.ndim == 2 && kernel != 2 && (xchunk > 16 || nt > 1 || (nt == 1 && kernel != 1)))
     return s->fail(VSE_ERR_ARG, "2D tuning: kernel 0 or 1, xchunk (row groups per workgroup) 0 .. 16, "
                                 "nt 1 (x-row march) for the velocity pass only");
   OnDevice on_dev(s->device);

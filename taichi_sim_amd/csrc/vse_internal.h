@@ -249,6 +249,14 @@ bool vse_fused3d_supported(long ncell, long rs, int L, int visco, int ord);
 // t.ty in {4, 8}: the tile height; t.xchunk: x-planes per workgroup; t.remap: tile order
 hipError_t vse_launch_fused3d(int math, const FusedArgs &F, const KTune &t, hipStream_t st);
 hipError_t vse_launch_velocity3d(int visco, int ord, int math, const StepArgs &A, const KTune &t, hipStream_t st);
+// Fused 2D step (velocity of step *step + i, then stress of the next step)
+// reading F.A and writing F.vo[0..1] (vx, vy), F.so[0], F.so[1], F.so[3] (sxx,
+// syy, sxy) and F.psio[0, 1, 3, 4] (the velocity-pass psi slots); xchunk rows per
+// workgroup.  vse_fused2d_blocks is its workgroup count (each keeps one |v|^2 slot).
+long vse_fused2d_blocks(int nx, int ny, int ord, int xchunk);
+hipError_t vse_launch_fused2d(int variant, int visco, int ord, int math, const FusedArgs &F, int xchunk, int i,
+                              hipStream_t st);
+
 hipError_t vse_launch_sparse(const SparseArgs &S, hipStream_t st);
 // the velocity pass (stress = false) advances every member's step *step + i
 // and does its sources and receivers; vse_launch_advance_batch moves the

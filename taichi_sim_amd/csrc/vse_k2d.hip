@@ -24,14 +24,27 @@ __device__ __forceinline__ float dterm2(const StepArgs &A, int c, float a, float
 }
 
 
-// LC > 0: the mechanism count is the compile-time LC and every memory-variable
-// load of a point is issued before its arithmetic (same operations, same
-// order: bit-identical to the runtime-L loop, which waits on each
-// mechanism's loads in turn).
-template <int VAR, bool VISCO, int ORD, int LC = 0, bool FAST = false>
-__device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
+// Velocity operands of a stress cell read from the fields: vx(dx, dy) is vx
+// at (x + dx, y + dy).
+struct VGlobal {
+  const VSE_G float *vx_, *vy_;
+  long id;
+  int ny;
+  __device__ __forceinline__ float vx(int dx, int dy) const { return vx_[id + (long)dx * ny + dy]; }
+  __device__ __forceinline__ float vy(int dx, int dy) const { return vy_[id + (long)dx * ny + dy]; }
+};
+
+// The stress update of cell (x, y) < (nx, ny): V supplies the velocity
+// operands, sigma is read from A and written to osxx / osyy / osxy (A's own
+// fields in place, or another buffer set), memory variables and the
+// stress-pass psi are updated in place.  LC > 0: the mechanism count is the
+// compile-time LC and every memory-variable load of a point is issued before
+// its arithmetic (same operations, same order: bit-identical to the
+// runtime-L loop, which waits on each mechanism's loads in turn).
+template <int VAR, bool VISCO, int ORD, int LC, bool FAST, class VS>
+__device__ __forceinline__ void stress2d_core(const StepArgs &A, int x, int y, const VS &V, VSE_G float *osxx,
+                                              VSE_G float *osyy, VSE_G float *osxy) {
   const int nx = A.nx, ny = A.ny;
-  if (x >= nx || y >= ny) return;
   constexpr int last = ORD - 1, off = ORD - 1;
   const long id = (long)x * ny + y;
   const float dt = A.dt;
@@ -41,8 +54,8 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
     float dvx = 0.f, dvy = 0.f;
 #pragma unroll
     for (int c = 0; c < ORD; c++) {
-      dvx += dterm2<VAR, FAST, 0>(A, c, A.vx[id + (long)(c + 1) * ny], A.vx[id - (long)c * ny]);
-      dvy += dterm2<VAR, FAST, 1>(A, c, A.vy[id + c], A.vy[id - c - 1]);
+      dvx += dterm2<VAR, FAST, 0>(A, c, V.vx(c + 1, 0), V.vx(-c, 0));
+      dvy += dterm2<VAR, FAST, 1>(A, c, V.vy(0, c), V.vy(0, -c - 1));
     }
     const int jx = x - off, jy = y - off;
     const int bx = band_index(A.ax[0], jx), by = band_index(A.ax[1], jy);
@@ -60,8 +73,8 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
     const float mu = rho_h * (cs_m * cs_m);
     if constexpr (VAR == VSE_VAR_TAICHI) {
       const float dl = lambda * (dvx + dvy);
-      A.sxx[id] += (dl + 2.0f * mu * dvx) * dt;
-      A.syy[id] += (dl + 2.0f * mu * dvy) * dt;
+      osxx[id] = A.sxx[id] + (dl + 2.0f * mu * dvx) * dt;
+      osyy[id] = A.syy[id] + (dl + 2.0f * mu * dvy) * dt;
     } else {
       const float lp2m = lambda + 2.0f * mu;
       if constexpr (VISCO && LC > 0) {
@@ -83,8 +96,8 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
           A.r[0][l * A.rs + id] = nxx;
           A.r[1][l * A.rs + id] = nyy;
         }
-        A.sxx[id] = sxx0 + (lp2m * dvx + lambda * dvy + 0.5f * lpm * sum_xx + mu * sum_yy) * dt;
-        A.syy[id] = syy0 + (lambda * dvx + lp2m * dvy + 0.5f * lpm * sum_xx - mu * sum_yy) * dt;
+        osxx[id] = sxx0 + (lp2m * dvx + lambda * dvy + 0.5f * lpm * sum_xx + mu * sum_yy) * dt;
+        osyy[id] = syy0 + (lambda * dvx + lp2m * dvy + 0.5f * lpm * sum_xx - mu * sum_yy) * dt;
       } else if constexpr (VISCO) {
         const float lpm = lambda + mu;
         float sum_xx = 0.f, sum_yy = 0.f;
@@ -98,11 +111,11 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
           *pxx = nxx;
           *pyy = nyy;
         }
-        A.sxx[id] = A.sxx[id] + (lp2m * dvx + lambda * dvy + 0.5f * lpm * sum_xx + mu * sum_yy) * dt;
-        A.syy[id] = A.syy[id] + (lambda * dvx + lp2m * dvy + 0.5f * lpm * sum_xx - mu * sum_yy) * dt;
+        osxx[id] = A.sxx[id] + (lp2m * dvx + lambda * dvy + 0.5f * lpm * sum_xx + mu * sum_yy) * dt;
+        osyy[id] = A.syy[id] + (lambda * dvx + lp2m * dvy + 0.5f * lpm * sum_xx - mu * sum_yy) * dt;
       } else {
-        A.sxx[id] = A.sxx[id] + (lp2m * dvx + lambda * dvy) * dt;
-        A.syy[id] = A.syy[id] + (lambda * dvx + lp2m * dvy) * dt;
+        osxx[id] = A.sxx[id] + (lp2m * dvx + lambda * dvy) * dt;
+        osyy[id] = A.syy[id] + (lambda * dvx + lp2m * dvy) * dt;
       }
     }
   }
@@ -112,8 +125,8 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
     float dvydx = 0.f, dvxdy = 0.f;
 #pragma unroll
     for (int c = 0; c < ORD; c++) {
-      dvydx += dterm2<VAR, FAST, 0>(A, c, A.vy[id + (long)c * ny], A.vy[id - (long)(c + 1) * ny]);
-      dvxdy += dterm2<VAR, FAST, 1>(A, c, A.vx[id + c + 1], A.vx[id - c]);
+      dvydx += dterm2<VAR, FAST, 0>(A, c, V.vy(c, 0), V.vy(-c - 1, 0));
+      dvxdy += dterm2<VAR, FAST, 1>(A, c, V.vx(0, c + 1), V.vx(0, -c));
     }
     const int jx = x - off, jy = y - off;
     const int bx = band_index(A.ax[0], jx), by = band_index(A.ax[1], jy);
@@ -138,7 +151,7 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
         sum_xy += o[l] + n;
         A.r[3][l * A.rs + id] = n;
       }
-      A.sxy[id] = sxy0 + (mu * (dvydx + dvxdy) + 0.5f * mu * sum_xy) * dt;
+      osxy[id] = sxy0 + (mu * (dvydx + dvxdy) + 0.5f * mu * sum_xy) * dt;
     } else if constexpr (VISCO && VAR == VSE_VAR_WGSL) {
       float sum_xy = 0.f;
       for (int l = 0; l < A.L; l++) {
@@ -148,11 +161,18 @@ __device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
         sum_xy += o + n;
         *pxy = n;
       }
-      A.sxy[id] = A.sxy[id] + (mu * (dvydx + dvxdy) + 0.5f * mu * sum_xy) * dt;
+      osxy[id] = A.sxy[id] + (mu * (dvydx + dvxdy) + 0.5f * mu * sum_xy) * dt;
     } else {
-      A.sxy[id] = A.sxy[id] + (dvxdy + dvydx) * mu * dt;
+      osxy[id] = A.sxy[id] + (dvxdy + dvydx) * mu * dt;
     }
   }
+}
+
+template <int VAR, bool VISCO, int ORD, int LC = 0, bool FAST = false>
+__device__ __forceinline__ void stress2d_cell(const StepArgs &A, int x, int y) {
+  if (x >= A.nx || y >= A.ny) return;
+  const VGlobal V{A.vx, A.vy, (long)x * A.ny + y, A.ny};
+  stress2d_core<VAR, VISCO, ORD, LC, FAST>(A, x, y, V, A.sxx, A.syy, A.sxy);
 }
 
 // The stencil operands of one velocity cell: vx, vy at the cell, sigma_xx
@@ -183,9 +203,13 @@ __device__ __forceinline__ int source_row(const StepArgs &A, int x) {
 // source_row of the cell's row.  The WGSL source comes after the update,
 // anywhere on the grid (wgsl2d :1059-1065): vy += src[s, col] * dt / rho at
 // each source cell, rho the vy-grid density with out-of-grid neighbours 0.
+// The update itself: psi is read from A and, when `store`, the new psi is
+// written to po (the slots P_DSXX_DX, P_DSXY_DY, P_DSXY_DX, P_DSYY_DY of A in
+// place, or of another buffer set); the new velocity is returned in vxo / vyo.
 template <int VAR, bool VISCO, int ORD, bool FAST = false>
-__device__ __forceinline__ unsigned velocity2d_apply(const StepArgs &A, int x, int y, int s, int srow,
-                                                     VelOps2<ORD> &o) {
+__device__ __forceinline__ unsigned velocity2d_update(const StepArgs &A, int x, int y, int s, int srow,
+                                                      VelOps2<ORD> &o, float &vxo, float &vyo,
+                                                      VSE_G float *const *po, bool store) {
   const int nx = A.nx, ny = A.ny;
   constexpr int last = ORD - 1, off = ORD - 1;
   const long id = (long)x * ny + y;
@@ -236,12 +260,12 @@ __device__ __forceinline__ unsigned velocity2d_apply(const StepArgs &A, int x, i
     }
     if (bx >= 0) {
       const float pn = xb * q1 + xa * d1;
-      A.psi[P_DSXX_DX][pxi] = pn;
+      if (store) po[0][pxi] = pn;
       d1 = (FAST ? d1 * xk : d1 / xk) + pn;
     }
     if (by >= 0) {
       const float pn = yb * q2 + ya * d2;
-      A.psi[P_DSXY_DY][pyi] = pn;
+      if (store) po[1][pyi] = pn;
       d2 = (FAST ? d2 * yk : d2 / yk) + pn;
     }
     if (o.r00 > 0.f) {
@@ -261,12 +285,12 @@ __device__ __forceinline__ unsigned velocity2d_apply(const StepArgs &A, int x, i
     }
     if (bx >= 0) {
       const float pn = xbh * q3 + xah * d1;
-      A.psi[P_DSXY_DX][pxi] = pn;
+      if (store) po[2][pxi] = pn;
       d1 = (FAST ? d1 * xkh : d1 / xkh) + pn;
     }
     if (by >= 0) {
       const float pn = ybh * q4 + yah * d2;
-      A.psi[P_DSYY_DY][pyi] = pn;
+      if (store) po[3][pyi] = pn;
       d2 = (FAST ? d2 * ykh : d2 / ykh) + pn;
     }
     if constexpr (VAR == VSE_VAR_TAICHI) {
@@ -296,6 +320,19 @@ __device__ __forceinline__ unsigned velocity2d_apply(const StepArgs &A, int x, i
       }
     }
   }
+  vxo = vxn;
+  vyo = vyn;
+  return u;
+}
+
+// The update of cell (x, y) in place: psi and v written back to A.
+template <int VAR, bool VISCO, int ORD, bool FAST = false>
+__device__ __forceinline__ unsigned velocity2d_apply(const StepArgs &A, int x, int y, int s, int srow,
+                                                     VelOps2<ORD> &o) {
+  VSE_G float *const po[4] = {A.psi[P_DSXX_DX], A.psi[P_DSXY_DY], A.psi[P_DSXY_DX], A.psi[P_DSYY_DY]};
+  float vxn, vyn;
+  const unsigned u = velocity2d_update<VAR, VISCO, ORD, FAST>(A, x, y, s, srow, o, vxn, vyn, po, true);
+  const long id = (long)x * A.ny + y;
   A.vx[id] = vxn;
   A.vy[id] = vyn;
   return u;
@@ -465,6 +502,126 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d_march(StepArgs A, int nt
     }
   }
   if (blockIdx.x == 0 && !A.rec_after) record_receivers(A, s);
+  wave_max_to_slot(v2, A.v2slots + blockIdx.x);
+}
+
+// ---- fused 2D step: velocity of step n, then stress of step n+1 -----------
+// One wavefront per workgroup owns a strip of W = 64 - 2 ORD columns and
+// marches x over a chunk of rows.  Lane j holds column y0 - ORD + j: the ORD
+// lanes on each side are a halo that computes velocities only.  At row x the
+// wavefront computes v(n+1) of row x + ORD over all 64 columns (register
+// windows over x as in k_velocity2d_march), keeps v of rows x - ORD .. x + ORD
+// of its own column in a register ring, and updates the stress of row x in
+// the owned lanes, taking the y-neighbour velocities of row x from the lanes
+// beside it (cross-lane shuffles).  The march starts 2 ORD velocity rows
+// early.
+// * Halo lanes and lead-in rows compute velocities other wavefronts own,
+//   from the same inputs with the same arithmetic, so every copy has the same
+//   bits; only the owner stores.  Those inputs must not change during the
+//   launch, so sigma, v and the velocity-pass psi are read from one buffer set
+//   (F.A) and written to the other (F.so, F.vo, F.psio).  Memory variables and
+//   the stress-pass psi are touched only by their own cell and stay in place.
+// * sigma outside the stress interiors is never written by either schedule,
+//   so both sets keep the value both started with there.
+// * Every expression is the two-pass kernels' (velocity2d_update, then
+//   stress2d_core), in the same order: the fused step is bit-identical.
+// * The receivers of step n read the stress channel from the input set, where
+//   the previous stress pass left it final (a velocity channel is gathered
+//   from the output set after the launch).
+template <int ORD>
+struct VRing {
+  float rx[2 * ORD + 1], ry[2 * ORD + 1];   // v of rows x - ORD .. x + ORD, own column
+  float sx[2 * ORD + 1], sy[2 * ORD + 1];   // v of row x, columns y - ORD .. y + ORD
+  __device__ __forceinline__ float vx(int dx, int dy) const { return dy == 0 ? rx[ORD + dx] : sx[ORD + dy]; }
+  __device__ __forceinline__ float vy(int dx, int dy) const { return dy == 0 ? ry[ORD + dx] : sy[ORD + dy]; }
+};
+
+template <int VAR, bool VISCO, int ORD, int LC, bool FAST>
+__global__ __launch_bounds__(64) void k_fused2d(FusedArgs F, int xchunk, int nstrip, int i) {
+  constexpr int W = 64 - 2 * ORD;
+  const StepArgs &A = F.A;
+  const int nx = A.nx, ny = A.ny;
+  const int b = xcd_tile(blockIdx.x, gridDim.x, 1);
+  const int strip = b % nstrip, chunk = b / nstrip;
+  const int lane = threadIdx.x;
+  const int y = strip * W - ORD + lane;
+  const bool col = y >= 0 && y < ny;
+  const bool own = lane >= ORD && lane < 64 - ORD && y < ny;
+  const int x0 = chunk * xchunk, x1 = min(nx, x0 + xchunk);
+  const int s = *A.step + i;
+  if (blockIdx.x == 0 && !A.rec_after) record_receivers(A, s);
+  const FieldBuf Bvx(A.vx, A.ncell), Bvy(A.vy, A.ncell), Bsxx(A.sxx, A.ncell), Bsxy(A.sxy, A.ncell),
+      Bsyy(A.syy, A.ncell), Brho(A.rho, A.ncell);
+  VSE_G float *const po[4] = {F.psio[0], F.psio[1], F.psio[3], F.psio[4]};
+  // windows over the velocity rows xv: wxx[j] = sxx(xv - ORD + j), wxy[j] = sxy(xv - ORD + 1 + j)
+  float wxx[2 * ORD], wxy[2 * ORD];
+  const int xv0 = x0 - ORD;
+  const int i0 = xv0 * ny + y;
+#pragma unroll
+  for (int j = 0; j < 2 * ORD - 1; j++) {
+    wxx[j + 1] = Bsxx[i0 + (j - ORD) * ny];
+    wxy[j + 1] = Bsxy[i0 + (j - ORD + 1) * ny];
+  }
+  float rn0 = Brho[i0], rn1 = Brho[i0 + 1];
+  unsigned v2 = 0;
+  VRing<ORD> R;
+  // v(n+1) of row xv in this lane's column (0 outside the grid, where no
+  // stress operand uses it); stored when this wavefront owns the cell
+  auto vel_row = [&](int xv, float &vxo, float &vyo) {
+    const int ic = xv * ny + y;
+#pragma unroll
+    for (int j = 0; j < 2 * ORD - 1; j++) { wxx[j] = wxx[j + 1]; wxy[j] = wxy[j + 1]; }
+    wxx[2 * ORD - 1] = Bsxx[ic + (ORD - 1) * ny];
+    wxy[2 * ORD - 1] = Bsxy[ic + ORD * ny];
+    VelOps2<ORD> o;
+    o.vxn = Bvx[ic];
+    o.vyn = Bvy[ic];
+#pragma unroll
+    for (int c = 0; c < ORD; c++) {
+      o.xx_p[c] = wxx[ORD + c];
+      o.xx_m[c] = wxx[ORD - 1 - c];
+      o.xy_p[c] = c == 0 ? wxy[ORD - 1] : Bsxy[ic + c];
+      o.xy_m[c] = Bsxy[ic - c - 1];
+      o.yx_p[c] = wxy[ORD + c];
+      o.yx_m[c] = wxy[ORD - 1 - c];
+      o.yy_p[c] = Bsyy[ic + c + 1];
+      o.yy_m[c] = Bsyy[ic - c];
+    }
+    o.r00 = rn0;
+    o.r01 = rn1;
+    rn0 = Brho[ic + ny];
+    rn1 = Brho[ic + ny + 1];
+    o.r10 = rn0;
+    o.r11 = rn1;
+    vxo = 0.f;
+    vyo = 0.f;
+    if (col && xv >= 0 && xv < nx) {
+      const bool mine = own && xv >= x0 && xv < x1;
+      const unsigned u = velocity2d_update<VAR, VISCO, ORD, FAST>(A, xv, y, s, source_row<VAR>(A, xv), o, vxo, vyo,
+                                                                  po, mine);
+      if (mine) {
+        const long id = (long)xv * ny + y;
+        F.vo[0][id] = vxo;
+        F.vo[1][id] = vyo;
+        v2 = v2 > u ? v2 : u;
+      }
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < 2 * ORD; k++) vel_row(xv0 + k, R.rx[k], R.ry[k]);
+#pragma unroll 1
+  for (int x = x0; x < x1; x++) {
+    vel_row(x + ORD, R.rx[2 * ORD], R.ry[2 * ORD]);
+    // row x's velocities of the neighbouring columns; every lane takes part
+#pragma unroll
+    for (int d = -ORD; d <= ORD; d++) {
+      R.sx[ORD + d] = __shfl(R.rx[ORD], lane + d, 64);
+      R.sy[ORD + d] = __shfl(R.ry[ORD], lane + d, 64);
+    }
+    if (own) stress2d_core<VAR, VISCO, ORD, LC, FAST>(A, x, y, R, F.so[0], F.so[1], F.so[3]);
+#pragma unroll
+    for (int k = 0; k < 2 * ORD; k++) { R.rx[k] = R.rx[k + 1]; R.ry[k] = R.ry[k + 1]; }
+  }
   wave_max_to_slot(v2, A.v2slots + blockIdx.x);
 }
 
@@ -752,6 +909,38 @@ hipError_t vse_launch_batch2d_persistent(int variant, int visco, int ord, int ma
   int total = total_tiles, n = nsteps, ra = rec_after ? 1 : 0;
   void *args[] = {(void *)&b, (void *)&total, (void *)&n, (void *)&ra, (void *)&bar};
   return hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3(BY, BX), args, 0, st);
+}
+
+// Fused 2D step: grid of strips (64 - 2 ORD columns) x x-chunks.
+long vse_fused2d_blocks(int nx, int ny, int ord, int xchunk) {
+  const int W = 64 - 2 * ord;
+  return (long)((ny + W - 1) / W) * ((nx + xchunk - 1) / xchunk);
+}
+
+hipError_t vse_launch_fused2d(int variant, int visco, int ord, int math, const FusedArgs &F, int xchunk, int i,
+                              hipStream_t st) {
+  if (xchunk < 1 || ord < 2 || ord > VSE_MAX_ORD) return hipErrorInvalidValue;
+  const StepArgs &A = F.A;
+  const int nstrip = (A.ny + (64 - 2 * ord) - 1) / (64 - 2 * ord);
+  const dim3 grid((unsigned)vse_fused2d_blocks(A.nx, A.ny, ord, xchunk)), block(64);
+  const bool fast = math == VSE_MATH_FAST;
+  const bool l3 = visco && variant == VSE_VAR_WGSL && A.L == 3;
+#define VSE_F2(VAR, VISCO, FASTV)                                                                              \
+  switch (ord) {                                                                                              \
+    case 2: if (l3) hipLaunchKernelGGL((k_fused2d<VAR, VISCO, 2, 3, FASTV>), grid, block, 0, st, F, xchunk, nstrip, i); \
+            else hipLaunchKernelGGL((k_fused2d<VAR, VISCO, 2, 0, FASTV>), grid, block, 0, st, F, xchunk, nstrip, i); break; \
+    case 3: hipLaunchKernelGGL((k_fused2d<VAR, VISCO, 3, 0, FASTV>), grid, block, 0, st, F, xchunk, nstrip, i); break; \
+    case 4: if (l3) hipLaunchKernelGGL((k_fused2d<VAR, VISCO, 4, 3, FASTV>), grid, block, 0, st, F, xchunk, nstrip, i); \
+            else hipLaunchKernelGGL((k_fused2d<VAR, VISCO, 4, 0, FASTV>), grid, block, 0, st, F, xchunk, nstrip, i); break; \
+    case 5: hipLaunchKernelGGL((k_fused2d<VAR, VISCO, 5, 0, FASTV>), grid, block, 0, st, F, xchunk, nstrip, i); break; \
+    case 6: hipLaunchKernelGGL((k_fused2d<VAR, VISCO, 6, 0, FASTV>), grid, block, 0, st, F, xchunk, nstrip, i); break; \
+    default: return hipErrorInvalidValue;                                                                     \
+  }
+  if (variant == VSE_VAR_TAICHI) { VSE_F2(VSE_VAR_TAICHI, false, false) }
+  else if (visco) { if (fast) { VSE_F2(VSE_VAR_WGSL, true, true) } else { VSE_F2(VSE_VAR_WGSL, true, false) } }
+  else { if (fast) { VSE_F2(VSE_VAR_WGSL, false, true) } else { VSE_F2(VSE_VAR_WGSL, false, false) } }
+#undef VSE_F2
+  return hipGetLastError();
 }
 
 hipError_t vse_launch_advance_batch(const BatchArgs &B, int n, hipStream_t st) {

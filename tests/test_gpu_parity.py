@@ -296,6 +296,61 @@ def test_3d_fused_step_bit_identical(math, channel):
     g.close()
 
 
+def _state2d(g):
+    out = {k: g.field(k) for k in ("vx", "vy", "sxx", "syy", "sxy")}
+    if g.visco and g.variant == "wgsl":
+        for c in ("xx", "yy", "xy"):
+            out["r" + c] = g.memory(c)
+    out["traces"] = g.traces()
+    out["step"] = np.array([g.step()])
+    out["v2max"] = np.array([g.v2max()], dtype=np.float32)
+    return out
+
+
+@pytest.mark.parametrize("variant,visco,acc,math,channel", [
+    ("wgsl", True, 2, "strict", None), ("wgsl", True, 2, "fast", "vy"), ("wgsl", True, 4, "strict", None),
+    ("wgsl", False, 3, "strict", None), ("taichi", False, 2, "strict", None)])
+def test_2d_fused_step_bit_identical(variant, visco, acc, math, channel):
+    """The fused 2D schedule (one x-marching launch per step: velocity of step
+    n, then stress of step n+1, y-neighbour velocities by cross-lane shuffles,
+    sigma / v / velocity-pass psi double-buffered) ends bit for bit where the
+    two-pass schedule ends: fields, memory variables, traces (sigma or a
+    velocity channel), |v|^2 max and step counter, for several x-chunks, for
+    runs split into calls of 1, 2 and odd numbers of steps, on a grid whose
+    extents no strip or chunk divides, with voids, CPML on every side and an
+    array of sources inside the marched rows."""
+    steps = 75
+    cfg = workloads.visco_2d_array(n=(203, 171), steps=steps, n_elem=8, n_voids=2)
+    cfg["simul_params"]["acc"] = acc
+    sim = Simulator(cfg)
+    att = attenuation_from_config(cfg)
+    ref = VSESim.from_simulator(sim, variant=variant, visco=visco, att=att, channel=channel)
+    ref.set_math(math)
+    ref.run(steps, check_every=10)
+    want = _state2d(ref)
+    ref.close()
+    assert np.abs(want["syy"]).max() > 0 and np.abs(want["traces"]).max() > 0
+    g = VSESim.from_simulator(sim, variant=variant, visco=visco, att=att, channel=channel)
+    g.set_math(math)
+    g.set_schedule("fused")
+    print(f"fused 2D {variant} visco {visco} acc {acc} {math} channel {channel}", flush=True)
+    for xc, rm, split in [(16, 1, (75,)), (8, 0, (1, 2, 72)), (7, 1, (74, 1)), (37, 0, (6, 69)), (1000, 1, (75,)),
+                          (11, 1, (40, 35))]:
+        assert sum(split) == steps
+        g.set_tuning(2, 4, 0, xc, rm)
+        g.reset()
+        for n in split:
+            g.run(n, check_every=8)
+        got = _state2d(g)
+        for k in want:
+            assert np.array_equal(got[k], want[k]), ((xc, rm, split), k, float(np.abs(got[k] - want[k]).max()))
+    ms = g.profile(3)
+    assert ms[0] > 0 and ms[1] == 0
+    with pytest.raises(VSEError):
+        g.set_tuning(2, 4, 0, 1, 1)         # more workgroups than |v|^2 slots
+    g.close()
+
+
 def test_3d_fused_schedule_refusals():
     cfg = workloads.visco_3d_block(n=56, steps=4, n_incl=1, n_rec_side=2)
     cfg["simul_params"]["acc"] = 4
@@ -306,8 +361,10 @@ def test_3d_fused_schedule_refusals():
         g.set_schedule("fused")
     g.close()
     g = VSESim.from_simulator(Simulator(workloads.elastic_2d_point(n=64, steps=4, n_rec=2)), variant="wgsl")
+    g.set_tuning(2, 4, 0, 16, 1)
+    g.set_schedule("fused")                  # 2D: any variant and order
     with pytest.raises(VSEError):
-        g.set_schedule("fused")
+        g.set_tuning(2, 4, 0, 1, 1)          # one |v|^2 slot per workgroup: x-chunks of ~4 rows at least
     g.close()
 
 

@@ -101,10 +101,11 @@ def bytes_per_cell_2d(g, L):
 
 def bench_2d(args):
     """2D viscoelastic time loop at 2048^2 (see module doc)."""
-    emit(measure_2d(args.n2d, args.steps, args.warmup, args.math2d, args.profile_steps, args.cpu_steps))
+    emit(measure_2d(args.n2d, args.steps, args.warmup, args.math2d, args.profile_steps, args.cpu_steps,
+                    args.fused_sweep))
 
 
-def measure_2d(n, steps, warmup, math, profile_steps, cpu_steps):
+def measure_2d(n, steps, warmup, math, profile_steps, cpu_steps, fused_sweep=False):
     """The 2D line: Gcell-updates/s of the n^2 viscoelastic loop, its stress
     pass against the roofline and the CPU oracle on a bounded sample."""
     from taichi_sim_amd import autotune, workloads
@@ -136,6 +137,27 @@ def measure_2d(n, steps, warmup, math, profile_steps, cpu_steps):
     k_ms = g.profile(profile_steps)
     b_step, b_stress, b_vel = bytes_per_cell_2d(g, g.n_sls)
     achieved = b_stress * ncell / (k_ms[0] * 1e-3) / 1e9
+    fused = None
+    if fused_sweep:
+        # the opt-in fused schedule (velocity of step n + stress of step n+1 per
+        # launch) at several x-chunks, timed like the line above, then the
+        # two-pass step again in the same process
+        fused = {}
+        for xc in (8, 16, 32, 64):
+            g.set_tuning(2, 4, 0, xc, 1)
+            g.set_schedule("fused")
+            g.reset()
+            g.run(min(steps, 256), check_every=256)
+            g.reset()
+            t = time.perf_counter()
+            g.run(steps, check_every=500)
+            fused[f"xchunk {xc}"] = round(ncell * steps / (time.perf_counter() - t) / 1e9, 3)
+            print(f"# fused schedule, x-chunk {xc}: {fused[f'xchunk {xc}']} Gcell/s", file=sys.stderr, flush=True)
+        g.set_schedule("two-pass")
+        g.reset()
+        t = time.perf_counter()
+        g.run(steps, check_every=500)
+        fused["two-pass again"] = round(ncell * steps / (time.perf_counter() - t) / 1e9, 3)
     g.close()
     print(f"# timed {steps} steps: wall {wall:.3f} s, device {dev_ms / 1e3:.3f} s", file=sys.stderr, flush=True)
 
@@ -175,6 +197,8 @@ def measure_2d(n, steps, warmup, math, profile_steps, cpu_steps):
                      "kernel": "stress2d", "bytes_per_cell": round(b_stress, 2)},
         "cpu_baseline": cpu,
     }
+    if fused is not None:
+        line["fused_schedule_gcell_per_s"] = fused
     return line
 
 
@@ -236,6 +260,8 @@ def main():
     ap.add_argument("--sizes", default="512,1024,1536,2048",
                     help="specimen grid sizes (--workload shots), comma separated")
     ap.add_argument("--shots", type=int, default=16, help="array elements = FMC shots per specimen (--workload shots)")
+    ap.add_argument("--fused-sweep", action="store_true",
+                    help="--workload 2d: also time the opt-in fused schedule at x-chunks 8..64")
     ap.add_argument("--group-sweep", action="store_true",
                     help="also time the batch at cache-group budgets 0..320 MB (--workload shots)")
     ap.add_argument("--extras", type=int, default=1,

@@ -267,7 +267,7 @@ def main():
     ap.add_argument("--extras", type=int, default=1,
                     help="3D line: also measure batched FMC shots/s, the adjoint/forward ratio and the 2D "
                          "2048^2 line (key visco2d, reference operation order) (0: skip)")
-    ap.add_argument("--shots-window", type=int, default=240,
+    ap.add_argument("--shots-window", type=int, default=3000,
                     help="steps of each shot timed for the 3D line's shots/s (batch and per-shot loop)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 5 launches): rocprof counter calibration")
@@ -410,9 +410,8 @@ def main():
     if args.extras:
         # the rest of the metric: batched shots/s and the gradient's cost, each on its
         # own configuration, measured after the 3D handle is released
-        s = measure_shots((512, 1024, 1536, 2048), 16, args.shots_window, profile_steps=0)
+        s = measure_shots((512, 1024, 1536, 2048), 16, args.shots_window, profile_steps=0, persistent=False)
         line["shots_per_s"] = {"batched": s["value"], "per_shot_loop": s["per_shot_loop_shots_per_s"],
-                               "batched_persistent": s["batched_persistent_shots_per_s"],
                                "speedup": s["batch_speedup"], "shots": s["config"]["shots"],
                                "window_steps": s["config"]["window_steps"], "shot_steps": 3000,
                                "cache_groups": s["config"]["cache_groups"], "tuning": s["tuning"],
@@ -463,7 +462,7 @@ def bench_shots(args):
     emit(measure_shots(sizes, args.shots, args.steps, args.warmup, args.profile_steps, args.group_sweep))
 
 
-def measure_shots(sizes, n_shots, steps, warmup=0, profile_steps=10, group_sweep=False):
+def measure_shots(sizes, n_shots, steps, warmup=0, profile_steps=10, group_sweep=False, persistent=True):
     """FMC on plates of the given sizes x n_shots: every shot as one batch
     against the same shots one after another, over a window of `steps` steps of
     each shot; shots/s is for 3000-step shots (window time x 3000 / steps)."""
@@ -533,11 +532,13 @@ def measure_shots(sizes, n_shots, steps, warmup=0, profile_steps=10, group_sweep
     print(f"# per-shot loop: {wall_l:.3f} s", file=sys.stderr, flush=True)
     # the same batch under the persistent schedule: each group's window as
     # cooperative launches of up to `check` steps (grid-wide barriers between passes)
-    batch.set_schedule("persistent")
-    wall_p, dev_p = timed(lambda n: batch.run(n, check_every=check, timed=True))
-    batch.set_schedule("graph")
-    print(f"# batch, persistent schedule: {steps} steps in {wall_p:.3f} s (device {dev_p / 1e3:.3f} s)",
-          file=sys.stderr, flush=True)
+    wall_p = None
+    if persistent:
+        batch.set_schedule("persistent")
+        wall_p, dev_p = timed(lambda n: batch.run(n, check_every=check, timed=True))
+        batch.set_schedule("graph")
+        print(f"# batch, persistent schedule: {steps} steps in {wall_p:.3f} s (device {dev_p / 1e3:.3f} s)",
+              file=sys.stderr, flush=True)
     sweep = None
     if group_sweep:
         # batch Gcell/s at each cache-group budget (MB of member device memory per group)
@@ -571,7 +572,7 @@ def measure_shots(sizes, n_shots, steps, warmup=0, profile_steps=10, group_sweep
                    "shots": len(sims), "cells_per_step": cells, "shot_steps": shot_len, "window_steps": steps,
                    "parallelism": "single GPU, one batch", "cache_groups": len(groups)},
         "per_shot_loop_shots_per_s": round(shots / wall_l, 4),
-        "batched_persistent_shots_per_s": round(shots / wall_p, 4),
+        "batched_persistent_shots_per_s": None if wall_p is None else round(shots / wall_p, 4),
         "group_sweep_gcell_per_s": sweep,
         "batch_speedup": round(wall_l / wall_b, 3),
         "gcell_per_s_batched": round(cells * steps / wall_b / 1e9, 3),

@@ -506,21 +506,24 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d_march(StepArgs A, int nt
 }
 
 // ---- fused 2D step: velocity of step n, then stress of step n+1 -----------
-// One wavefront per workgroup owns a strip of W = 64 - 2 ORD columns and
-// marches x over a chunk of rows.  Lane j holds column y0 - ORD + j: the ORD
-// lanes on each side are a halo that computes velocities only.  At row x the
-// wavefront computes v(n+1) of row x + ORD over all 64 columns (register
-// windows over x as in k_velocity2d_march), keeps v of rows x - ORD .. x + ORD
-// of its own column in a register ring, and updates the stress of row x in
-// the owned lanes, taking the y-neighbour velocities of row x from the lanes
-// beside it (cross-lane shuffles).  The march starts 2 ORD velocity rows
-// early.
-// * Halo lanes and lead-in rows compute velocities other wavefronts own,
-//   from the same inputs with the same arithmetic, so every copy has the same
-//   bits; only the owner stores.  Those inputs must not change during the
-//   launch, so sigma, v and the velocity-pass psi are read from one buffer set
-//   (F.A) and written to the other (F.so, F.vo, F.psio).  Memory variables and
-//   the stress-pass psi are touched only by their own cell and stay in place.
+// A workgroup of two wavefronts owns a strip of W = 64 - 2 ORD columns and
+// marches x over a chunk of rows.  Lane j of each wavefront holds column
+// y0 - ORD + j; the ORD lanes on each side are a halo.
+// * Wavefront 0 (producer) computes v(n+1) row by row over all 64 columns,
+//   starting 2 ORD rows before the chunk (register windows over x as in
+//   k_velocity2d_march), stores the rows it owns and puts every row into an
+//   LDS ring of 2 ORD + 2 rows.
+// * Wavefront 1 (consumer) updates the stress of the chunk's rows in the owned
+//   lanes, one row behind: at iteration t the producer writes ring row t while
+//   the consumer reads rows t-1-2 ORD .. t-1 (one barrier per iteration; the
+//   ring is one row longer than those, so the row being written is never one
+//   being read).  Each role keeps only its own registers.
+// * Halo lanes and lead-in rows compute velocities other workgroups own, from
+//   the same inputs with the same arithmetic, so every copy has the same bits;
+//   only the owner stores.  Those inputs must not change during the launch, so
+//   sigma, v and the velocity-pass psi are read from one buffer set (F.A) and
+//   written to the other (F.so, F.vo, F.psio).  Memory variables and the
+//   stress-pass psi are touched only by their own cell and stay in place.
 // * sigma outside the stress interiors is never written by either schedule,
 //   so both sets keep the value both started with there.
 // * Every expression is the two-pass kernels' (velocity2d_update, then
@@ -529,98 +532,105 @@ __global__ __launch_bounds__(BX * BY) void k_velocity2d_march(StepArgs A, int nt
 //   the previous stress pass left it final (a velocity channel is gathered
 //   from the output set after the launch).
 template <int ORD>
-struct VRing {
-  float rx[2 * ORD + 1], ry[2 * ORD + 1];   // v of rows x - ORD .. x + ORD, own column
-  float sx[2 * ORD + 1], sy[2 * ORD + 1];   // v of row x, columns y - ORD .. y + ORD
-  __device__ __forceinline__ float vx(int dx, int dy) const { return dy == 0 ? rx[ORD + dx] : sx[ORD + dy]; }
-  __device__ __forceinline__ float vy(int dx, int dy) const { return dy == 0 ? ry[ORD + dx] : sy[ORD + dy]; }
+struct VRingL {
+  const float (*vx_)[64];      // ring rows: v of this column's row r at vx_[slot(r)][lane]
+  const float (*vy_)[64];
+  int base, lane;              // ring slot of row x - ORD; this lane
+  static constexpr int R = 2 * ORD + 2;
+  __device__ __forceinline__ int slot(int dx) const { return (base + ORD + dx) % R; }
+  __device__ __forceinline__ float vx(int dx, int dy) const { return vx_[slot(dx)][lane + dy]; }
+  __device__ __forceinline__ float vy(int dx, int dy) const { return vy_[slot(dx)][lane + dy]; }
 };
 
 template <int VAR, bool VISCO, int ORD, int LC, bool FAST>
-__global__ __launch_bounds__(64) void k_fused2d(FusedArgs F, int xchunk, int nstrip, int i) {
-  constexpr int W = 64 - 2 * ORD;
+__global__ __launch_bounds__(128) void k_fused2d(FusedArgs F, int xchunk, int nstrip, int i) {
+  constexpr int W = 64 - 2 * ORD, R = 2 * ORD + 2;
+  __shared__ float ring_x[R][64], ring_y[R][64];
   const StepArgs &A = F.A;
   const int nx = A.nx, ny = A.ny;
   const int b = xcd_tile(blockIdx.x, gridDim.x, 1);
   const int strip = b % nstrip, chunk = b / nstrip;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, role = threadIdx.x >> 6;
   const int y = strip * W - ORD + lane;
   const bool col = y >= 0 && y < ny;
   const bool own = lane >= ORD && lane < 64 - ORD && y < ny;
   const int x0 = chunk * xchunk, x1 = min(nx, x0 + xchunk);
+  const int T = (x1 - x0) + 2 * ORD;                 // velocity rows x0 - ORD .. x1 + ORD - 1
   const int s = *A.step + i;
   if (blockIdx.x == 0 && !A.rec_after) record_receivers(A, s);
-  const FieldBuf Bvx(A.vx, A.ncell), Bvy(A.vy, A.ncell), Bsxx(A.sxx, A.ncell), Bsxy(A.sxy, A.ncell),
-      Bsyy(A.syy, A.ncell), Brho(A.rho, A.ncell);
-  VSE_G float *const po[4] = {F.psio[0], F.psio[1], F.psio[3], F.psio[4]};
-  // windows over the velocity rows xv: wxx[j] = sxx(xv - ORD + j), wxy[j] = sxy(xv - ORD + 1 + j)
-  float wxx[2 * ORD], wxy[2 * ORD];
-  const int xv0 = x0 - ORD;
-  const int i0 = xv0 * ny + y;
-#pragma unroll
-  for (int j = 0; j < 2 * ORD - 1; j++) {
-    wxx[j + 1] = Bsxx[i0 + (j - ORD) * ny];
-    wxy[j + 1] = Bsxy[i0 + (j - ORD + 1) * ny];
-  }
-  float rn0 = Brho[i0], rn1 = Brho[i0 + 1];
   unsigned v2 = 0;
-  VRing<ORD> R;
-  // v(n+1) of row xv in this lane's column (0 outside the grid, where no
-  // stress operand uses it); stored when this wavefront owns the cell
-  auto vel_row = [&](int xv, float &vxo, float &vyo) {
-    const int ic = xv * ny + y;
+  if (role == 0) {
+    const FieldBuf Bvx(A.vx, A.ncell), Bvy(A.vy, A.ncell), Bsxx(A.sxx, A.ncell), Bsxy(A.sxy, A.ncell),
+        Bsyy(A.syy, A.ncell), Brho(A.rho, A.ncell);
+    VSE_G float *const po[4] = {F.psio[0], F.psio[1], F.psio[3], F.psio[4]};
+    // windows over the velocity rows xv: wxx[j] = sxx(xv - ORD + j), wxy[j] = sxy(xv - ORD + 1 + j)
+    float wxx[2 * ORD], wxy[2 * ORD];
+    const int xv0 = x0 - ORD;
+    const int i0 = xv0 * ny + y;
 #pragma unroll
-    for (int j = 0; j < 2 * ORD - 1; j++) { wxx[j] = wxx[j + 1]; wxy[j] = wxy[j + 1]; }
-    wxx[2 * ORD - 1] = Bsxx[ic + (ORD - 1) * ny];
-    wxy[2 * ORD - 1] = Bsxy[ic + ORD * ny];
-    VelOps2<ORD> o;
-    o.vxn = Bvx[ic];
-    o.vyn = Bvy[ic];
-#pragma unroll
-    for (int c = 0; c < ORD; c++) {
-      o.xx_p[c] = wxx[ORD + c];
-      o.xx_m[c] = wxx[ORD - 1 - c];
-      o.xy_p[c] = c == 0 ? wxy[ORD - 1] : Bsxy[ic + c];
-      o.xy_m[c] = Bsxy[ic - c - 1];
-      o.yx_p[c] = wxy[ORD + c];
-      o.yx_m[c] = wxy[ORD - 1 - c];
-      o.yy_p[c] = Bsyy[ic + c + 1];
-      o.yy_m[c] = Bsyy[ic - c];
+    for (int j = 0; j < 2 * ORD - 1; j++) {
+      wxx[j + 1] = Bsxx[i0 + (j - ORD) * ny];
+      wxy[j + 1] = Bsxy[i0 + (j - ORD + 1) * ny];
     }
-    o.r00 = rn0;
-    o.r01 = rn1;
-    rn0 = Brho[ic + ny];
-    rn1 = Brho[ic + ny + 1];
-    o.r10 = rn0;
-    o.r11 = rn1;
-    vxo = 0.f;
-    vyo = 0.f;
-    if (col && xv >= 0 && xv < nx) {
-      const bool mine = own && xv >= x0 && xv < x1;
-      const unsigned u = velocity2d_update<VAR, VISCO, ORD, FAST>(A, xv, y, s, source_row<VAR>(A, xv), o, vxo, vyo,
-                                                                  po, mine);
-      if (mine) {
-        const long id = (long)xv * ny + y;
-        F.vo[0][id] = vxo;
-        F.vo[1][id] = vyo;
-        v2 = v2 > u ? v2 : u;
-      }
-    }
-  };
-#pragma unroll
-  for (int k = 0; k < 2 * ORD; k++) vel_row(xv0 + k, R.rx[k], R.ry[k]);
+    float rn0 = Brho[i0], rn1 = Brho[i0 + 1];
 #pragma unroll 1
-  for (int x = x0; x < x1; x++) {
-    vel_row(x + ORD, R.rx[2 * ORD], R.ry[2 * ORD]);
-    // row x's velocities of the neighbouring columns; every lane takes part
+    for (int t = 0; t <= T; t++) {
+      if (t < T) {
+        const int xv = xv0 + t;
+        const int ic = xv * ny + y;
 #pragma unroll
-    for (int d = -ORD; d <= ORD; d++) {
-      R.sx[ORD + d] = __shfl(R.rx[ORD], lane + d, 64);
-      R.sy[ORD + d] = __shfl(R.ry[ORD], lane + d, 64);
+        for (int j = 0; j < 2 * ORD - 1; j++) { wxx[j] = wxx[j + 1]; wxy[j] = wxy[j + 1]; }
+        wxx[2 * ORD - 1] = Bsxx[ic + (ORD - 1) * ny];
+        wxy[2 * ORD - 1] = Bsxy[ic + ORD * ny];
+        VelOps2<ORD> o;
+        o.vxn = Bvx[ic];
+        o.vyn = Bvy[ic];
+#pragma unroll
+        for (int c = 0; c < ORD; c++) {
+          o.xx_p[c] = wxx[ORD + c];
+          o.xx_m[c] = wxx[ORD - 1 - c];
+          o.xy_p[c] = c == 0 ? wxy[ORD - 1] : Bsxy[ic + c];
+          o.xy_m[c] = Bsxy[ic - c - 1];
+          o.yx_p[c] = wxy[ORD + c];
+          o.yx_m[c] = wxy[ORD - 1 - c];
+          o.yy_p[c] = Bsyy[ic + c + 1];
+          o.yy_m[c] = Bsyy[ic - c];
+        }
+        o.r00 = rn0;
+        o.r01 = rn1;
+        rn0 = Brho[ic + ny];
+        rn1 = Brho[ic + ny + 1];
+        o.r10 = rn0;
+        o.r11 = rn1;
+        // v(n+1) of row xv in this lane's column; 0 outside the grid, where no
+        // stress operand uses it
+        float vxo = 0.f, vyo = 0.f;
+        if (col && xv >= 0 && xv < nx) {
+          const bool mine = own && xv >= x0 && xv < x1;
+          const unsigned u = velocity2d_update<VAR, VISCO, ORD, FAST>(A, xv, y, s, source_row<VAR>(A, xv), o, vxo,
+                                                                      vyo, po, mine);
+          if (mine) {
+            const long id = (long)xv * ny + y;
+            F.vo[0][id] = vxo;
+            F.vo[1][id] = vyo;
+            v2 = v2 > u ? v2 : u;
+          }
+        }
+        ring_x[t % R][lane] = vxo;
+        ring_y[t % R][lane] = vyo;
+      }
+      __syncthreads();
     }
-    if (own) stress2d_core<VAR, VISCO, ORD, LC, FAST>(A, x, y, R, F.so[0], F.so[1], F.so[3]);
-#pragma unroll
-    for (int k = 0; k < 2 * ORD; k++) { R.rx[k] = R.rx[k + 1]; R.ry[k] = R.ry[k + 1]; }
+  } else {
+#pragma unroll 1
+    for (int t = 0; t <= T; t++) {
+      const int c = t - 1 - 2 * ORD;                   // stress row x0 + c, from ring rows c .. c + 2 ORD
+      if (c >= 0 && own) {
+        const VRingL<ORD> V{ring_x, ring_y, c % R, lane};
+        stress2d_core<VAR, VISCO, ORD, LC, FAST>(A, x0 + c, y, V, F.so[0], F.so[1], F.so[3]);
+      }
+      __syncthreads();
+    }
   }
   wave_max_to_slot(v2, A.v2slots + blockIdx.x);
 }
@@ -922,7 +932,7 @@ hipError_t vse_launch_fused2d(int variant, int visco, int ord, int math, const F
   if (xchunk < 1 || ord < 2 || ord > VSE_MAX_ORD) return hipErrorInvalidValue;
   const StepArgs &A = F.A;
   const int nstrip = (A.ny + (64 - 2 * ord) - 1) / (64 - 2 * ord);
-  const dim3 grid((unsigned)vse_fused2d_blocks(A.nx, A.ny, ord, xchunk)), block(64);
+  const dim3 grid((unsigned)vse_fused2d_blocks(A.nx, A.ny, ord, xchunk)), block(128);
   const bool fast = math == VSE_MATH_FAST;
   const bool l3 = visco && variant == VSE_VAR_WGSL && A.L == 3;
 #define VSE_F2(VAR, VISCO, FASTV)                                                                              \

@@ -258,8 +258,10 @@ def test_batch_fast_math_matches_solo_fast_runs():
         g.close()
 
 
-def test_batch_two_mechanisms_matches_solo_runs():
-    # L = 2 mechanisms: the batched and solo kernels' runtime mechanism loop
+@pytest.mark.parametrize("schedule", ["graph", "persistent"])
+def test_batch_two_mechanisms_matches_solo_runs(schedule):
+    # L = 2 mechanisms: the batched and solo kernels' runtime mechanism loop,
+    # under both batch schedules (the persistent kernel's runtime-L instance)
     from taichi_sim_amd.sim_support.attenuation import fit_constant_q
     steps = 150
     att = fit_constant_q(50.0, 50.0, 2.0, 8.0, n_sls=2)
@@ -270,9 +272,10 @@ def test_batch_two_mechanisms_matches_solo_runs():
         for law in range(2):
             sim.select_law(law)
             members.append((sim, law, VSESim.from_simulator(sim, variant="wgsl", att=att)))
-    print("batch of 4 members with 2 mechanisms, lockstep", flush=True)
+    print(f"batch of 4 members with 2 mechanisms, lockstep, {schedule} schedule", flush=True)
     batch = VSEBatch([m[2] for m in members])
     batch.set_group_bytes(0)
+    batch.set_schedule(schedule)
     batch.run(steps, check_every=100)
     batch.close()
     for sim, law, g in members:

@@ -307,10 +307,10 @@ def _state2d(g):
     return out
 
 
-@pytest.mark.parametrize("variant,visco,acc,math,channel", [
-    ("wgsl", True, 2, "strict", None), ("wgsl", True, 2, "fast", "vy"), ("wgsl", True, 4, "strict", None),
-    ("wgsl", False, 3, "strict", None), ("taichi", False, 2, "strict", None)])
-def test_2d_fused_step_bit_identical(variant, visco, acc, math, channel):
+@pytest.mark.parametrize("variant,visco,acc,math,channel,n_sls", [
+    ("wgsl", True, 2, "strict", None, 3), ("wgsl", True, 2, "fast", "vy", 3), ("wgsl", True, 4, "strict", None, 3),
+    ("wgsl", True, 2, "strict", None, 2), ("wgsl", False, 3, "strict", None, 3), ("taichi", False, 2, "strict", None, 3)])
+def test_2d_fused_step_bit_identical(variant, visco, acc, math, channel, n_sls):
     """The fused 2D schedule (one x-marching launch per step: velocity of step
     n, then stress of step n+1, y-neighbour velocities by cross-lane shuffles,
     sigma / v / velocity-pass psi double-buffered) ends bit for bit where the
@@ -322,6 +322,7 @@ def test_2d_fused_step_bit_identical(variant, visco, acc, math, channel):
     steps = 75
     cfg = workloads.visco_2d_array(n=(203, 171), steps=steps, n_elem=8, n_voids=2)
     cfg["simul_params"]["acc"] = acc
+    cfg["attenuation"]["n_sls"] = n_sls          # 2: the kernels' runtime mechanism loop
     sim = Simulator(cfg)
     att = attenuation_from_config(cfg)
     ref = VSESim.from_simulator(sim, variant=variant, visco=visco, att=att, channel=channel)
@@ -333,7 +334,7 @@ def test_2d_fused_step_bit_identical(variant, visco, acc, math, channel):
     g = VSESim.from_simulator(sim, variant=variant, visco=visco, att=att, channel=channel)
     g.set_math(math)
     g.set_schedule("fused")
-    print(f"fused 2D {variant} visco {visco} acc {acc} {math} channel {channel}", flush=True)
+    print(f"fused 2D {variant} visco {visco} acc {acc} {math} channel {channel} L {n_sls}", flush=True)
     for xc, rm, split in [(16, 1, (75,)), (8, 0, (1, 2, 72)), (7, 1, (74, 1)), (37, 0, (6, 69)), (1000, 1, (75,)),
                           (11, 1, (40, 35))]:
         assert sum(split) == steps

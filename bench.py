@@ -421,11 +421,12 @@ def main():
                            "steps": c2["steps"], "math": "strict", "kernel_ms": c2["kernel_ms"],
                            "roofline": c2["roofline"], "hbm_frac_step": c2["hbm_frac_step"],
                            "launch_shapes": c2["launch_shapes"], "workload": c2["config"]["workload"]}
-        a = measure_adjoint(1)
+        a = measure_adjoint(1, store_all=True)
         line["adjoint_over_forward"] = {"value": a["value"], "forward_ms": a["forward_ms_per_shot"],
                                         "checkpointed_forward_ms": a["checkpointed_forward_ms_per_shot"],
                                         "recompute_adjoint_ms": a["recompute_adjoint_ms_per_shot"],
                                         "segment": a["config"]["segment"], "shots": 1, "steps": a["steps"],
+                                        "every_state_stored": a["every_state_stored"],
                                         "gradient_accumulation": a["gradient_accumulation"],
                                         "workload": a["config"]["workload"]}
     emit(line)

@@ -631,10 +631,14 @@ def measure_adjoint(n_shots, fd_dirs=0, store_all=False, schedules=False):
     for k in range(n_shots):
         sim.select_law(k)
         g.load_sources(sim)
-        # the plain forward of the same shot at the starting model
+        # the plain forward of the same shot at the starting model: the median of
+        # three runs (one run alone varies by a few percent between calls)
         g.set_material(*start)
-        g.reset()
-        t_fwd.append(g.run(steps, check_every=500, timed=True))
+        runs = []
+        for _ in range(3):
+            g.reset()
+            runs.append(g.run(steps, check_every=500, timed=True))
+        t_fwd.append(float(np.median(runs)))
         # checkpointed forward + FP64 misfit + recompute/adjoint
         eng.set_model(*start)
         eng.forward(steps, check_every=500)
